@@ -1,0 +1,193 @@
+"""Benchmark of the per-step EM-PIC hot path on one GPU.
+
+Workload: the production-size single run (1.6-cell S-band gun, dr = dz = 0.1 mm,
+1051 x 421 grid lines, 1M macroparticles / 1 nC, TM+TE fields, Poisson
+correction every 10 steps to 1e-8, one screen at z = 10 cm), FP64, synthetic
+field and bunch contents (spiffe_amd.workloads).  A step is one pass of the
+loop body of src/integrate.c:134-194: J reset, NGP deposition, Poisson
+correction (every 10th step), FDTD, push with loss/screen handling.
+
+Prints ONE JSON line (rank 0).  `value` is particle-pushes/s over the timed
+steps with all state resident in HBM; cell-updates/s is reported beside it.
+The `roofline` object is for the kernel that takes the most time per step,
+timed with HIP events on the library's own stream; `cpu_baseline` is the
+CPU restatement (oracle/) timed on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-pushes/s + cell-updates/s (% HBM roofline); scan runs/hr + p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per unit (FP64 = 8 B, flags 2 B)
+B_CELL_TMTE = 146.0      # TM+TE cell-update: reads Ez Er Bphi Jz Jr Ephi Br Bz Jphi + flag, writes 9 arrays
+B_PUSH = 196.0           # particle-push: 12 doubles + status read, 12 doubles written
+B_SOR = 22.0             # SOR point of one half-sweep: Psi rhs flag read, half of points written
+
+
+class HipTimer:
+    """HIP events recorded on an arbitrary stream (the library's), via the HIP runtime."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def event(self):
+        e = ctypes.c_void_p()
+        if self.hip.hipEventCreate(ctypes.byref(e)) != 0:
+            raise RuntimeError("hipEventCreate failed")
+        return e
+
+    def time_ms(self, stream, fn, reps):
+        a, b = self.event(), self.event()
+        self.hip.hipEventRecord(a, stream)
+        for _ in range(reps):
+            fn()
+        self.hip.hipEventRecord(b, stream)
+        self.hip.hipEventSynchronize(b)
+        ms = ctypes.c_float()
+        self.hip.hipEventElapsedTime(ctypes.byref(ms), a, b)
+        self.hip.hipEventDestroy(a)
+        self.hip.hipEventDestroy(b)
+        return ms.value
+
+
+def build_run(ctx, n_particles):
+    from spiffe_amd import workloads as W
+    from spiffe_amd.device import DeviceRun
+    g = W.gun_grid()
+    f = W.gun_flags(g)
+    run = DeviceRun(ctx, g, f, max_particles=n_particles, screens=W.GUN_SCREENS, compat=False)
+    for k, v in W.gun_fields(g).items():
+        run.upload(k, v)
+    run.append(W.gun_bunch(g, n_particles))
+    run.setup_poisson_correction(**W.GUN_POISSON)
+    return g, f, run
+
+
+def cpu_baseline(g, f, n_particles, budget_s):
+    """The restatement on one host core, same workload, bounded sample of steps."""
+    from oracle.oracle import OracleSim
+    from spiffe_amd import workloads as W
+    o = OracleSim(g, f, max_particles=n_particles)
+    for k, v in W.gun_fields(g).items():
+        o.set_field(k, v)
+    o.append(W.gun_bunch(g, n_particles))
+    o.setup_poisson_correction(**W.GUN_POISSON)
+    steps, pushes = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        pushes += int(o.b.np)
+        o.step(screens=W.GUN_SCREENS, compat=False)
+        steps += 1
+        if time.perf_counter() - t0 > budget_s and steps >= 11:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": pushes / dt, "unit": "particle-pushes/s", "cores": 1, "kind": "port",
+            "sample": "%d whole steps (incl. %d Poisson corrections) of the same 1M-particle gun workload, "
+                      "%.1f s, serial restatement built -O2 -ffp-contract=off" % (steps, (steps + 9) // 10, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--particles", type=int, default=1_000_000)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: independent gun runs shard across GPUs with no exchange, "
+              "and this build measures the single-GPU path only (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from spiffe_amd.device import Context
+    ctx = Context(0)
+    g, f, run = build_run(ctx, args.particles)
+    stream = ctx.stream()
+    timer = HipTimer()
+    cells = (g.nz - 1) * (g.nr - 1)
+
+    for _ in range(args.warmup):
+        run.step()
+    ctx.synchronize()
+    np0, _ = run.count()
+    # timed region: K whole steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run.step()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    np1, _ = run.count()
+    pushes = 0.5 * (np0 + np1) * args.steps
+    ms_per_step = 1e3 * wall / args.steps
+
+    # per-phase kernel timing with HIP events on the library stream
+    reps = 50
+    t_fdtd = timer.time_ms(stream, run.advance_field_solutions, reps) / reps
+    npk, _ = run.count()
+    t_push = timer.time_ms(stream, run.advance_particles, reps) / reps
+    t_dep = timer.time_ms(stream, lambda: (run.zero_current_arrays(), run.add_particle_currents()), reps) / reps
+    run.set_step(0)
+    t_pc0 = time.perf_counter()
+    it = run.add_poisson_correction(True)
+    t_pc = (time.perf_counter() - t_pc0) * 1e3
+    per_step = {"fdtd_ms": t_fdtd, "push_ms": t_push, "deposit_ms": t_dep, "poisson_ms_per_step": t_pc / 10}
+    phases = {"fdtd": t_fdtd, "push": t_push, "deposit": t_dep, "poisson": t_pc / 10}
+    dominant = max(phases, key=phases.get)
+    if dominant == "fdtd":
+        ach = cells * B_CELL_TMTE / (t_fdtd * 1e-3) / 1e9
+        rl = {"bound": "hbm", "kernel": "fdtd_kernel", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+              "frac": ach / HBM_PEAK_GBS, "traffic": None,
+              "algorithmic_bytes_per_launch": cells * B_CELL_TMTE}
+    elif dominant == "push":
+        ach = npk * B_PUSH / (t_push * 1e-3) / 1e9
+        rl = {"bound": "hbm", "kernel": "push phase (push_kernel + event compaction)", "achieved": ach,
+              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+              "algorithmic_bytes_per_launch": npk * B_PUSH}
+    else:
+        ach = None
+        rl = {"bound": "hbm", "kernel": dominant, "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+              "frac": None, "traffic": None}
+    # FDTD roofline is always reported beside the dominant one
+    fdtd_gbs = cells * B_CELL_TMTE / (t_fdtd * 1e-3) / 1e9
+    push_gbs = npk * B_PUSH / (t_push * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(g, f, args.particles, args.cpu_budget)
+
+    out = {
+        "metric": METRIC, "value": pushes / wall, "unit": "particle-pushes/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "1.6-cell S-band gun, dr=dz=0.1 mm (1051x421 lines), 1M macroparticles 1 nC, "
+                               "TM+TE, Poisson every 10 steps to 1e-8, screen z=10 cm",
+                   "grid_lines": [g.nz, g.nr], "cells": cells, "particles": args.particles, "dt": g.dt},
+        "cell_updates_per_s": cells * args.steps / wall,
+        "phase_ms": per_step, "poisson_iterations": it,
+        "fdtd_gbs": fdtd_gbs, "fdtd_frac": fdtd_gbs / HBM_PEAK_GBS,
+        "push_gbs": push_gbs, "push_frac": push_gbs / HBM_PEAK_GBS,
+        "roofline": rl, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    run.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,213 @@
+/*
+ * spiffe_amd.h -- C ABI of the MI355X-native spiffe time-step library.
+ *
+ * The reference calls its per-step physics as plain C functions on
+ * caller-owned FIELDS/BEAM structs (src/spiffe.h:474-561).  This library owns
+ * the field and particle state in HBM instead, behind a context handle; the
+ * host keeps the deck, the emission models and all file I/O.  Every entry
+ * point returns 0 on success or a nonzero SPF_E* code; spf_last_error() gives
+ * the message.  Calls are stream-ordered on the context's HIP stream and do
+ * not block unless documented.  A context is used from one host thread and
+ * holds any number of runs.
+ *
+ * Host arrays crossing this boundary use the reference's layout: a 2-D field
+ * is (nz+1) x (nr+1) doubles with element [iz][ir] at iz*(nr+1)+ir
+ * (src/geometry.c:78-107).  On the device the library stores fields
+ * z-contiguous, particles as structure-of-arrays.
+ */
+#ifndef SPIFFE_AMD_H
+#define SPIFFE_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---- */
+#define SPF_OK 0
+#define SPF_EINVAL 1      /* bad argument */
+#define SPF_EHIP 2        /* HIP runtime error */
+#define SPF_ENOMEM 3      /* device allocation failed */
+#define SPF_EUNSUPPORTED 4 /* mode not implemented on the device path */
+#define SPF_ECAPACITY 5   /* particle or event capacity exceeded */
+#define SPF_ENOCONV 6     /* Poisson solve hit its iteration limit (reference returns -1) */
+
+/* ---- mode bits: same values as src/spiffe.h:50-56 ---- */
+#define SPF_TM_FIELDS 1u
+#define SPF_TE_FIELDS 2u
+#define SPF_SPACE_CHARGE 4u
+#define SPF_RADIAL_INTERP 8u
+#define SPF_LONGIT_INTERP 16u
+#define SPF_RADIAL_SMEAR 32u
+#define SPF_LONGIT_SMEAR 64u
+
+/* ---- metal flag bits: same values as src/spiffe.h:90-100 ---- */
+#define SPF_FL_IS_SURFACE 1
+#define SPF_FL_IS_INSIDE 2
+#define SPF_FL_EZ_ZERO 4
+#define SPF_FL_ER_ZERO 8
+#define SPF_FL_IS_VACUUM 16
+#define SPF_FL_BPHI_ZERO 32
+#define SPF_FL_EPHI_ZERO 64
+#define SPF_FL_BZ_ZERO 128
+#define SPF_FL_BR_ZERO 256
+#define SPF_FL_LAST_ACTIVE_PT 512
+
+#define SPF_DIRICHLET 0
+#define SPF_NEUMANN 1
+
+/* particle status bits, src/spiffe.h:219-220 */
+#define SPF_PART_ACTIVE 1u
+#define SPF_PART_LOST 2u
+
+/* ---- field identifiers for upload/download ---- */
+enum spf_field {
+  SPF_EZ = 0, SPF_ER, SPF_BPHI, SPF_BPHI2,
+  SPF_EPHI, SPF_BR, SPF_BR2, SPF_BZ, SPF_BZ2,
+  SPF_JZ, SPF_JR, SPF_JPHI,
+  SPF_PSI, SPF_Q,
+  SPF_EZ_IMPOSED, SPF_ER_IMPOSED, SPF_PSI_IMPOSED,
+  SPF_BZ_IMPOSED, SPF_BR_IMPOSED,
+  SPF_N_FIELDS
+};
+
+typedef struct spf_ctx spf_ctx;
+
+/* Geometry, modes and run-time options of one run.  Replaces what
+ * process_geometry_definition (src/geometry.c:21-369) and the integrate
+ * namelist (src/integrate.c:62-108) leave in FIELDS. */
+typedef struct {
+  int32_t nz, nr;                 /* grid lines, >= 3 each */
+  double zmin, zmax, rmax;
+  double dz, dr;                  /* (zmax-zmin)/(nz-1), rmax/(nr-1), computed by the host */
+  int32_t lower_bc, upper_bc, left_bc, right_bc;
+  uint32_t modes;                 /* SPF_* mode bits */
+  double dt, start_time;
+  double epsilon_o, mu_o, c_mks;  /* physical constants used by the host */
+  double QoMC, stiffness;         /* beam charge/(m c) and mass/me */
+  int16_t turn_off[6];            /* Ez, Er, Ephi, Bz, Br, Bphi, src/spiffe.h:74-75 */
+  double constant_fields[6];      /* Ez, Er, Ephi, Bz, Br, Bphi, src/spiffe.h:118-120 */
+  int32_t has_solenoid;           /* nonzero: BzImposed/BrImposed are gathered */
+  const int16_t *metal_flags;     /* host (nz+1)*(nr+1), [iz][ir], flags of src/geometry.c:319-362 */
+  int64_t max_particles;          /* device particle capacity */
+  int32_t n_screens;              /* <= 24 */
+  const double *screen_z;         /* screen z positions */
+  const int32_t *screen_direction;/* +1 forward, -1 backward */
+  int32_t compat;                 /* 1: reproduce compaction and screen-momentum defects of the reference */
+  int64_t screen_row_capacity;    /* device ring size for screen rows (0: default) */
+  int64_t lost_row_capacity;      /* device ring size for lost rows (0: default) */
+} spf_run_desc;
+
+/* Particle SoA on the host, src/spiffe.h:201-221 (status is 32-bit here). */
+typedef struct {
+  double *Q, *z, *r, *pz, *pr, *r_pphi, *vz, *vr, *vphi;
+  double *r0, *z0, *t0, *rmax, *rmin, *zmax, *zmin;
+  int32_t *particleID;
+  int16_t *generation;
+  uint32_t *status;
+} spf_particles;
+
+/* One screen-crossing row: the columns of src/screens.c:82-98 plus bookkeeping. */
+typedef struct {
+  double r, pz, pr, pphi, t, q, r0, t0;
+  double rmin, rmax, zmin, zmax;
+  int64_t ip;          /* particle index during that step */
+  int64_t step;        /* time_step of the row */
+  int32_t particleID;
+  int16_t generation;
+  int16_t screen;
+} spf_screen_row;
+
+/* One lost-particle row: the columns of src/advance_particles.c:581-599. */
+typedef struct {
+  double t;
+  float rLost, zLost, pz, pr, r0, z0, q;
+  int32_t particleID;
+  int64_t ip;
+  int64_t step;
+  int16_t generation;
+  int16_t directionCode;
+  int32_t pad;
+} spf_lost_row;
+
+/* poisson_correction namelist, src/poisson_correction.nl and defaults at
+ * src/poisson_correction.c:37-46 */
+typedef struct {
+  double start_time;
+  int64_t step_interval;
+  double accuracy;
+  double error_charge_threshold;
+  int64_t maximum_iterations;
+  int64_t guess_code;   /* 0 none (warm start), 3 zero; 1 and 2 unsupported on device */
+} spf_poisson_params;
+
+/* ---- context ---- */
+int spf_ctx_create(int32_t device, spf_ctx **out);
+int spf_ctx_destroy(spf_ctx *ctx);
+const char *spf_last_error(spf_ctx *ctx);
+int spf_synchronize(spf_ctx *ctx);
+/* HIP stream (hipStream_t) the context launches on, for callers that time or order work around it. */
+void *spf_stream(spf_ctx *ctx);
+
+/* ---- run lifecycle; replaces the allocations of src/geometry.c:78-107 ---- */
+int spf_run_create(spf_ctx *ctx, const spf_run_desc *desc, int32_t *run_out);
+int spf_run_destroy(spf_ctx *ctx, int32_t run);
+/* bytes of HBM a run with this description would hold (admission control) */
+int64_t spf_run_footprint(const spf_run_desc *desc);
+
+/* ---- fields ---- */
+int spf_field_upload(spf_ctx *ctx, int32_t run, int32_t field, const double *host);
+int spf_field_download(spf_ctx *ctx, int32_t run, int32_t field, double *host);
+/* time_step and time = time_step*dt + start_time for the next step, src/integrate.c:138 */
+int spf_set_step(spf_ctx *ctx, int32_t run, int64_t time_step);
+int spf_get_step(spf_ctx *ctx, int32_t run, int64_t *time_step, double *time);
+
+/* FDTD leapfrog of TM and TE fields; replaces advance_field_solutions, src/integrate.c:277-541 */
+int spf_fields_advance(spf_ctx *ctx, int32_t run);
+/* J <- 0; replaces zero_current_arrays, src/integrate.c:210-241 */
+int spf_currents_reset(spf_ctx *ctx, int32_t run);
+/* J <- J*factor; replaces multiply_current_arrays, src/integrate.c:243-275 */
+int spf_currents_scale(spf_ctx *ctx, int32_t run, double factor);
+/* adds J along one grid line (direction 0: Jz[i][pos], 1: Jr[pos][i], 2: Jphi[i][pos],
+ * i = istart..iend); the line loops of add_antenna_excitation, src/antenna.c:137-154 */
+int spf_currents_add_line(spf_ctx *ctx, int32_t run, int32_t direction, int64_t iposition,
+                          int64_t istart, int64_t iend, double J);
+/* NGP current deposition; replaces add_particle_currents, src/space_charge.c:34-295 */
+int spf_currents_deposit(spf_ctx *ctx, int32_t run);
+
+/* ---- particles ---- */
+/* appends n particles after the current ones (emission sink, src/cathode.c:427-456) */
+int spf_particles_append(spf_ctx *ctx, int32_t run, int64_t n, const spf_particles *host);
+/* synchronizing: current np and npActive */
+int spf_particles_count(spf_ctx *ctx, int32_t run, int64_t *np, int64_t *np_active);
+/* synchronizing: copies up to max particles into host arrays (any pointer may be NULL) */
+int spf_particles_download(spf_ctx *ctx, int32_t run, const spf_particles *host, int64_t max, int64_t *n);
+/* push, gather, loss, screens, compaction; replaces advance_particles,
+ * src/advance_particles.c:67-575 (zForcesStart as in src/integrate.c:186-188) */
+int spf_particles_push(spf_ctx *ctx, int32_t run, double zForcesStart);
+/* synchronizing: copies out and clears the accumulated screen and lost rows */
+int spf_events_drain(spf_ctx *ctx, int32_t run, spf_screen_row *rows, int64_t max_rows, int64_t *n_rows,
+                     spf_lost_row *lost, int64_t max_lost, int64_t *n_lost);
+
+/* ---- Poisson ---- */
+int spf_poisson_setup(spf_ctx *ctx, int32_t run, const spf_poisson_params *p);
+/* replaces add_poisson_correction, src/poisson_correction.c:73-454; synchronizing
+ * when a solve happens.  *iterations: solve result, -1 not converged, -2 no solve. */
+int spf_poisson_correct(spf_ctx *ctx, int32_t run, int32_t space_charge, int64_t *iterations);
+/* replaces solve_poisson_cyl, src/poisson.c:17-130, on the run's Psi (which=0,
+ * rhs = Q) or PsiImposed (which=1, no rhs).  Synchronizing. */
+int spf_poisson_solve(spf_ctx *ctx, int32_t run, int32_t which, double accuracy, int64_t max_iterations,
+                      int64_t *iterations);
+/* replaces compute_imposed_Efield, src/poisson_correction.c:456-562 */
+int spf_imposed_efield(spf_ctx *ctx, int32_t run, int32_t initialize, double ramp_time, double flat_top_time);
+
+/* ---- one whole step of the loop body, src/integrate.c:134-194 ----
+ * time <- step*dt+start; J reset; deposit (if SPF_SPACE_CHARGE); n_lines line
+ * currents; Poisson correction (if set up); FDTD; push; step++. */
+int spf_step(spf_ctx *ctx, int32_t run, int32_t n_lines, const int32_t *line_dir, const int64_t *line_pos,
+             const int64_t *line_start, const int64_t *line_end, const double *line_J);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,280 @@
+"""Python handle on the CPU restatement (oracle/libspf_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: tests/, the smoke check and bench.py's CPU-baseline
+leg use this as the checker.  The product path (spiffe_amd) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libspf_oracle.so")
+
+_d, _dp = ctypes.c_double, ctypes.POINTER(ctypes.c_double)
+_i16p = ctypes.POINTER(ctypes.c_int16)
+
+
+class OFields(ctypes.Structure):
+    _fields_ = [("nz", ctypes.c_int32), ("nr", ctypes.c_int32),
+                ("zmin", _d), ("zmax", _d), ("rmax", _d), ("dz", _d), ("dr", _d),
+                ("lower_bc", ctypes.c_int32), ("upper_bc", ctypes.c_int32),
+                ("right_bc", ctypes.c_int32), ("left_bc", ctypes.c_int32),
+                ("start_time", _d), ("time", _d), ("dti", _d),
+                ("time_step", ctypes.c_int64), ("modes", ctypes.c_uint32)] + \
+               [(n, ctypes.c_int16) for n in ["turnOffEz", "turnOffEr", "turnOffEphi", "turnOffBz", "turnOffBr",
+                                               "turnOffBphi"]] + \
+               [(n, _d) for n in ["constantEz", "constantEr", "constantEphi", "constantBz", "constantBr",
+                                  "constantBphi", "epsilon_o", "mu_o", "c_mks"]] + \
+               [(n, _dp) for n in ["Ez", "Jz", "Er", "Jr", "Bphi", "Bphi2", "Ephi", "Jphi", "Br", "Br2", "Bz",
+                                   "Bz2"]] + \
+               [("metal_flags", _i16p)] + \
+               [(n, _dp) for n in ["EzImposed", "ErImposed", "PsiImposed", "BzImposed", "BrImposed"]] + \
+               [("imposedEFieldRampTime", _d), ("imposedEFieldFlatTopTime", _d), ("Psi", _dp), ("Q", _dp)]
+
+
+PDOUBLES = ["Q", "z", "r", "pz", "pr", "r_pphi", "vz", "vr", "vphi", "r0", "z0", "t0", "rmax", "rmin", "zmax",
+            "zmin"]
+
+
+class OBeam(ctypes.Structure):
+    _fields_ = [("np", ctypes.c_int64), ("npActive", ctypes.c_int64), ("max_np", ctypes.c_int64),
+                ("QoMC", _d), ("stiffness", _d)] + [(n, _dp) for n in PDOUBLES] + \
+               [("particleID", ctypes.POINTER(ctypes.c_int32)), ("generation", _i16p),
+                ("status", ctypes.POINTER(ctypes.c_uint32))]
+
+
+class OScreen(ctypes.Structure):
+    _fields_ = [("z_position", _d), ("direction", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class OScreenRow(ctypes.Structure):
+    _fields_ = [(n, _d) for n in ["r", "pz", "pr", "pphi", "t", "q", "r0", "t0", "rmin", "rmax", "zmin",
+                                  "zmax"]] + \
+               [("ip", ctypes.c_int64), ("particleID", ctypes.c_int32), ("generation", ctypes.c_int16),
+                ("screen", ctypes.c_int16)]
+
+
+class OLostRow(ctypes.Structure):
+    _fields_ = [("t", _d)] + [(n, ctypes.c_float) for n in ["rLost", "zLost", "pz", "pr", "r0", "z0", "q"]] + \
+               [("particleID", ctypes.c_int32), ("ip", ctypes.c_int64), ("generation", ctypes.c_int16),
+                ("directionCode", ctypes.c_int16), ("pad", ctypes.c_int32)]
+
+
+class OPoisson(ctypes.Structure):
+    _fields_ = [("start_time", _d), ("step_interval", ctypes.c_int64), ("accuracy", _d),
+                ("error_charge_threshold", _d), ("maximum_iterations", ctypes.c_int64),
+                ("guess_code", ctypes.c_int64), ("pc_start_step", ctypes.c_int64),
+                ("last_iterations", ctypes.c_int64), ("Qreal", _d), ("Qerror", _d), ("Qsurface", _d)]
+
+
+SCREEN_ROW_DTYPE = np.dtype(OScreenRow)
+LOST_ROW_DTYPE = np.dtype(OLostRow)
+FIELD_NAMES = ["Ez", "Jz", "Er", "Jr", "Bphi", "Bphi2", "Ephi", "Jphi", "Br", "Br2", "Bz", "Bz2",
+               "EzImposed", "ErImposed", "PsiImposed", "BzImposed", "BrImposed", "Psi", "Q"]
+
+_lib = None
+
+
+def build():
+    """Compiles the restatement with the committed Makefile (gcc, -ffp-contract=off)."""
+    r = subprocess.run(["make", "-C", HERE, "-s"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    return LIB
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        lib = ctypes.CDLL(LIB)
+        P = ctypes.POINTER
+        lib.oracle_calculate_flags.argtypes = [_i16p, ctypes.c_int32, ctypes.c_int32]
+        lib.oracle_advance_field_solutions.argtypes = [P(OFields)]
+        lib.oracle_zero_current_arrays.argtypes = [P(OFields)]
+        lib.oracle_multiply_current_arrays.argtypes = [P(OFields), _d]
+        lib.oracle_add_line_current.argtypes = [P(OFields), ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,
+                                                ctypes.c_int64, _d]
+        lib.oracle_add_particle_currents.argtypes = [P(OFields), P(OBeam)]
+        lib.oracle_check_loss_code.argtypes = [ctypes.c_int64, ctypes.c_int64, P(OFields)]
+        lib.oracle_check_loss_code.restype = ctypes.c_short
+        lib.oracle_interpolate_fields.argtypes = [_dp] * 6 + [_d, _d, P(OFields)]
+        lib.oracle_find_loss_coordinates.argtypes = [_dp, _dp, P(ctypes.c_short), _d, _d, _d, _d, P(OFields)]
+        lib.oracle_advance_particles.argtypes = [P(OFields), P(OBeam), P(OScreen), ctypes.c_int32, _d,
+                                                 ctypes.c_int32, P(OScreenRow), ctypes.c_int64,
+                                                 P(ctypes.c_int64), P(OLostRow), ctypes.c_int64,
+                                                 P(ctypes.c_int64)]
+        lib.oracle_solve_poisson_cyl.argtypes = [_dp, _dp, _i16p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
+                                                 ctypes.c_int64, _d, _d, _d, ctypes.c_int64, _dp]
+        lib.oracle_solve_poisson_cyl.restype = ctypes.c_int64
+        lib.oracle_poisson_omega_schedule.argtypes = [ctypes.c_int64, ctypes.c_int64, _d, _d, ctypes.c_int64, _dp]
+        lib.oracle_add_poisson_correction.argtypes = [P(OFields), P(OBeam), ctypes.c_int32, P(OPoisson)]
+        lib.oracle_compute_imposed_efield.argtypes = [P(OFields), ctypes.c_int32, P(OPoisson), _dp]
+        lib.oracle_antenna_current.argtypes = [_dp, _dp, ctypes.c_int64, P(ctypes.c_int64), _d, _d, _d, _d,
+                                               P(ctypes.c_int32)]
+        lib.oracle_antenna_current.restype = _d
+        _lib = lib
+    return _lib
+
+
+def derive_flags(marks):
+    """The C restatement of src/geometry.c:319-362 applied to a copy of `marks`."""
+    f = np.ascontiguousarray(marks, dtype=np.int16).copy()
+    load().oracle_calculate_flags(f.ctypes.data_as(_i16p), f.shape[0] - 1, f.shape[1] - 1)
+    return f
+
+
+class OracleSim:
+    """FIELDS + BEAM state of the restatement in numpy arrays ([iz][ir] layout)."""
+
+    def __init__(self, grid, flags, max_particles=0, QoMC=None, stiffness=1.0, has_solenoid=False, constants=None):
+        from spiffe_amd import constants as K   # the same constants the product passes to the device
+        self.lib = load()
+        self.grid = grid
+        shape = grid.shape
+        self.arr = {n: np.zeros(shape, dtype=np.float64) for n in FIELD_NAMES}
+        self.flags = np.ascontiguousarray(flags, dtype=np.int16).copy()
+        f = OFields()
+        f.nz, f.nr = grid.nz, grid.nr
+        f.zmin, f.zmax, f.rmax, f.dz, f.dr = grid.zmin, grid.zmax, grid.rmax, grid.dz, grid.dr
+        f.lower_bc, f.upper_bc, f.right_bc, f.left_bc = grid.lower_bc, grid.upper_bc, grid.right_bc, grid.left_bc
+        f.start_time, f.time, f.dti, f.time_step, f.modes = grid.start_time, 0.0, grid.dt, 0, grid.modes
+        for k, n in enumerate(["turnOffEz", "turnOffEr", "turnOffEphi", "turnOffBz", "turnOffBr", "turnOffBphi"]):
+            setattr(f, n, int(grid.turn_off[k]))
+        for k, n in enumerate(["constantEz", "constantEr", "constantEphi", "constantBz", "constantBr",
+                               "constantBphi"]):
+            setattr(f, n, float(grid.constant_fields[k]))
+        f.epsilon_o, f.mu_o, f.c_mks = K.epsilon_o, K.mu_o, K.c_mks
+        for n in FIELD_NAMES:
+            if n in ("BzImposed", "BrImposed") and not has_solenoid:
+                setattr(f, n, ctypes.cast(None, _dp))
+            else:
+                setattr(f, n, self.arr[n].ctypes.data_as(_dp))
+        f.metal_flags = self.flags.ctypes.data_as(_i16p)
+        self.f = f
+        cap = max(int(max_particles), 1)
+        self.p = {n: np.zeros(cap, dtype=np.float64) for n in PDOUBLES}
+        self.p["particleID"] = np.zeros(cap, dtype=np.int32)
+        self.p["generation"] = np.zeros(cap, dtype=np.int16)
+        self.p["status"] = np.zeros(cap, dtype=np.uint32)
+        b = OBeam()
+        b.np, b.npActive, b.max_np = 0, 0, cap
+        b.QoMC = K.QoMC_electron if QoMC is None else QoMC
+        b.stiffness = stiffness
+        for n in PDOUBLES:
+            setattr(b, n, self.p[n].ctypes.data_as(_dp))
+        b.particleID = self.p["particleID"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        b.generation = self.p["generation"].ctypes.data_as(_i16p)
+        b.status = self.p["status"].ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        self.b = b
+        self.pc = None
+        self.last_field_factor = ctypes.c_double(-1.0)
+
+    # ---- state ----
+    def set_field(self, name, a):
+        self.arr[name][...] = a
+
+    def field(self, name):
+        return self.arr[name].copy()
+
+    def append(self, parts):
+        n = len(parts["z"])
+        s = self.b.np
+        if s + n > self.b.max_np:
+            raise ValueError("oracle capacity exceeded")
+        for k in PDOUBLES + ["particleID", "generation", "status"]:
+            self.p[k][s:s + n] = parts[k]
+        self.b.np = s + n
+        self.b.npActive += int(np.count_nonzero(np.asarray(parts["status"]) & 1))
+
+    def particles(self):
+        n = self.b.np
+        return {k: v[:n].copy() for k, v in self.p.items()}
+
+    # ---- the restated functions ----
+    def advance_field_solutions(self):
+        self.lib.oracle_advance_field_solutions(ctypes.byref(self.f))
+
+    def zero_current_arrays(self):
+        self.lib.oracle_zero_current_arrays(ctypes.byref(self.f))
+
+    def multiply_current_arrays(self, factor):
+        self.lib.oracle_multiply_current_arrays(ctypes.byref(self.f), factor)
+
+    def add_line_current(self, direction, iposition, istart, iend, J):
+        self.lib.oracle_add_line_current(ctypes.byref(self.f), direction, iposition, istart, iend, J)
+
+    def add_particle_currents(self):
+        self.lib.oracle_add_particle_currents(ctypes.byref(self.f), ctypes.byref(self.b))
+
+    def interpolate(self, z, r):
+        out = [ctypes.c_double() for _ in range(6)]
+        self.lib.oracle_interpolate_fields(*[ctypes.byref(o) for o in out], z, r, ctypes.byref(self.f))
+        return tuple(o.value for o in out)   # Ez, Er, Ephi, Bz, Br, Bphi
+
+    def advance_particles(self, screens=(), compat=True, z_forces_start=-1.0):
+        ns = len(screens)
+        scr = (OScreen * max(ns, 1))()
+        for k, (z, d) in enumerate(screens):
+            scr[k].z_position, scr[k].direction = z, d
+        cap_rows = max(ns * max(self.b.np, 1), 1)
+        rows = np.zeros(cap_rows, dtype=SCREEN_ROW_DTYPE)
+        lost = np.zeros(max(self.b.np, 1), dtype=LOST_ROW_DTYPE)
+        nr_, nl_ = ctypes.c_int64(), ctypes.c_int64()
+        rc = self.lib.oracle_advance_particles(ctypes.byref(self.f), ctypes.byref(self.b), scr, ns, z_forces_start,
+                                               1 if compat else 0,
+                                               rows.ctypes.data_as(ctypes.POINTER(OScreenRow)), cap_rows,
+                                               ctypes.byref(nr_), lost.ctypes.data_as(ctypes.POINTER(OLostRow)),
+                                               len(lost), ctypes.byref(nl_))
+        if rc:
+            raise RuntimeError("oracle row buffers overflowed")
+        return rows[:nr_.value], lost[:nl_.value]
+
+    def setup_poisson_correction(self, step_interval, accuracy=1e-6, maximum_iterations=1000, start_time=0.0,
+                                 error_charge_threshold=0.0, guess_code=0):
+        pc = OPoisson()
+        pc.start_time, pc.step_interval, pc.accuracy = start_time, step_interval, accuracy
+        pc.error_charge_threshold, pc.maximum_iterations, pc.guess_code = error_charge_threshold, \
+            maximum_iterations, guess_code
+        pc.pc_start_step = -1
+        self.pc = pc
+
+    def add_poisson_correction(self, space_charge):
+        rc = self.lib.oracle_add_poisson_correction(ctypes.byref(self.f), ctypes.byref(self.b),
+                                                    1 if space_charge else 0, ctypes.byref(self.pc))
+        if rc:
+            raise RuntimeError("oracle Poisson correction failed")
+        return self.pc.last_iterations
+
+    def solve_poisson_cyl(self, Psi_name, rhs_name, accuracy, maximum_iterations):
+        res = ctypes.c_double()
+        rhs = self.arr[rhs_name].ctypes.data_as(_dp) if rhs_name else ctypes.cast(None, _dp)
+        return self.lib.oracle_solve_poisson_cyl(self.arr[Psi_name].ctypes.data_as(_dp), rhs,
+                                                 self.flags.ctypes.data_as(_i16p), self.f.left_bc,
+                                                 self.f.right_bc, self.grid.nz, self.grid.nr, self.grid.dz,
+                                                 self.grid.dr, accuracy, maximum_iterations, ctypes.byref(res)), \
+            res.value
+
+    def compute_imposed_Efield(self, initialize, ramp_time=0.0, flat_top_time=0.0):
+        self.f.imposedEFieldRampTime, self.f.imposedEFieldFlatTopTime = ramp_time, flat_top_time
+        self.lib.oracle_compute_imposed_efield(ctypes.byref(self.f), 1 if initialize else 0, ctypes.byref(self.pc),
+                                               ctypes.byref(self.last_field_factor))
+
+    def step(self, lines=(), screens=(), compat=True):
+        """Loop body of src/integrate.c:134-194 (after emission), same order as the device's spf_step."""
+        f = self.f
+        f.time = f.time_step * f.dti + f.start_time
+        self.zero_current_arrays()
+        if f.modes & 4:
+            self.add_particle_currents()
+        for (d, pos, s, e, J) in lines:
+            self.add_line_current(d, pos, s, e, J)
+        if self.pc is not None:
+            self.add_poisson_correction((f.modes & 4) != 0)
+        self.advance_field_solutions()
+        rows, lost = self.advance_particles(screens, compat)
+        f.time_step += 1
+        return rows, lost

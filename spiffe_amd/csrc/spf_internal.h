@@ -1,0 +1,232 @@
+// spf_internal.h -- device-side state and kernel launch interfaces shared by
+// the HIP translation units of libspiffe_amd.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/spiffe_amd.h"
+
+namespace spf {
+
+constexpr int kMaxScreens = 24;
+
+// Field arrays live z-contiguous: element (iz, ir) at ir*ldz + iz, rows
+// ir = 0..nr, ldz >= nz+1 rounded up to 32 doubles (256 B) so each row starts
+// on its own cache-line group and a 64-lane wavefront reads 512 contiguous bytes.
+__host__ __device__ inline int64_t fix(int ldz, int iz, int ir) { return (int64_t)ir * ldz + iz; }
+
+// Parameters of the FDTD kernel: coefficient tables are computed on the host
+// with the reference's expressions so the device only repeats its +,-,*,/ steps.
+struct FdtdArgs {
+  int nz, nr, ldz;
+  int tm, te, left_neu, right_neu, upper_neu;
+  double dti, eps0, er_factor, ephi_factor, top_factor, dz, dr;
+  const double *Ez, *Er, *Bphi, *Ephi, *Br, *Bz;   // time level n
+  double *Ezn, *Ern, *Bphin, *Ephin, *Brn, *Bzn;   // time level n+1
+  double *Bphi2, *Br2, *Bz2;                       // half-step B, written in place
+  const double *Jz, *Jr, *Jphi;
+  const int16_t *flags;
+  const int32_t *cut0, *cut1;                      // per-row loop cut-offs
+  const double *cp, *cm, *bzf;                     // per-row coefficients
+};
+
+struct PushArgs {
+  int nz, nr, ldz;
+  int tm, te, longit, radial, has_sol;
+  double zmin, zmax, rmax, dz, dr, dt, time, c, QoMC, stiffness, zForcesStart;
+  const double *Ez, *Er, *Ephi, *Bphi2, *Br2, *Bz2, *EzImp, *ErImp, *BzImp, *BrImp;
+  const int16_t *flags;
+  int16_t turnoff[6];
+  double cst[6];
+  double *Q, *z, *r, *pz, *pr, *rpphi, *vz, *vr, *vphi, *r0, *z0, *t0, *rmax_p, *rmin_p, *zmax_p, *zmin_p;
+  uint32_t *status;
+  const int64_t *np;
+  // per-step event staging (indexed by particle)
+  uint32_t *evt;
+  uint8_t *evflag;
+  double *ev_zold, *ev_rold, *ev_vz, *ev_vr, *ev_pz0, *ev_pr0, *ev_rp0;
+  double *last_old;                 // old pz, pr, r_pphi of particle np-1
+  unsigned long long *new_active;   // activations this step
+  int nscreens;
+  double sz[kMaxScreens];
+  int sdir[kMaxScreens];
+};
+
+// Event rows of one step (single workgroup kernel) and compaction.
+struct RowsArgs {
+  int nz, nr, ldz, compat, nscreens;
+  double zmin, zmax, dz, dr, dt, time;
+  int64_t step;
+  const int16_t *flags;
+  const int32_t *evlist;
+  const int64_t *n_ev;
+  int64_t *np, *np_active;
+  const unsigned long long *new_active;
+  const uint32_t *evt;
+  const double *ev_zold, *ev_rold, *ev_vz, *ev_vr, *ev_pz0, *ev_pr0, *ev_rp0, *last_old;
+  double *Q, *z, *r, *pz, *pr, *rpphi, *vz, *vr, *vphi, *r0, *z0, *t0, *rmax_p, *rmin_p, *zmax_p, *zmin_p;
+  int32_t *pid;
+  int16_t *gen;
+  uint32_t *status;
+  double sz[kMaxScreens];
+  int sdir[kMaxScreens];
+  spf_screen_row *srows;
+  int64_t *n_srows;
+  int64_t srow_cap;
+  spf_lost_row *lrows;
+  int64_t *n_lrows;
+  int64_t lrow_cap;
+  int32_t *move_src, *move_dst;     // scratch, capacity = particle capacity
+  int32_t *overflow;
+};
+
+struct Run {
+  bool live = false;
+  spf_run_desc d{};
+  int ldz = 0;
+  int64_t cells = 0;        // (nr+1)*ldz
+  int64_t time_step = 0;
+  double time = 0;          // time of the current step, src/integrate.c:138
+  // fields: ping-pong pairs for E and B, single arrays for the rest
+  double *E2[6][2] = {};    // Ez, Er, Bphi, Ephi, Br, Bz
+  int cur[6] = {};
+  double *Bphi2 = nullptr, *Br2 = nullptr, *Bz2 = nullptr;
+  double *Jz = nullptr, *Jr = nullptr, *Jphi = nullptr;
+  double *Psi = nullptr, *Q = nullptr;
+  double *EzImp = nullptr, *ErImp = nullptr, *PsiImp = nullptr, *BzImp = nullptr, *BrImp = nullptr;
+  int16_t *flags = nullptr;
+  int32_t *cut0 = nullptr, *cut1 = nullptr;
+  double *cp = nullptr, *cm = nullptr, *bzf = nullptr;
+  double *sor_c = nullptr, *sor_d = nullptr, *volume = nullptr;
+  // particles
+  int64_t cap = 0;
+  int64_t np_ub = 0;        // host-side upper bound of the device particle count
+  double *P[16] = {};       // Q z r pz pr r_pphi vz vr vphi r0 z0 t0 rmax rmin zmax zmin
+  int32_t *pid = nullptr;
+  int16_t *gen = nullptr;
+  uint32_t *status = nullptr;
+  int64_t *counters = nullptr;   // [0] np, [1] npActive, [2] n_ev, [3] n_srows, [4] n_lrows, [5] G, [6] nvalid, [7] spare
+  unsigned long long *new_active = nullptr;
+  int32_t *overflow = nullptr;
+  // per-step events
+  uint32_t *evt = nullptr;
+  uint8_t *evflag = nullptr;
+  double *ev[7] = {};
+  double *last_old = nullptr;
+  int32_t *evlist = nullptr;
+  int32_t *move_src = nullptr, *move_dst = nullptr;
+  spf_screen_row *srows = nullptr;
+  spf_lost_row *lrows = nullptr;
+  int64_t srow_cap = 0, lrow_cap = 0;
+  // deposition sort state
+  bool sort_valid = false;
+  uint32_t *keys = nullptr, *skeys = nullptr;
+  int32_t *vals = nullptr, *svals = nullptr;
+  uint8_t *dbits = nullptr, *sbits = nullptr;
+  double *cz = nullptr, *cr = nullptr, *cph = nullptr, *scz = nullptr, *scr = nullptr, *scph = nullptr, *sq = nullptr;
+  int32_t *gstart = nullptr;
+  uint32_t *gkey = nullptr;
+  uint8_t *gflag = nullptr;
+  void *sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  int key_bits = 0;
+  // compaction scratch
+  int32_t *bcount = nullptr, *boff = nullptr;
+  int64_t nblk_cap = 0;
+  // staging
+  double *stage = nullptr;      // (nz+1)*(nr+1) doubles
+  void *append_stage = nullptr;
+  size_t append_cap = 0;
+  double **Pdev = nullptr;      // device copy of P[16]
+  // Poisson
+  bool pc_set = false;
+  spf_poisson_params pc{};
+  int64_t pc_start_step = -1;
+  double *omega = nullptr;      // omega per half-sweep
+  int64_t omega_n = 0;
+  double *red = nullptr;        // reduction partials
+  int64_t red_n = 0;
+  int32_t *red_counter = nullptr;
+  int64_t *sor_state = nullptr; // [0] done, [1] iterations
+  double *sor_sums = nullptr;   // [0] drive sum, [1] last residual sum, [2..] misc
+  double last_field_factor = -1;
+  std::vector<double> screen_z;
+  std::vector<int> screen_dir;
+  double ramp_time = 0, flat_top_time = 0;
+};
+
+// ---- launchers implemented in the .hip files ----
+hipError_t launch_fdtd(const FdtdArgs &a, hipStream_t s);
+hipError_t launch_transpose_in(const double *host_layout, double *dev, int nz, int nr, int ldz, hipStream_t s);
+hipError_t launch_transpose_out(const double *dev, double *host_layout, int nz, int nr, int ldz, hipStream_t s);
+hipError_t launch_transpose_in_i16(const int16_t *host_layout, int16_t *dev, int nz, int nr, int ldz, hipStream_t s);
+hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t s);
+hipError_t launch_set_i64(int64_t *p, int64_t v, hipStream_t s);
+hipError_t launch_scale(double *p, int64_t n, double f, hipStream_t s);
+hipError_t launch_add_line(double *J, int ldz, int direction, int64_t ipos, int64_t istart, int64_t iend, double v,
+                           hipStream_t s);
+hipError_t launch_push(const PushArgs &a, int64_t n_ub, hipStream_t s);
+hipError_t launch_rows(const RowsArgs &a, hipStream_t s);
+size_t append_stage_bytes(int64_t n);
+hipError_t launch_append_kernels(Run &R, const void *stage, int64_t n, int64_t n_active, double *const *Pdev,
+                                 hipStream_t s);
+// ordered compaction: out[k] = k-th index i < n with flags[i] != 0; *count = total.
+hipError_t ordered_compact(const uint8_t *flags, int64_t n, int32_t *out, int64_t *count, int32_t *bcount,
+                           int32_t *boff, hipStream_t s);
+int64_t compact_blocks(int64_t n);
+// deposition
+hipError_t deposit_prepare(Run &R, hipStream_t s);   // keys + sort + groups
+hipError_t deposit_fold(Run &R, hipStream_t s);      // J folds
+size_t deposit_sort_bytes(int64_t n, int key_bits);
+// Poisson
+hipError_t launch_gauss_q(Run &R, hipStream_t s);
+hipError_t launch_q_subtract(Run &R, hipStream_t s);
+hipError_t launch_q_to_rhs(Run &R, hipStream_t s);
+hipError_t launch_sum_abs(const double *a, int ldz, int iz_n, int ir_n, double *partials, int64_t pn, double *out,
+                          hipStream_t s);
+hipError_t launch_sum_plain(const double *a, int ldz, int iz_n, int ir_n, double *partials, int64_t pn, double *out,
+                            hipStream_t s);
+hipError_t launch_sor_halfsweep(Run &R, double *Psi, const double *rhs, int64_t n, double accuracy, hipStream_t s);
+int64_t sor_blocks(const Run &R);
+hipError_t launch_sor_reset(Run &R, hipStream_t s);
+hipError_t launch_field_from_psi(Run &R, int imposed, double field_factor, hipStream_t s);
+hipError_t launch_zero_psi(Run &R, hipStream_t s);
+
+}  // namespace spf
+
+// The context behind the opaque spf_ctx handle.
+struct spf_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<spf::Run *> runs;
+  std::string err;
+  int64_t *pinned = nullptr;   // small readbacks
+  void *hstage = nullptr;      // pinned staging for appends
+  size_t hstage_bytes = 0;
+  hipEvent_t stage_done = nullptr;
+};
+
+namespace spf {
+
+inline int fail(spf_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+inline Run *get_run(spf_ctx *c, int32_t run) {
+  if (!c || run < 0 || run >= (int32_t)c->runs.size() || !c->runs[run] || !c->runs[run]->live) return nullptr;
+  return c->runs[run];
+}
+
+}  // namespace spf
+
+#define HIPCALL(ctx, expr)                                                                         \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      return spf::fail(ctx, e_ == hipErrorOutOfMemory ? SPF_ENOMEM : SPF_EHIP,                     \
+                       std::string(#expr) + ": " + hipGetErrorString(e_));                         \
+  } while (0)

@@ -1,0 +1,691 @@
+// spf_push.hip -- relativistic particle push with staggered field gather,
+// loss detection and screen crossing (src/advance_particles.c:67-575), the
+// ordered per-step event rows, and particle compaction.
+//
+// One thread per particle, particles as structure-of-arrays in the
+// reference's index order.  Events (losses, screen crossings) are flagged per
+// particle; an ordered compaction turns them into an index-ordered list, so
+// screen and lost rows come out in the same order as the reference's serial
+// loops regardless of scheduling.
+#include <climits>
+
+#include "spf_internal.h"
+
+namespace spf {
+
+namespace {
+
+constexpr uint32_t ACTIVE = SPF_PART_ACTIVE, LOST = SPF_PART_LOST;
+constexpr uint32_t EV_LOST = 1u, EV_RNEG = 2u;  // screen s crossing: bit 2+s
+
+// C conversion double -> long truncates toward zero; out-of-range values and
+// NaN give LONG_MIN, as the x86-64 conversion instruction does.
+__device__ inline long long trunc_l(double x) {
+  if (!(x > -9.2233720368547758e18 && x < 9.2233720368547758e18)) return LLONG_MIN;
+  return (long long)x;
+}
+
+__device__ inline double sqr(double x) { return x * x; }
+
+struct G {
+  const PushArgs &a;
+  __device__ int16_t fl(long long iz, long long ir) const { return a.flags[fix(a.ldz, (int)iz, (int)ir)]; }
+  __device__ double at(const double *p, long long iz, long long ir) const { return p[fix(a.ldz, (int)iz, (int)ir)]; }
+
+  // src/advance_particles.c:31-65
+  __device__ short loss_code(long long iz, long long ir) const {
+    short lossCode = 0;
+    int16_t f = fl(iz, ir);
+    if (f & SPF_FL_IS_INSIDE)
+      lossCode = 3;
+    else if (f & SPF_FL_IS_SURFACE) {
+      long long iz1 = iz + 1;
+      if (iz1 > a.nz)
+        lossCode = 4;
+      else {
+        long long ir1 = ir + 1;
+        if (ir1 > a.nr)
+          lossCode = 5;
+        else if ((fl(iz1, ir1) & 3) && (fl(iz, ir1) & 3) && (fl(iz1, ir) & 3))
+          lossCode = 6;
+      }
+    }
+    return lossCode;
+  }
+
+  // src/advance_particles.c:622-890
+  __device__ void interp(double z, double r, double &EzR, double &ErR, double &EphiR, double &BzR, double &BrR,
+                         double &BphiR) const {
+    long long iz, ir, iz1, ir1, nz = a.nz, nr = a.nr;
+    double fr, fl_, fa, fb, dz = a.dz, dr = a.dr, zmin = a.zmin;
+    double Ez = 0, Er = 0, Ephi = 0, Bz = 0, Br = 0, Bphi = 0;
+    // -- Ez and Br
+    if (a.longit) {
+      fr = (z - zmin) / dz + 0.5;
+      iz = trunc_l(fr);
+      fr -= iz;
+      fl_ = 1 - fr;
+    } else {
+      iz = trunc_l((z - zmin) / dz + 1);
+      fl_ = 1;
+      fr = 0;
+    }
+    if (a.radial) {
+      fa = r / dr;
+      ir = trunc_l(fa);
+      fa -= ir;
+      fb = 1 - fa;
+    } else {
+      ir = trunc_l(r / dr + 0.5);
+      fa = 0;
+      fb = 1;
+    }
+    if (iz >= 0 && iz < nz && ir >= 0 && ir < nr) {
+      if ((iz1 = iz + 1) >= nz) iz1 = iz;
+      if ((ir1 = ir + 1) >= nr) ir1 = ir;
+      if (fl(iz, ir + 1) & SPF_FL_ER_ZERO && fl(iz, ir) & SPF_FL_ER_ZERO) {
+        if (iz > 0 && (fl(iz - 1, ir) || fl(iz - 1, ir + 1)))
+          iz = iz1;
+        else
+          iz1 = iz;
+      } else if (fl(iz1, ir + 1) & SPF_FL_ER_ZERO && fl(iz1, ir) & SPF_FL_ER_ZERO) {
+        iz1 = iz;
+      }
+      if (a.tm)
+        Ez = fb * (fl_ * (at(a.Ez, iz, ir) + at(a.EzImp, iz, ir)) + fr * (at(a.Ez, iz1, ir) + at(a.EzImp, iz1, ir))) +
+             fa * (fl_ * (at(a.Ez, iz, ir1) + at(a.EzImp, iz, ir1)) + fr * (at(a.Ez, iz1, ir1) + at(a.EzImp, iz1, ir1)));
+      if (r != 0 && a.te)
+        Br = fb * (fl_ * at(a.Br2, iz, ir) + fr * at(a.Br2, iz1, ir)) + fa * (fl_ * at(a.Br2, iz, ir1) + fr * at(a.Br2, iz1, ir1));
+    }
+    // -- Er and Bz
+    if (a.longit) {
+      fr = (z - zmin) / dz;
+      iz = trunc_l(fr);
+      fr -= iz;
+      fl_ = 1 - fr;
+    } else {
+      iz = trunc_l((z - zmin) / dz + 0.5);
+      fr = 0;
+      fl_ = 1;
+    }
+    if (a.radial) {
+      fa = r / dr + 0.5;
+      ir = trunc_l(fa);
+      fa -= ir;
+      fb = 1 - fa;
+    } else {
+      ir = trunc_l(r / dr + 1);
+      fa = 0;
+      fb = 1;
+    }
+    if (iz >= 0 && iz < nz && ir >= 0) {
+      if (ir >= nr) ir = nr - 1;
+      iz1 = iz + 1;
+      ir1 = ir + 1;
+      if (fl(iz + 1, ir) & SPF_FL_EZ_ZERO && fl(iz, ir) & SPF_FL_EZ_ZERO)
+        ir = ir1;
+      else if (fl(iz + 1, ir + 1) & SPF_FL_EZ_ZERO && fl(iz, ir + 1) & SPF_FL_EZ_ZERO)
+        ir1 = ir;
+      if (r != 0 && a.tm)
+        Er = fl_ * (fb * (at(a.Er, iz, ir) + at(a.ErImp, iz, ir)) + fa * (at(a.Er, iz, ir1) + at(a.ErImp, iz, ir1))) +
+             fr * (fb * (at(a.Er, iz1, ir) + at(a.ErImp, iz1, ir)) + fa * (at(a.Er, iz1, ir1) + at(a.ErImp, iz1, ir1)));
+      if (a.te)
+        Bz = fl_ * (fb * at(a.Bz2, iz, ir) + fa * at(a.Bz2, iz, ir1)) + fr * (fb * at(a.Bz2, iz1, ir) + fa * at(a.Bz2, iz1, ir1));
+    }
+    // -- Ephi
+    if (a.longit) {
+      fr = (z - zmin) / dz;
+      iz = trunc_l(fr);
+      fr -= iz;
+      fl_ = 1 - fr;
+    } else {
+      iz = trunc_l((z - zmin) / dz + 0.5);
+      fr = 0;
+      fl_ = 1;
+    }
+    if (a.radial) {
+      fa = r / dr;
+      ir = trunc_l(fa);
+      fa -= ir;
+      fb = 1 - fa;
+    } else {
+      ir = trunc_l(r / dr + 0.5);
+      fa = 0;
+      fb = 1;
+    }
+    if (r != 0 && a.te && iz >= 0 && iz < nz && ir >= 0 && ir < nr) {
+      if ((iz1 = iz + 1) >= nz) iz1 = iz;
+      if ((ir1 = ir + 1) >= nr) ir1 = ir;
+      Ephi = fl_ * fb * at(a.Ephi, iz, ir) + fr * fb * at(a.Ephi, iz1, ir) + fl_ * fa * at(a.Ephi, iz, ir1) +
+             fr * fa * at(a.Ephi, iz1, ir1);
+    }
+    // -- Bphi at time t
+    if (a.tm) {
+      if (a.longit) {
+        fr = (z - zmin) / dz + 0.5;
+        iz = trunc_l(fr);
+        fr -= iz;
+        fl_ = 1 - fr;
+      } else {
+        iz = trunc_l((z - zmin) / dz + 1);
+        fr = 0;
+        fl_ = 1;
+      }
+      if (a.radial) {
+        fa = r / dr + 0.5;
+        ir = trunc_l(fa);
+        fa -= ir;
+        fb = 1 - fa;
+      } else {
+        ir = trunc_l(r / dr + 1);
+        fa = 0;
+        fb = 1;
+      }
+      // ir < 0 only for NaN or astronomically large r, where the reference
+      // would index outside its arrays; the device skips the component.
+      if (r != 0 && iz >= 0 && iz < nz && ir >= 0) {
+        if ((iz1 = iz + 1) >= nz) iz1 = iz;
+        if (ir >= nr) ir = nr - 1;
+        if ((ir1 = ir + 1) >= nr) ir1 = ir;
+        Bphi = fl_ * fb * at(a.Bphi2, iz, ir) + fr * fb * at(a.Bphi2, iz1, ir) + fa * fl_ * at(a.Bphi2, iz, ir1) +
+               fr * fa * at(a.Bphi2, iz1, ir1);
+      }
+    }
+    // constants, then the on-axis expansion's +0.0 (none defined), :829-844
+    Ez += a.cst[0];
+    Er += a.cst[1];
+    Ephi += a.cst[2];
+    Bz += a.cst[3];
+    Br += a.cst[4];
+    Bphi += a.cst[5];
+    Ez += 0.0;
+    Er += 0.0;
+    Bphi += 0.0;
+    if (a.turnoff[0]) Ez = 0;
+    if (a.turnoff[1]) Er = 0;
+    if (a.turnoff[2]) Ephi = 0;
+    if (a.turnoff[3]) Bz = 0;
+    if (a.turnoff[4]) Br = 0;
+    if (a.turnoff[5]) Bphi = 0;
+    if (a.has_sol) {
+      fr = (z - zmin) / dz;
+      iz = trunc_l(fr);
+      fa = r / dr;
+      ir = trunc_l(fa);
+      if (ir >= 0 && iz >= 0 && ir < nr && iz < nz) {
+        fa -= ir;
+        fb = 1 - fa;
+        fr -= iz;
+        fl_ = 1 - fr;
+        Bz += fl_ * (fb * at(a.BzImp, iz, ir) + fa * at(a.BzImp, iz, ir + 1)) +
+              fr * (fb * at(a.BzImp, iz + 1, ir) + fa * at(a.BzImp, iz + 1, ir + 1));
+        Br += fl_ * (fb * at(a.BrImp, iz, ir) + fa * at(a.BrImp, iz, ir + 1)) +
+              fr * (fb * at(a.BrImp, iz + 1, ir) + fa * at(a.BrImp, iz + 1, ir + 1));
+      }
+    }
+    EzR = Ez;
+    ErR = Er;
+    EphiR = Ephi;
+    BzR = Bz;
+    BrR = Br;
+    BphiR = Bphi;
+  }
+};
+
+// src/advance_particles.c:131-339 for one particle
+__global__ __launch_bounds__(256) void push_kernel(PushArgs a, int64_t n_ub) {
+  int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= n_ub) return;
+  int64_t np = *a.np;
+  if (ip >= np) {
+    a.evflag[ip] = 0;
+    return;
+  }
+  G g{a};
+  double dt = a.dt, dz = a.dz, dr = a.dr, zmin = a.zmin, zmax = a.zmax, rmax = a.rmax;
+  double z = a.z[ip], r = a.r[ip];
+  const double zold = z, rold = r;
+  double zmx = a.zmax_p[ip], zmn = a.zmin_p[ip], rmx = a.rmax_p[ip], rmn = a.rmin_p[ip];
+  if (z > zmx) zmx = z;
+  if (z < zmn) zmn = z;
+  if (r > rmx) rmx = r;
+  if (r < rmn) rmn = r;
+  double vz = a.vz[ip], vr = a.vr[ip];
+  z += vz * dt;
+  r += vr * dt;
+  uint32_t st = a.status[ip];
+  double adjustFactor = 1;
+  if (!(st & ACTIVE)) {
+    long long iz = trunc_l((z - zmin) / dz);
+    long long ir = trunc_l(fabs(r) / dr);
+    bool inside = z >= a.zmin && z <= a.zmax && r <= a.rmax;
+    if (inside && !g.loss_code(iz, ir)) {
+      st |= ACTIVE;
+      atomicAdd(a.new_active, 1ull);
+      a.t0[ip] = a.time;
+      a.r0[ip] = rold;
+      a.z0[ip] = zold;
+      if (iz > 0 && g.loss_code(iz - 1, ir))
+        adjustFactor = (z - (iz * a.dz + a.zmin)) / (z - zold);
+      else
+        adjustFactor = (z - a.zmin) / (z - zold);
+      if (adjustFactor > 1) adjustFactor = 1;
+    }
+  }
+  if (a.zForcesStart > a.zmin && a.zForcesStart < a.zmax) {
+    if (zold < a.zForcesStart) {
+      if (z < a.zForcesStart)
+        adjustFactor = 0;
+      else
+        adjustFactor = (z - a.zForcesStart) / (z - zold);
+    }
+  }
+  double pz = a.pz[ip], pr = a.pr[ip], r_pphi = a.rpphi[ip], vphi = a.vphi[ip];
+  const double pz0 = pz, pr0 = pr, rp0 = r_pphi;
+  if (ip == np - 1) {
+    a.last_old[0] = pz0;
+    a.last_old[1] = pr0;
+    a.last_old[2] = rp0;
+  }
+  bool lost = false;
+  if ((st & ACTIVE) && (z < zmin || z > zmax || r > rmax)) lost = true;
+  long long iz = trunc_l((z - zmin) / dz);
+  long long ir = trunc_l(fabs(r) / dr);
+  if ((st & ACTIVE) && (iz > a.nz || ir > a.nr || iz < 0 || ir < 0)) lost = true;
+  if (!lost && (st & ACTIVE)) {
+    if (g.loss_code(iz, ir)) lost = true;
+  }
+  bool rneg = false;
+  if (r < 0) {
+    r *= -1;
+    pr *= -1;
+    rneg = true;
+  }
+  if (!lost && (st & ACTIVE)) {
+    if (z > zmx) zmx = z;
+    if (z < zmn) zmn = z;
+    if (r > rmx) rmx = r;
+    if (r < rmn) rmn = r;
+  }
+  double Ez, Er, Ephi, Bz, Br, Bphi;
+  if (st & ACTIVE)
+    g.interp(z, r, Ez, Er, Ephi, Bz, Br, Bphi);
+  else
+    Ez = Er = Ephi = Bz = Br = Bphi = 0;
+  // momentum kick, :282-287
+  double pphi = (r != 0 ? r_pphi / r : 0);
+  double gamma = sqrt(sqr(pr) + sqr(pz) + sqr(pphi) + 1);
+  pr += adjustFactor * dt * ((r != 0 ? sqr(pphi) * a.c / gamma / r : 0) + a.QoMC * (Er + vphi * Bz - vz * Bphi)) / a.stiffness;
+  pz += adjustFactor * dt * a.QoMC * (Ez + vr * Bphi - vphi * Br) / a.stiffness;
+  r_pphi += adjustFactor * dt * a.QoMC * r * (Ephi + vz * Br - vr * Bz) / a.stiffness;
+  // velocities, :605-620
+  double nvz, nvr, nvphi;
+  if (r != 0) {
+    gamma = sqrt(sqr(pz) + sqr(pr) + sqr(r_pphi / r) + 1);
+    nvphi = a.c * r_pphi / r / gamma;
+  } else {
+    gamma = sqrt(sqr(pz) + sqr(pr) + 1);
+    nvphi = 0;
+  }
+  nvz = a.c * pz / gamma;
+  nvr = a.c * pr / gamma;
+  a.z[ip] = z;
+  a.r[ip] = r;
+  a.pz[ip] = pz;
+  a.pr[ip] = pr;
+  a.rpphi[ip] = r_pphi;
+  a.vz[ip] = nvz;
+  a.vr[ip] = nvr;
+  a.vphi[ip] = nvphi;
+  a.zmax_p[ip] = zmx;
+  a.zmin_p[ip] = zmn;
+  a.rmax_p[ip] = rmx;
+  a.rmin_p[ip] = rmn;
+  if (lost) st = (st & ~ACTIVE) | LOST;
+  a.status[ip] = st;
+  // screen crossings, tested as in :370 and :399
+  uint32_t ev = (lost ? EV_LOST : 0u) | (rneg ? EV_RNEG : 0u);
+  for (int s = 0; s < a.nscreens; s++) {
+    double zs = a.sz[s];
+    bool fwd = zs <= z && zs > zold && a.sdir[s] == 1;
+    bool bwd = !fwd && zs >= z && zs < zold && a.sdir[s] == -1;
+    if (fwd || bwd) ev |= 1u << (2 + s);
+  }
+  bool any = (ev & ~EV_RNEG) != 0;
+  a.evflag[ip] = any ? 1 : 0;
+  if (any) {
+    a.evt[ip] = ev;
+    a.ev_zold[ip] = zold;
+    a.ev_rold[ip] = rold;
+    a.ev_vz[ip] = vz;
+    a.ev_vr[ip] = vr;
+    a.ev_pz0[ip] = pz0;
+    a.ev_pr0[ip] = pr0;
+    a.ev_rp0[ip] = rp0;
+  }
+}
+
+// src/secondaryEmission.c:239-372
+__device__ void find_loss_coordinates(const RowsArgs &a, double *zReturn, double *rReturn, short *directionCode,
+                                      double z0, double r0, double z1, double r1) {
+  double zm, rm;
+  long long iz1, ir1, iz0, ir0;
+  auto FL = [&](long long iz, long long ir) -> int16_t {
+    if (iz < 0 || iz > a.nz || ir < 0 || ir > a.nr) return 0;  // the reference reads outside its array here
+    return a.flags[fix(a.ldz, (int)iz, (int)ir)];
+  };
+  *directionCode = 0;
+  iz0 = trunc_l((z0 - a.zmin) / a.dz);
+  ir0 = trunc_l(r0 / a.dr);
+  if (z1 < a.zmin)
+    iz1 = -1;
+  else
+    iz1 = trunc_l((z1 - a.zmin) / a.dz);
+  if (r1 < 0)
+    ir1 = -1;
+  else
+    ir1 = trunc_l(r1 / a.dr);
+  if (r0 == r1 && z0 == z1) {
+    *directionCode = 1;
+    *zReturn = z0;
+    *rReturn = r0;
+    return;
+  }
+  if (ir0 == ir1) {
+    *directionCode = iz0 > iz1 ? 1 : 2;
+  } else if (iz0 == iz1) {
+    *directionCode = ir0 > ir1 ? 4 : 3;
+  } else {
+    if (z1 < z0) {
+      if (z1 < 0)
+        *directionCode = 1;
+      else if (ir0 != ir1) {
+        if (ir1 >= 0 && ir1 < a.nr && FL(iz0, ir1) & 3)
+          *directionCode = 1;
+        else if (ir1 < 0 && FL(iz0, -ir1) & 3)
+          *directionCode = 1;
+      } else if (ir0 == ir1 && (ir0 < (a.nr - 1) && FL(iz0, ir0 + 1) & 3))
+        *directionCode = 1;
+      else
+        *directionCode = r1 < r0 ? 4 : 3;
+    } else if (z1 > z0) {
+      if (z1 > a.zmax)
+        *directionCode = 2;
+      else if (ir0 != ir1) {
+        if (ir1 >= 0 && ir1 < a.nr && FL(iz1, ir1) & 3)
+          *directionCode = 2;
+        else if (ir1 < 0 && FL(iz1, -ir1) & 3)
+          *directionCode = 2;
+      } else if (ir0 == ir1 && (ir0 < (a.nr - 1) && FL(iz1, ir0 + 1) & 3))
+        *directionCode = 2;
+      else
+        *directionCode = r1 < r0 ? 4 : 3;
+    } else
+      *directionCode = r1 < r0 ? 4 : 3;
+  }
+  if (*directionCode == 1) {
+    zm = ((long long)((z0 - a.zmin) / a.dz)) * a.dz + a.zmin + a.dz * 1e-12;
+    rm = (r1 - r0) / (z1 - z0) * (zm - z0) + r0;
+  } else if (*directionCode == 2) {
+    zm = ((long long)((z0 - a.zmin) / a.dz)) * a.dz + a.zmin + a.dz * (1 - 1e-12);
+    rm = (r1 - r0) / (z1 - z0) * (zm - z0) + r0;
+  } else if (*directionCode == 4) {
+    rm = ((long long)(r0 / a.dr)) * a.dr + a.dr * 1e-12;
+    zm = (z1 - z0) / (r1 - r0) * (rm - r0) + z0;
+  } else {
+    rm = ((long long)(r0 / a.dr)) * a.dr + a.dr * (1 - 1e-12);
+    zm = (z1 - z0) / (r1 - r0) * (rm - r0) + z0;
+  }
+  *zReturn = zm;
+  *rReturn = fabs(rm);
+}
+
+constexpr int ROWS_THREADS = 1024;
+
+// exclusive prefix of a 0/1 flag over the workgroup; returns the total
+__device__ int block_flag_scan(bool f, int *wsum, int &excl) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  unsigned long long m = __ballot(f);
+  unsigned long long lt = lane ? (m & ((1ull << lane) - 1ull)) : 0ull;
+  int in_wave = __popcll(lt);
+  if (lane == 0) wsum[w] = __popcll(m);
+  __syncthreads();
+  int before = 0, total = 0;
+  for (int k = 0; k < nw; k++) {
+    int v = wsum[k];
+    if (k < w) before += v;
+    total += v;
+  }
+  __syncthreads();
+  excl = before + in_wave;
+  return total;
+}
+
+// Builds the step's screen rows (grouped per screen, index order within a
+// screen) and lost rows (index order), then compacts the particle arrays.
+__global__ __launch_bounds__(ROWS_THREADS) void rows_kernel(RowsArgs a) {
+  __shared__ int wsum[ROWS_THREADS / 64];
+  __shared__ long long sh_first_lost;
+  const int64_t n_ev = *a.n_ev;
+  const int64_t np = a.np[0];
+  const double dt = a.dt;
+  // ---- screens
+  int64_t base = *a.n_srows;
+  for (int s = 0; s < a.nscreens; s++) {
+    int64_t running = 0;
+    for (int64_t c = 0; c < n_ev; c += blockDim.x) {
+      int64_t k = c + threadIdx.x;
+      int32_t ip = k < n_ev ? a.evlist[k] : 0;
+      bool has = k < n_ev && ((a.evt[ip] >> (2 + s)) & 1u);
+      int excl;
+      int tot = block_flag_scan(has, wsum, excl);
+      if (has) {
+        double zold = a.ev_zold[ip], vzz = a.ev_vz[ip], vra = a.ev_vr[ip];
+        double znew = a.z[ip], rnew = a.r[ip];
+        double r0v = a.ev_rold[ip];
+        spf_screen_row row;
+        for (int js = 0; js <= s; js++) {
+          double zs = a.sz[js];
+          bool fwd = zs <= znew && zs > zold && a.sdir[js] == 1;
+          bool bwd = !fwd && zs >= znew && zs < zold && a.sdir[js] == -1;
+          if (!fwd && !bwd) continue;
+          double dtp = (zs - zold) / vzz;
+          if (fwd) {
+            if (js != s) continue;
+            double spz0 = a.compat ? a.last_old[0] : a.ev_pz0[ip];
+            double spr0 = a.compat ? a.last_old[1] : a.ev_pr0[ip];
+            double srp0 = a.compat ? a.last_old[2] : a.ev_rp0[ip];
+            double pphi_a = rnew != 0 ? a.rpphi[ip] / rnew : 0;
+            double pphi0 = r0v != 0 ? srp0 / r0v : 0;
+            row.r = r0v + vra * dtp;
+            row.pz = spz0 + (a.pz[ip] - spz0) / dt * dtp;
+            row.pr = spr0 + (a.pr[ip] - spr0) / dt * dtp;
+            row.pphi = pphi0 + (pphi_a - pphi0) / dt * dtp;
+          } else {
+            r0v += vra * dtp;
+            if (js != s) continue;
+            row.r = r0v;
+            row.pz = a.pz[ip];
+            row.pr = a.pr[ip];
+            row.pphi = r0v != 0 ? a.rpphi[ip] / r0v : 0;
+          }
+          row.t = (a.time - dt + dtp);
+        }
+        row.q = a.Q[ip];
+        row.r0 = a.r0[ip];
+        row.t0 = a.t0[ip];
+        row.rmin = a.rmin_p[ip];
+        row.rmax = a.rmax_p[ip];
+        row.zmin = a.zmin_p[ip];
+        row.zmax = a.zmax_p[ip];
+        row.ip = ip;
+        row.step = a.step;
+        row.particleID = a.pid[ip];
+        row.generation = a.gen[ip];
+        row.screen = (int16_t)s;
+        int64_t pos = base + running + excl;
+        if (pos < a.srow_cap)
+          a.srows[pos] = row;
+        else
+          atomicOr(a.overflow, 1);
+      }
+      running += tot;
+    }
+    base += running;
+  }
+  if (threadIdx.x == 0) *a.n_srows = base;
+  // ---- lost rows; the ordered lost list goes to move_dst
+  int64_t lbase = *a.n_lrows;
+  int64_t nlost = 0;
+  for (int64_t c = 0; c < n_ev; c += blockDim.x) {
+    int64_t k = c + threadIdx.x;
+    int32_t ip = k < n_ev ? a.evlist[k] : 0;
+    bool has = k < n_ev && (a.evt[ip] & EV_LOST);
+    int excl;
+    int tot = block_flag_scan(has, wsum, excl);
+    if (has) {
+      double zLost, rLost;
+      short code;
+      double rr = a.r[ip] * ((a.evt[ip] & EV_RNEG) ? -1 : 1);
+      find_loss_coordinates(a, &zLost, &rLost, &code, a.ev_zold[ip], a.ev_rold[ip], a.z[ip], rr);
+      spf_lost_row row;
+      row.t = a.time;
+      row.rLost = (float)rLost;
+      row.zLost = (float)zLost;
+      row.pz = (float)a.pz[ip];
+      row.pr = (float)a.pr[ip];
+      row.r0 = (float)a.r0[ip];
+      row.z0 = (float)a.z0[ip];
+      row.q = (float)a.Q[ip];
+      row.particleID = a.pid[ip];
+      row.ip = ip;
+      row.step = a.step;
+      row.generation = a.gen[ip];
+      row.directionCode = code;
+      row.pad = 0;
+      int64_t pos = lbase + nlost + excl;
+      if (pos < a.lrow_cap)
+        a.lrows[pos] = row;
+      else
+        atomicOr(a.overflow, 1);
+      a.move_dst[nlost + excl] = ip;
+    }
+    nlost += tot;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *a.n_lrows = lbase + nlost;
+  // ---- compaction, src/advance_particles.c:490-518
+  int64_t new_active = (int64_t)*a.new_active;
+  if (a.compat) {
+    // the reference re-tests the same slot without moving its lost flag, so
+    // np ends at the first lost index
+    if (threadIdx.x == 0) {
+      int64_t first = nlost ? a.move_dst[0] : np;
+      int64_t removed = np - first;
+      a.np[0] = first;
+      a.np_active[0] += new_active - removed;
+    }
+    return;
+  }
+  // swap-remove with the flag moved too: the k-th lost slot below the new np
+  // receives the k-th survivor counted from the end
+  int64_t new_np = np - nlost;
+  if (threadIdx.x == 0) {
+    int64_t m = 0;
+    while (m < nlost && a.move_dst[m] < new_np) m++;
+    int64_t t = nlost - 1, j = np - 1;
+    for (int64_t k = 0; k < m; k++) {
+      while (t >= 0 && a.move_dst[t] == j) {
+        t--;
+        j--;
+      }
+      a.move_src[k] = (int32_t)j;
+      j--;
+    }
+    sh_first_lost = m;
+  }
+  __syncthreads();
+  int64_t m = sh_first_lost;
+  for (int64_t k = threadIdx.x; k < m; k += blockDim.x) {
+    int32_t d = a.move_dst[k], s = a.move_src[k];
+    a.Q[d] = a.Q[s];
+    a.z[d] = a.z[s];
+    a.r[d] = a.r[s];
+    a.pz[d] = a.pz[s];
+    a.pr[d] = a.pr[s];
+    a.vz[d] = a.vz[s];
+    a.vr[d] = a.vr[s];
+    a.rpphi[d] = a.rpphi[s];
+    a.vphi[d] = a.vphi[s];
+    a.t0[d] = a.t0[s];
+    a.r0[d] = a.r0[s];
+    a.z0[d] = a.z0[s];
+    a.rmin_p[d] = a.rmin_p[s];
+    a.zmin_p[d] = a.zmin_p[s];
+    a.rmax_p[d] = a.rmax_p[s];
+    a.zmax_p[d] = a.zmax_p[s];
+    a.gen[d] = a.gen[s];
+    a.pid[d] = a.pid[s];
+    a.status[d] = a.status[s];
+  }
+  if (threadIdx.x == 0) {
+    a.np[0] = new_np;
+    a.np_active[0] += new_active - nlost;
+  }
+}
+
+__global__ void append_kernel(const uint8_t *stage, int64_t n, int64_t cap, double *const *P, int32_t *pid,
+                              int16_t *gen, uint32_t *status, const int64_t *np) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int64_t dst = *np + k;
+  if (dst >= cap) return;
+  const double *d = (const double *)stage;
+  for (int j = 0; j < 16; j++) P[j][dst] = d[(int64_t)j * n + k];
+  const int32_t *ids = (const int32_t *)(d + 16 * n);
+  const int16_t *gens = (const int16_t *)(ids + n);
+  const uint32_t *sts = (const uint32_t *)((const uint8_t *)stage + 16 * 8 * n + 4 * n + ((2 * n + 3) / 4) * 4);
+  pid[dst] = ids[k];
+  gen[dst] = gens[k];
+  status[dst] = sts[k];
+}
+
+__global__ void append_commit_kernel(int64_t *counters, int64_t n, int64_t n_active, int64_t cap,
+                                     int32_t *overflow) {
+  int64_t np = counters[0] + n;
+  if (np > cap) {
+    *overflow |= 2;
+    np = cap;
+  }
+  counters[0] = np;
+  counters[1] += n_active;
+}
+
+}  // namespace
+
+hipError_t launch_push(const PushArgs &a, int64_t n_ub, hipStream_t s) {
+  if (n_ub <= 0) return hipSuccess;
+  hipLaunchKernelGGL(push_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, s, a, n_ub);
+  return hipGetLastError();
+}
+
+hipError_t launch_rows(const RowsArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(rows_kernel, dim3(1), dim3(ROWS_THREADS), 0, s, a);
+  return hipGetLastError();
+}
+
+// stage layout: 16 double columns of n, int32 ids, int16 generations (padded
+// to 4 bytes), uint32 status
+size_t append_stage_bytes(int64_t n) { return 16 * 8 * n + 4 * n + ((2 * n + 3) / 4) * 4 + 4 * n; }
+
+hipError_t launch_append_kernels(Run &R, const void *stage, int64_t n, int64_t n_active, double *const *Pdev,
+                                 hipStream_t s) {
+  hipLaunchKernelGGL(append_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const uint8_t *)stage, n,
+                     R.cap, Pdev, R.pid, R.gen, R.status, R.counters);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(append_commit_kernel, dim3(1), dim3(1), 0, s, R.counters, n, n_active, R.cap, R.overflow);
+  return hipGetLastError();
+}
+
+}  // namespace spf

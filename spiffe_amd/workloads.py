@@ -1,0 +1,65 @@
+"""Synthetic workloads of the benchmark configurations (seeded).
+
+gun_16cell(): the production-size single run -- a 1.6-cell S-band gun
+(half cell 0-3.5 cm, full cell 3.5-10.5 cm, cell radius 4.2 cm, iris radius
+1.25 cm, iris 0.5 cm thick), dr = dz = 0.1 mm (1051 x 421 grid lines),
+TM+TE fields, 1M macroparticles carrying 1 nC, Poisson correction every 10
+steps to 1e-8, a screen at z = 10 cm.  Fields start from a synthetic
+standing-wave pattern scaled to 120 MV/m on axis and the bunch starts as a
+uniform 1 mm-radius slug just off the cathode: the shapes and sizes of the
+configuration with synthetic contents (no eigenmode solve, no emission
+history).
+"""
+import numpy as np
+
+from . import abi
+from . import constants as K
+from .device import Grid, courant_dt
+from .geometry import derive_flags, polygon_marks
+
+GUN_POLY_16 = [(0.0, 0.0), (0.0, 0.042), (0.0325, 0.042), (0.0325, 0.0125), (0.0375, 0.0125), (0.0375, 0.042),
+               (0.105, 0.042), (0.105, 0.0), (0.0, 0.0)]
+
+
+def gun_grid(dz=1e-4, modes=abi.TM_FIELDS | abi.TE_FIELDS | abi.RADIAL_INTERP | abi.LONGIT_INTERP |
+             abi.SPACE_CHARGE):
+    zmax, rmax = 0.105, 0.042
+    nz = int(round(zmax / dz)) + 1
+    nr = int(round(rmax / dz)) + 1
+    g = Grid(nz=nz, nr=nr, zmin=0.0, zmax=zmax, rmax=rmax, dt=0.0, modes=modes)
+    g.dt = courant_dt(g.dz, g.dr, 0.9)
+    return g
+
+
+def gun_flags(grid):
+    return derive_flags(polygon_marks(grid.nz, grid.nr, grid.zmin, grid.zmax, grid.rmax, GUN_POLY_16))
+
+
+def gun_fields(grid, E0=120e6):
+    """A standing-wave Ez pattern, pi-mode-like between the two cells, zero inside metal."""
+    z = grid.zmin + grid.dz * (np.arange(grid.nz + 1) - 0.5)
+    r = grid.dr * np.arange(grid.nr + 1)
+    from scipy.special import j0
+    ez_z = np.where(z < 0.035, 1.0, -1.0) * np.abs(np.cos(np.pi * (z - 0.035) / 0.07))
+    Ez = E0 * np.outer(ez_z, j0(2.404825557695773 * np.minimum(r, 0.042) / 0.042))
+    return {"Ez": Ez}
+
+
+def gun_bunch(grid, n, charge=1e-9, radius=1e-3, seed=7, z0=0.3e-3, length=1.5e-3, pz=0.6):
+    rng = np.random.default_rng(seed)
+    r = radius * np.sqrt(rng.random(n))
+    z = grid.zmin + z0 + length * rng.random(n)
+    pzv = pz * (1 + 0.02 * rng.standard_normal(n))
+    prv = 0.01 * rng.standard_normal(n)
+    rp = np.zeros(n)
+    g = np.sqrt(pzv * pzv + prv * prv + 1)
+    q = -charge / n
+    return {"Q": np.full(n, q), "z": z, "r": r, "pz": pzv, "pr": prv, "r_pphi": rp,
+            "vz": K.c_mks * pzv / g, "vr": K.c_mks * prv / g, "vphi": np.zeros(n),
+            "r0": r.copy(), "z0": z.copy(), "t0": np.zeros(n), "rmax": r.copy(), "rmin": r.copy(),
+            "zmax": z.copy(), "zmin": z.copy(), "particleID": np.arange(n, dtype=np.int32),
+            "generation": np.zeros(n, dtype=np.int16), "status": np.full(n, abi.PART_ACTIVE, dtype=np.uint32)}
+
+
+GUN_SCREENS = [(0.10, 1)]
+GUN_POISSON = dict(step_interval=10, accuracy=1e-8, maximum_iterations=3000)
