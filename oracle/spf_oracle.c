@@ -509,6 +509,10 @@ short oracle_check_loss_code(int64_t iz, int64_t ir, const ofields *f) {
   short lossCode = 0;
   int64_t iz1, ir1;
   const int16_t *mf = f->metal_flags;
+  /* outside the flag array (inactive particle at r < -rmax): the reference
+   * reads out of bounds here; both implementations count it as metal */
+  if (iz < 0 || iz > f->nz || ir < 0 || ir > f->nr)
+    return 3;
   if (mf[IDX(f, iz, ir)] & OFL_IS_INSIDE)
     lossCode = 3;
   else if (mf[IDX(f, iz, ir)] & OFL_IS_SURFACE) {

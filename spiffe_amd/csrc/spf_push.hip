@@ -34,6 +34,9 @@ struct G {
 
   // src/advance_particles.c:31-65
   __device__ short loss_code(long long iz, long long ir) const {
+    // an inactive particle at r < -rmax reaches here with ir > nr, where the
+    // reference reads outside its flag array; such a point counts as metal
+    if (iz < 0 || iz > a.nz || ir < 0 || ir > a.nr) return 3;
     short lossCode = 0;
     int16_t f = fl(iz, ir);
     if (f & SPF_FL_IS_INSIDE)
