@@ -741,11 +741,19 @@ void oracle_interpolate_fields(double *EzR, double *ErR, double *EphiR, double *
 }
 
 /* src/secondaryEmission.c:239-372 */
+/* flag lookup for the loss-coordinate search: the reference indexes with
+ * iz1 = -1 when a particle ends below zmin (src/secondaryEmission.c:314) and
+ * reads outside its array; both implementations read such points as 0 */
+static int16_t loss_flag(const ofields *f, int64_t iz, int64_t ir) {
+  if (iz < 0 || iz > f->nz || ir < 0 || ir > f->nr)
+    return 0;
+  return f->metal_flags[IDX(f, iz, ir)];
+}
+
 void oracle_find_loss_coordinates(double *zReturn, double *rReturn, short *directionCode,
                                   double z0, double r0, double z1, double r1, const ofields *f) {
   double zm, rm;
   int64_t iz1, ir1, iz0, ir0;
-  const int16_t *mf = f->metal_flags;
   *directionCode = 0;
   iz0 = (z0 - f->zmin) / f->dz;
   ir0 = r0 / f->dr;
@@ -772,11 +780,11 @@ void oracle_find_loss_coordinates(double *zReturn, double *rReturn, short *direc
       if (z1 < 0)
         *directionCode = O_GOING_LEFT;
       else if (ir0 != ir1) {
-        if (ir1 >= 0 && ir1 < f->nr && mf[IDX(f, iz0, ir1)] & OFL_IS_METAL)
+        if (ir1 >= 0 && ir1 < f->nr && loss_flag(f, iz0, ir1) & OFL_IS_METAL)
           *directionCode = O_GOING_LEFT;
-        else if (ir1 < 0 && mf[IDX(f, iz0, -ir1)] & OFL_IS_METAL)
+        else if (ir1 < 0 && loss_flag(f, iz0, -ir1) & OFL_IS_METAL)
           *directionCode = O_GOING_LEFT;
-      } else if (ir0 == ir1 && (ir0 < (f->nr - 1) && mf[IDX(f, iz0, ir0 + 1)] & OFL_IS_METAL))
+      } else if (ir0 == ir1 && (ir0 < (f->nr - 1) && loss_flag(f, iz0, ir0 + 1) & OFL_IS_METAL))
         *directionCode = O_GOING_LEFT;
       else
         *directionCode = r1 < r0 ? O_GOING_DOWN : O_GOING_UP;
@@ -784,11 +792,11 @@ void oracle_find_loss_coordinates(double *zReturn, double *rReturn, short *direc
       if (z1 > f->zmax)
         *directionCode = O_GOING_RIGHT;
       else if (ir0 != ir1) {
-        if (ir1 >= 0 && ir1 < f->nr && mf[IDX(f, iz1, ir1)] & OFL_IS_METAL)
+        if (ir1 >= 0 && ir1 < f->nr && loss_flag(f, iz1, ir1) & OFL_IS_METAL)
           *directionCode = O_GOING_RIGHT;
-        else if (ir1 < 0 && mf[IDX(f, iz1, -ir1)] & OFL_IS_METAL)
+        else if (ir1 < 0 && loss_flag(f, iz1, -ir1) & OFL_IS_METAL)
           *directionCode = O_GOING_RIGHT;
-      } else if (ir0 == ir1 && (ir0 < (f->nr - 1) && mf[IDX(f, iz1, ir0 + 1)] & OFL_IS_METAL))
+      } else if (ir0 == ir1 && (ir0 < (f->nr - 1) && loss_flag(f, iz1, ir0 + 1) & OFL_IS_METAL))
         *directionCode = O_GOING_RIGHT;
       else
         *directionCode = r1 < r0 ? O_GOING_DOWN : O_GOING_UP;
