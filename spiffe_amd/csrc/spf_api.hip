@@ -27,28 +27,20 @@ hipError_t dalloc(T **p, int64_t n) {
 
 int64_t ldz_for(int nz) { return ((int64_t)(nz + 1) + 31) / 32 * 32; }
 
-int key_bits_for(int nz, int nr) {
-  uint64_t groups = (uint64_t)(nz + 2) * (uint64_t)(nr + 2);
-  int b = 1;
-  while (b < 32 && (1ull << b) <= groups) b++;
-  return b;
-}
-
 void free_run(Run *R) {
   auto F = [](void *p) {
     if (p) (void)hipFree(p);
   };
   for (int k = 0; k < 6; k++)
     for (int j = 0; j < 2; j++) F(R->E2[k][j]);
-  F(R->Bphi2); F(R->Br2); F(R->Bz2); F(R->Jz); F(R->Jr); F(R->Jphi); F(R->Psi); F(R->Q);
+  F(R->Bphi2); F(R->Br2); F(R->Bz2); F(R->Jz); F(R->Jr); F(R->Jphi); F(R->Psi); F(R->Q); F(R->PsiAlt);
   F(R->EzImp); F(R->ErImp); F(R->PsiImp); F(R->BzImp); F(R->BrImp); F(R->flags); F(R->cut0); F(R->cut1);
   F(R->cp); F(R->cm); F(R->bzf); F(R->sor_c); F(R->sor_d); F(R->volume);
   for (int k = 0; k < 16; k++) F(R->P[k]);
   F(R->pid); F(R->gen); F(R->status); F(R->counters); F(R->evt); F(R->evflag);
   for (int k = 0; k < 7; k++) F(R->ev[k]);
   F(R->last_old); F(R->evlist); F(R->move_src); F(R->move_dst); F(R->srows); F(R->lrows);
-  F(R->keys); F(R->skeys); F(R->vals); F(R->svals); F(R->dbits); F(R->sbits); F(R->cz); F(R->cr); F(R->cph);
-  F(R->scz); F(R->scr); F(R->scph); F(R->sq); F(R->gstart); F(R->gkey); F(R->gflag); F(R->sort_tmp);
+  F(R->keys); F(R->skeys); F(R->vals); F(R->cz); F(R->scz); F(R->gstart); F(R->gflag); F(R->sort_tmp);
   F(R->bcount); F(R->boff); F(R->stage); F(R->append_stage); F(R->Pdev); F(R->omega); F(R->red);
   F(R->red_counter); F(R->sor_state); F(R->sor_sums);
   R->live = false;
@@ -119,7 +111,7 @@ int64_t spf_run_footprint(const spf_run_desc *d) {
   int64_t n = d->max_particles > 0 ? d->max_particles : 1;
   int64_t part_bytes = n * (16 * 8 + 4 + 2 + 4);                         // particle SoA
   int64_t event_bytes = n * (4 + 1 + 7 * 8 + 4 + 8);                     // staging, lists
-  int64_t sort_bytes = n * (4 * 4 + 2 + 7 * 8 + 4 + 1) + n * 8 / 4;      // keys, vals, values, groups, temp
+  int64_t sort_bytes = 3 * n * (4 + 4 + 8 + 8 + 4 + 4 + 1) + 3 * n * 8;  // keys, values, segments, sort temp
   int64_t ring = default_srow_cap(d) * (int64_t)sizeof(spf_screen_row) + default_lrow_cap(d) * (int64_t)sizeof(spf_lost_row);
   return field_bytes + part_bytes + event_bytes + sort_bytes + ring;
 }
@@ -133,8 +125,8 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   if (d->n_screens > 0 && (!d->screen_z || !d->screen_direction)) return fail(c, SPF_EINVAL, "screen arrays missing");
   if (d->modes & (SPF_RADIAL_SMEAR | SPF_LONGIT_SMEAR))
     return fail(c, SPF_EUNSUPPORTED, "area-weighted (smeared) deposition is not implemented on the device");
-  if (d->max_particles < 0 || d->max_particles > 0x7fffffffLL) return fail(c, SPF_EINVAL, "max_particles out of range");
-  if ((int64_t)(d->nz + 2) * (d->nr + 2) >= (1LL << 31)) return fail(c, SPF_EINVAL, "grid too large");
+  if (d->max_particles < 0 || d->max_particles > 0x20000000LL) return fail(c, SPF_EINVAL, "max_particles out of range");
+  if ((int64_t)(d->nz + 1) * (d->nr + 1) >= (1LL << 29)) return fail(c, SPF_EINVAL, "grid too large");
   HIPCALL(c, hipSetDevice(c->device));
   Run *R = new (std::nothrow) Run();
   if (!R) return fail(c, SPF_ENOMEM, "host allocation failed");
@@ -164,7 +156,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
     for (int j = 0; j < 2; j++) ALLOC(R->E2[k][j], cells);
   ALLOC(R->Bphi2, cells); ALLOC(R->Br2, cells); ALLOC(R->Bz2, cells);
   ALLOC(R->Jz, cells); ALLOC(R->Jr, cells); ALLOC(R->Jphi, cells);
-  ALLOC(R->Psi, cells); ALLOC(R->Q, cells);
+  ALLOC(R->Psi, cells); ALLOC(R->Q, cells); ALLOC(R->PsiAlt, cells);
   ALLOC(R->EzImp, cells); ALLOC(R->ErImp, cells); ALLOC(R->PsiImp, cells);
   ALLOC(R->BzImp, cells); ALLOC(R->BrImp, cells);
   ALLOC(R->flags, cells);
@@ -182,21 +174,19 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   R->srow_cap = default_srow_cap(d);
   R->lrow_cap = default_lrow_cap(d);
   ALLOC(R->srows, R->srow_cap); ALLOC(R->lrows, R->lrow_cap);
-  R->key_bits = key_bits_for(nz, nr);
-  ALLOC(R->keys, cap); ALLOC(R->skeys, cap); ALLOC(R->vals, cap); ALLOC(R->svals, cap);
-  ALLOC(R->dbits, cap); ALLOC(R->sbits, cap);
-  ALLOC(R->cz, cap); ALLOC(R->cr, cap); ALLOC(R->cph, cap);
-  ALLOC(R->scz, cap); ALLOC(R->scr, cap); ALLOC(R->scph, cap); ALLOC(R->sq, cap);
-  ALLOC(R->gstart, cap); ALLOC(R->gflag, cap);
-  R->sort_tmp_bytes = deposit_sort_bytes(cap, R->key_bits);
+  // ordered scatter-add: three components (Jz, Jr, Jphi) per particle
+  const int64_t cap3 = 3 * cap;
+  ALLOC(R->keys, cap3); ALLOC(R->skeys, cap3); ALLOC(R->cz, cap3); ALLOC(R->scz, cap3);
+  ALLOC(R->vals, cap3); ALLOC(R->gstart, cap3); ALLOC(R->gflag, cap3);
+  R->sort_tmp_bytes = deposit_sort_bytes(cap3, deposit_cell_bits(nz, nr) + 2);
   {
     uint8_t *t = nullptr;
     ALLOC(t, (int64_t)R->sort_tmp_bytes);
     R->sort_tmp = t;
   }
-  R->nblk_cap = compact_blocks(cap) + 1;
+  R->nblk_cap = compact_blocks(cap3) + 1;
   ALLOC(R->bcount, R->nblk_cap); ALLOC(R->boff, R->nblk_cap);
-  R->red_n = std::max<int64_t>(sor_blocks(*R), 512) + 16;
+  R->red_n = std::max<int64_t>(sor_tiles(*R) * sor_batch(), 512) + 16;
   ALLOC(R->red, R->red_n);
   ALLOC(R->red_counter, 4); ALLOC(R->sor_state, 4); ALLOC(R->sor_sums, 8);
   ALLOC(R->Pdev, 16);
@@ -477,8 +467,7 @@ int spf_currents_add_line(spf_ctx *c, int32_t run, int32_t direction, int64_t ip
 int spf_currents_deposit(spf_ctx *c, int32_t run) {
   Run *R = get_run(c, run);
   if (!R) return fail(c, SPF_EINVAL, "bad run id");
-  if (!R->sort_valid) HIPCALL(c, deposit_prepare(*R, c->stream));
-  HIPCALL(c, deposit_fold(*R, c->stream));
+  HIPCALL(c, deposit_currents(*R, c->stream));
   return SPF_OK;
 }
 
@@ -536,7 +525,6 @@ int spf_particles_append(spf_ctx *c, int32_t run, int64_t n, const spf_particles
   HIPCALL(c, hipEventRecord(c->stage_done, c->stream));
   HIPCALL(c, launch_append_kernels(*R, R->append_stage, n, n_active, R->Pdev, c->stream));
   R->np_ub += n;
-  R->sort_valid = false;
   return SPF_OK;
 }
 
@@ -667,7 +655,6 @@ int spf_particles_push(spf_ctx *c, int32_t run, double zForcesStart) {
   w.move_dst = R->move_dst;
   w.overflow = (int32_t *)(R->counters + 8);
   HIPCALL(c, launch_rows(w, c->stream));
-  R->sort_valid = false;
   return SPF_OK;
 }
 

@@ -1,6 +1,7 @@
 // spf_api_poisson.hip -- extern "C" entry points for the Poisson correction,
 // the SOR solve, the imposed electrostatic field and the whole-step call.
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -11,7 +12,7 @@ using namespace spf;
 namespace {
 
 constexpr double PI_D = 3.141592653589793238462643;
-constexpr int64_t kSorBatch = 48;   // half-sweeps queued between convergence checks
+constexpr int64_t kSorCheckBatches = 16;   // batches queued between host checks of the done flag
 
 // omega for half-sweep k: src/poisson.c:47-49 and :121-124
 int ensure_omega(spf_ctx *c, Run *R, int64_t n) {
@@ -36,29 +37,53 @@ int ensure_omega(spf_ctx *c, Run *R, int64_t n) {
   return SPF_OK;
 }
 
-// solve_poisson_cyl, src/poisson.c:17-130; *iters = n on convergence, -1 otherwise
-int sor_solve(spf_ctx *c, Run *R, double *Psi, const double *rhs, double accuracy, int64_t limit, int64_t *iters) {
+// solve_poisson_cyl, src/poisson.c:17-130, on *Psi (a run-owned buffer that
+// may be swapped with the run's spare buffer); *iters = n on convergence, -1 otherwise
+int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accuracy, int64_t limit, int64_t *iters) {
   if (accuracy <= 0) return fail(c, SPF_EINVAL, "accuracy is <= 0 in solve_poisson_cyl");
   if (limit <= 1) return fail(c, SPF_EINVAL, "iteration_limit is <= 1 in solve_poisson_cyl");
   int rc = ensure_omega(c, R, limit);
   if (rc) return rc;
   hipStream_t s = c->stream;
+  const int K = sor_batch();
+  double *buf[2] = {*Psi, R->PsiAlt};
   HIPCALL(c, launch_sor_reset(*R, s));
   if (rhs)
     HIPCALL(c, launch_sum_abs(rhs, R->ldz, R->d.nz, R->d.nr - 1, R->red, R->red_n, R->sor_sums, s));
-  int64_t n = 0;
-  while (n < limit) {
-    int64_t b = limit - n < kSorBatch ? limit - n : kSorBatch;
-    for (int64_t k = 0; k < b; k++) HIPCALL(c, launch_sor_halfsweep(*R, Psi, rhs, n + k, accuracy, s));
+  // batches alternate buffers: batch b reads buf[b%2] and writes buf[(b+1)%2];
+  // once converged, queued batches return at once and leave their input intact
+  int64_t nb = (limit + K - 1) / K, b = 0, converged = -1;
+  while (b < nb) {
+    int64_t stop = std::min<int64_t>(nb, b + kSorCheckBatches);
+    for (; b < stop; b++) {
+      int64_t n0 = b * K;
+      int k = (int)std::min<int64_t>(K, limit - n0);
+      HIPCALL(c, launch_sor_batch(*R, buf[b % 2], buf[(b + 1) % 2], rhs, n0, k, 0, s));
+      HIPCALL(c, launch_sor_check(*R, n0, k, accuracy, s));
+    }
     HIPCALL(c, hipMemcpyAsync(c->pinned, R->sor_state, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
     HIPCALL(c, hipStreamSynchronize(s));
     if (c->pinned[0]) {
-      *iters = c->pinned[1];
-      return SPF_OK;
+      converged = c->pinned[1];
+      break;
     }
-    n += b;
   }
-  *iters = -1;
+  int out;
+  if (converged >= 0) {
+    int64_t bc = converged / K, n0 = bc * K;
+    int k = (int)(converged - n0 + 1);
+    out = (int)((bc + 1) % 2);
+    if (k != std::min<int64_t>(K, limit - n0))   // stopped inside a batch: redo it exactly
+      HIPCALL(c, launch_sor_batch(*R, buf[bc % 2], buf[out], rhs, n0, k, 1, s));
+    *iters = converged;
+  } else {
+    out = (int)(nb % 2);
+    *iters = -1;
+  }
+  if (buf[out] != *Psi) {
+    R->PsiAlt = *Psi;
+    *Psi = buf[out];
+  }
   return SPF_OK;
 }
 
@@ -91,8 +116,8 @@ int spf_poisson_solve(spf_ctx *c, int32_t run, int32_t which, double accuracy, i
   Run *R = get_run(c, run);
   if (!R || (which != 0 && which != 1)) return fail(c, SPF_EINVAL, "bad run or solve target");
   int64_t it = -1;
-  int rc = which == 0 ? sor_solve(c, R, R->Psi, R->Q, accuracy, max_iterations, &it)
-                      : sor_solve(c, R, R->PsiImp, nullptr, accuracy, max_iterations, &it);
+  int rc = which == 0 ? sor_solve(c, R, &R->Psi, R->Q, accuracy, max_iterations, &it)
+                      : sor_solve(c, R, &R->PsiImp, nullptr, accuracy, max_iterations, &it);
   if (rc) return rc;
   if (iterations) *iterations = it;
   return SPF_OK;
@@ -108,10 +133,7 @@ int spf_poisson_correct(spf_ctx *c, int32_t run, int32_t space_charge, int64_t *
   if (p.step_interval <= 0 || R->time_step % p.step_interval != 0 || R->pc_start_step > R->time_step) return SPF_OK;
   hipStream_t s = c->stream;
   HIPCALL(c, launch_gauss_q(*R, s));
-  if (space_charge) {
-    if (!R->sort_valid) HIPCALL(c, deposit_prepare(*R, s));
-    HIPCALL(c, launch_q_subtract(*R, s));
-  }
+  if (space_charge) HIPCALL(c, launch_q_subtract(*R, s));
   if (p.error_charge_threshold > 0) {
     // src/poisson_correction.c:247-265 (Qerror summed in a fixed tree order)
     HIPCALL(c, launch_sum_plain(R->Q, R->ldz, R->d.nz, R->d.nr, R->red, R->red_n, R->sor_sums + 2, s));
@@ -124,7 +146,7 @@ int spf_poisson_correct(spf_ctx *c, int32_t run, int32_t space_charge, int64_t *
   HIPCALL(c, launch_q_to_rhs(*R, s));
   if (p.guess_code == 3) HIPCALL(c, launch_zero_psi(*R, s));
   int64_t it = -1;
-  int rc = sor_solve(c, R, R->Psi, R->Q, p.accuracy, p.maximum_iterations, &it);
+  int rc = sor_solve(c, R, &R->Psi, R->Q, p.accuracy, p.maximum_iterations, &it);
   if (rc) return rc;
   HIPCALL(c, launch_field_from_psi(*R, 0, 1.0, s));
   // recompute the apparent charge for output, :332-439
@@ -144,7 +166,7 @@ int spf_imposed_efield(spf_ctx *c, int32_t run, int32_t initialize, double ramp_
   if (R->pc_start_step == -1) R->pc_start_step = (int64_t)((p.start_time - R->d.start_time) / R->d.dt);
   if (initialize) {
     int64_t it;
-    int rc = sor_solve(c, R, R->PsiImp, nullptr, p.accuracy, p.maximum_iterations, &it);
+    int rc = sor_solve(c, R, &R->PsiImp, nullptr, p.accuracy, p.maximum_iterations, &it);
     if (rc) return rc;
   }
   // ramp, src/poisson_correction.c:504-517

@@ -96,6 +96,7 @@ struct Run {
   double *Bphi2 = nullptr, *Br2 = nullptr, *Bz2 = nullptr;
   double *Jz = nullptr, *Jr = nullptr, *Jphi = nullptr;
   double *Psi = nullptr, *Q = nullptr;
+  double *PsiAlt = nullptr;   // second buffer of the ping-ponged SOR solves
   double *EzImp = nullptr, *ErImp = nullptr, *PsiImp = nullptr, *BzImp = nullptr, *BrImp = nullptr;
   int16_t *flags = nullptr;
   int32_t *cut0 = nullptr, *cut1 = nullptr;
@@ -121,18 +122,14 @@ struct Run {
   spf_screen_row *srows = nullptr;
   spf_lost_row *lrows = nullptr;
   int64_t srow_cap = 0, lrow_cap = 0;
-  // deposition sort state
-  bool sort_valid = false;
+  // ordered scatter-add (deposition and charge removal): up to 3 components per particle
   uint32_t *keys = nullptr, *skeys = nullptr;
-  int32_t *vals = nullptr, *svals = nullptr;
-  uint8_t *dbits = nullptr, *sbits = nullptr;
-  double *cz = nullptr, *cr = nullptr, *cph = nullptr, *scz = nullptr, *scr = nullptr, *scph = nullptr, *sq = nullptr;
-  int32_t *gstart = nullptr;
-  uint32_t *gkey = nullptr;
+  double *cz = nullptr, *scz = nullptr;   // contributions, unsorted and sorted
+  int32_t *vals = nullptr;                // list of long segments
+  int32_t *gstart = nullptr;              // segment starts
   uint8_t *gflag = nullptr;
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
-  int key_bits = 0;
   // compaction scratch
   int32_t *bcount = nullptr, *boff = nullptr;
   int64_t nblk_cap = 0;
@@ -178,9 +175,9 @@ hipError_t ordered_compact(const uint8_t *flags, int64_t n, int32_t *out, int64_
                            int32_t *boff, hipStream_t s);
 int64_t compact_blocks(int64_t n);
 // deposition
-hipError_t deposit_prepare(Run &R, hipStream_t s);   // keys + sort + groups
-hipError_t deposit_fold(Run &R, hipStream_t s);      // J folds
+hipError_t deposit_currents(Run &R, hipStream_t s);   // Jz, Jr, Jphi += particle currents
 size_t deposit_sort_bytes(int64_t n, int key_bits);
+int deposit_cell_bits(int nz, int nr);
 // Poisson
 hipError_t launch_gauss_q(Run &R, hipStream_t s);
 hipError_t launch_q_subtract(Run &R, hipStream_t s);
@@ -189,9 +186,12 @@ hipError_t launch_sum_abs(const double *a, int ldz, int iz_n, int ir_n, double *
                           hipStream_t s);
 hipError_t launch_sum_plain(const double *a, int ldz, int iz_n, int ir_n, double *partials, int64_t pn, double *out,
                             hipStream_t s);
-hipError_t launch_sor_halfsweep(Run &R, double *Psi, const double *rhs, int64_t n, double accuracy, hipStream_t s);
-int64_t sor_blocks(const Run &R);
 hipError_t launch_sor_reset(Run &R, hipStream_t s);
+int64_t sor_tiles(const Run &R);
+int sor_batch();
+hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const double *rhs, int64_t n0, int k, int force,
+                            hipStream_t s);
+hipError_t launch_sor_check(Run &R, int64_t n0, int k, double accuracy, hipStream_t s);
 hipError_t launch_field_from_psi(Run &R, int imposed, double field_factor, hipStream_t s);
 hipError_t launch_zero_psi(Run &R, hipStream_t s);
 

@@ -18,95 +18,146 @@ namespace spf {
 
 namespace {
 
-constexpr int SB = 256;  // SOR block: 64 (iz pairs) x 4 (ir)
+// ---- temporally blocked red-black SOR ----
+// A batch launch performs up to SK consecutive half-sweeps.  Each workgroup
+// owns an STZ x STR tile of points, loads it with an SK-point halo into LDS
+// (clipped at the array edges), and updates colour n%2 for n = n0 .. n0+k-1
+// in LDS.  At local step j only points at least j+1 away from a halo edge
+// that is not an array edge are updated, so after the batch every owned
+// point holds exactly the value the one-colour-per-sweep order produces.
+// Psi is read from one buffer and the owned points written to the other.
+// Each workgroup also sums |residual| over its owned points for every step
+// (fixed tree order) into partials[j][tile]; sor_check_kernel then adds the
+// tiles in order and applies the stopping rule of src/poisson.c:125-126 for
+// each half-sweep of the batch in turn.
+constexpr int STZ = 128, STR = 16, SK = 8;
+constexpr int SLZ = STZ + 2 * SK, SLR = STR + 2 * SK;
+constexpr int SB = 256;
 
 struct SorArgs {
   int nz, nr, ldz, left_dir, right_dir;
-  double a, e, accuracy;
+  double a, e;
   const double *c, *d;   // per-row stencil coefficients
   const double *omega;   // per-half-sweep omega
   const double *rhs;     // may be null
-  double *Psi;
+  const double *Pin;
+  double *Pout;
   const int16_t *flags;
-  double *partials;
-  int32_t *counter;
-  int64_t *state;        // [0] done, [1] iterations
-  double *sums;          // [0] drive sum, [1] residual sum of the last half-sweep
-  int64_t n;
-  int tiles_k, nblocks;
+  double *partials;      // [SK][ntiles]
+  const int64_t *state;  // [0] done
+  int64_t n0;
+  int k;                 // half-sweeps in this batch
+  int force;             // run even if done is set (the exact re-run after convergence)
+  int tiles_z, ntiles;
 };
 
-__global__ __launch_bounds__(SB) void sor_kernel(SorArgs a) {
-  __shared__ double sh[SB];
-  __shared__ int last;
-  if (a.state[0]) return;
-  const int color = (int)(a.n & 1);
-  int tk = blockIdx.x % a.tiles_k, tr = blockIdx.x / a.tiles_k;
-  int k = tk * 64 + (threadIdx.x & 63);
-  int ir = tr * 4 + (threadIdx.x >> 6);
-  double ares = 0;
-  if (ir < a.nr - 1) {
-    int iz = 2 * k + ((color + ir) & 1);   // (iz + ir) % 2 == n % 2
-    bool in_range = (iz >= 1 && iz <= a.nz - 2) || (iz == 0 && a.left_dir && a.nz - 1 > 0) ||
-                    (iz == a.nz - 1 && a.right_dir && iz > 0);
-    if (iz < a.nz && in_range) {
-      int64_t p = fix(a.ldz, iz, ir);
-      if (!(a.flags[p] & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE))) {
-        const double *P = a.Psi;
-        double up = P[fix(a.ldz, iz, ir + 1)];
-        double zs;
-        if (iz == 0)
-          zs = P[fix(a.ldz, 1, ir)] + P[fix(a.ldz, 1, ir)];
-        else if (iz == a.nz - 1)
-          zs = P[fix(a.ldz, iz - 1, ir)] + P[fix(a.ldz, iz - 1, ir)];
-        else
-          zs = P[fix(a.ldz, iz + 1, ir)] + P[fix(a.ldz, iz - 1, ir)];
-        double drive = a.rhs ? a.rhs[p] : 0.0;
-        double c = a.c[ir], residual, omega = a.omega[a.n];
-        if (ir == 0) {
-          residual = a.a * zs + 4 * c * up + (a.e - 2 * c) * P[p] - drive;
-          a.Psi[p] = P[p] - omega * residual / (a.e - 2 * c);
-        } else {
-          double dn = P[fix(a.ldz, iz, ir - 1)];
-          residual = a.a * zs + c * up + a.d[ir] * dn + a.e * P[p] - drive;
-          a.Psi[p] = P[p] - omega * residual / a.e;
-        }
-        ares = fabs(residual);
+__global__ __launch_bounds__(SB) void sor_batch_kernel(SorArgs a) {
+  __shared__ double P[SLR][SLZ];
+  __shared__ double D[SLR][SLZ];
+  __shared__ uint8_t M[SLR][SLZ];
+  __shared__ double red[SB];
+  if (!a.force && a.state[0]) return;
+  const int tile = blockIdx.x;
+  const int z0 = (tile % a.tiles_z) * STZ, r0 = (tile / a.tiles_z) * STR;
+  // owned region covers the whole array [0, nz] x [0, nr] so every value is carried over
+  const int zo1 = min(z0 + STZ, a.nz + 1), ro1 = min(r0 + STR, a.nr + 1);
+  const int zlo = max(z0 - SK, 0), zhi = min(z0 + STZ + SK, a.nz + 1);
+  const int rlo = max(r0 - SK, 0), rhi = min(r0 + STR + SK, a.nr + 1);
+  const int LZ = zhi - zlo, LR = rhi - rlo;
+  for (int t = threadIdx.x; t < LZ * LR; t += SB) {
+    int lr = t / LZ, lz = t % LZ;
+    int iz = zlo + lz, ir = rlo + lr;
+    int64_t p = fix(a.ldz, iz, ir);
+    P[lr][lz] = a.Pin[p];
+    bool in_range = (iz >= 1 && iz <= a.nz - 2) || (iz == 0 && a.left_dir) || (iz == a.nz - 1 && a.right_dir);
+    bool upd = ir <= a.nr - 2 && iz <= a.nz - 1 && in_range && !(a.flags[p] & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE));
+    M[lr][lz] = upd ? 1 : 0;
+    D[lr][lz] = (upd && a.rhs) ? a.rhs[p] : 0.0;
+  }
+  __syncthreads();
+  const double A = a.a, E = a.e;
+  for (int j = 0; j < a.k; j++) {
+    const int64_t n = a.n0 + j;
+    const int color = (int)(n & 1);
+    const double omega = a.omega[n];
+    // shrink only at halo edges that are not array edges
+    const int sz0 = zlo > 0 ? j + 1 : 0, sz1 = zhi < a.nz + 1 ? j + 1 : 0;
+    const int sr0 = rlo > 0 ? j + 1 : 0, sr1 = rhi < a.nr + 1 ? j + 1 : 0;
+    double ares = 0;
+    const int half = LZ / 2 + 1;   // covers both parities of [zlo, zhi)
+    for (int t = threadIdx.x; t < half * LR; t += SB) {
+      int lr = t / half, kk = t % half;
+      int ir = rlo + lr;
+      int iz = 2 * (zlo / 2 + kk) + ((color + ir) & 1);   // (iz + ir) % 2 == n % 2
+      int lz = iz - zlo;
+      if (lz < sz0 || lz >= LZ - sz1 || lr < sr0 || lr >= LR - sr1) continue;
+      if (!M[lr][lz]) continue;
+      double zs;
+      if (iz == 0)
+        zs = P[lr][lz + 1] + P[lr][lz + 1];
+      else if (iz == a.nz - 1)
+        zs = P[lr][lz - 1] + P[lr][lz - 1];
+      else
+        zs = P[lr][lz + 1] + P[lr][lz - 1];
+      double up = P[lr + 1][lz], p0 = P[lr][lz], c = a.c[ir], residual;
+      if (ir == 0) {
+        residual = A * zs + 4 * c * up + (E - 2 * c) * p0 - D[lr][lz];
+        P[lr][lz] = p0 - omega * residual / (E - 2 * c);
+      } else {
+        residual = A * zs + c * up + a.d[ir] * P[lr - 1][lz] + E * p0 - D[lr][lz];
+        P[lr][lz] = p0 - omega * residual / E;
       }
+      if (iz >= z0 && iz < zo1 && ir >= r0 && ir < ro1) ares += fabs(residual);
     }
-  }
-  // fixed-order block sum
-  sh[threadIdx.x] = ares;
-  __syncthreads();
-  for (int off = SB / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + off];
+    // fixed-order block sum of the owned residuals of this half-sweep
+    red[threadIdx.x] = ares;
+    __syncthreads();
+    for (int off = SB / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) a.partials[(int64_t)j * a.ntiles + tile] = red[0];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    a.partials[blockIdx.x] = sh[0];
-    __threadfence();
-    int t = atomicAdd(a.counter, 1);
-    last = (t == a.nblocks - 1);
+  for (int t = threadIdx.x; t < (zo1 - z0) * (ro1 - r0); t += SB) {
+    int w = zo1 - z0;
+    int iz = z0 + t % w, ir = r0 + t / w;
+    a.Pout[fix(a.ldz, iz, ir)] = P[ir - rlo][iz - zlo];
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  volatile const double *vp = a.partials;
-  double s = 0;
-  for (int i = threadIdx.x; i < a.nblocks; i += SB) s += vp[i];
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = SB / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + off];
+}
+
+struct CheckArgs {
+  const double *partials;
+  int ntiles, k;
+  int64_t n0;
+  double accuracy;
+  int64_t *state;   // [0] done, [1] iterations
+  double *sums;     // [0] drive sum, [1] residual of the last checked half-sweep
+};
+
+// applies the stopping rule after each half-sweep of a batch, in order
+__global__ __launch_bounds__(SB) void sor_check_kernel(CheckArgs a) {
+  __shared__ double red[SB];
+  if (a.state[0]) return;
+  for (int j = 0; j < a.k; j++) {
+    double s = 0;
+    for (int t = threadIdx.x; t < a.ntiles; t += SB) s += a.partials[(int64_t)j * a.ntiles + t];
+    red[threadIdx.x] = s;
     __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    double total = sh[0];
-    *a.counter = 0;
-    a.sums[1] = total;
-    if (a.n > 0 && (total < a.accuracy * a.sums[0] || total == 0)) {
-      a.state[0] = 1;
-      a.state[1] = a.n;
+    for (int off = SB / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + off];
+      __syncthreads();
+    }
+    double total = red[0];
+    __syncthreads();
+    int64_t n = a.n0 + j;
+    if (threadIdx.x == 0) a.sums[1] = total;
+    if (n > 0 && (total < a.accuracy * a.sums[0] || total == 0)) {
+      if (threadIdx.x == 0) {
+        a.state[0] = 1;
+        a.state[1] = n;
+      }
+      return;
     }
   }
 }
@@ -254,13 +305,15 @@ hipError_t launch_sor_reset(Run &R, hipStream_t s) {
   return hipGetLastError();
 }
 
-int64_t sor_blocks(const Run &R) {
-  int tiles_k = ((R.d.nz + 1) / 2 + 63) / 64;
-  int tiles_r = (R.d.nr - 1 + 3) / 4;
-  return (int64_t)tiles_k * tiles_r;
+int64_t sor_tiles(const Run &R) {
+  int64_t tz = (R.d.nz + 1 + STZ - 1) / STZ, tr = (R.d.nr + 1 + STR - 1) / STR;
+  return tz * tr;
 }
 
-hipError_t launch_sor_halfsweep(Run &R, double *Psi, const double *rhs, int64_t n, double accuracy, hipStream_t s) {
+int sor_batch() { return SK; }
+
+hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const double *rhs, int64_t n0, int k, int force,
+                            hipStream_t s) {
   SorArgs a;
   a.nz = R.d.nz;
   a.nr = R.d.nr;
@@ -270,21 +323,34 @@ hipError_t launch_sor_halfsweep(Run &R, double *Psi, const double *rhs, int64_t 
   double dz = R.d.dz, dr = R.d.dr;
   a.a = 1 / (dz * dz);
   a.e = -2 * (1 / (dz * dz) + 1 / (dr * dr));
-  a.accuracy = accuracy;
   a.c = R.sor_c;
   a.d = R.sor_d;
   a.omega = R.omega;
   a.rhs = rhs;
-  a.Psi = Psi;
+  a.Pin = Pin;
+  a.Pout = Pout;
   a.flags = R.flags;
   a.partials = R.red;
-  a.counter = R.red_counter;
   a.state = R.sor_state;
-  a.sums = R.sor_sums;
-  a.n = n;
-  a.tiles_k = ((R.d.nz + 1) / 2 + 63) / 64;
-  a.nblocks = (int)sor_blocks(R);
-  hipLaunchKernelGGL(sor_kernel, dim3((unsigned)a.nblocks), dim3(SB), 0, s, a);
+  a.n0 = n0;
+  a.k = k;
+  a.force = force;
+  a.tiles_z = (R.d.nz + 1 + STZ - 1) / STZ;
+  a.ntiles = (int)sor_tiles(R);
+  hipLaunchKernelGGL(sor_batch_kernel, dim3((unsigned)a.ntiles), dim3(SB), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_sor_check(Run &R, int64_t n0, int k, double accuracy, hipStream_t s) {
+  CheckArgs c;
+  c.partials = R.red;
+  c.ntiles = (int)sor_tiles(R);
+  c.k = k;
+  c.n0 = n0;
+  c.accuracy = accuracy;
+  c.state = R.sor_state;
+  c.sums = R.sor_sums;
+  hipLaunchKernelGGL(sor_check_kernel, dim3(1), dim3(SB), 0, s, c);
   return hipGetLastError();
 }
 

@@ -267,6 +267,28 @@ def test_sor_parallel_plate_imposed_field(ctx):
     d.close()
 
 
+@pytest.mark.parametrize("limit,acc", [(2, 1e-3), (5, 1e-12), (8, 1e-12), (9, 1e-12), (17, 1e-12), (131, 1e-12),
+                                       (3000, 1e-6), (3000, 1e-9)])
+def test_sor_solve_counts_and_bits(ctx, limit, acc):
+    """solve_poisson_cyl: same iteration count (or -1) and the same Psi bits, whether the stop
+    falls inside a batch of half-sweeps, on its last half-sweep, or at the iteration limit."""
+    g = gun_grid(modes=abi.TM_FIELDS | abi.RADIAL_INTERP | abi.LONGIT_INTERP)
+    g.left_bc = abi.DIRICHLET
+    f = gun_flags(g)
+    o, d = pair(ctx, g, f)
+    rng = np.random.default_rng(limit)
+    psi0 = rng.standard_normal(g.shape)
+    rhs = rng.standard_normal(g.shape) * 1e3
+    for obj in (o, d):
+        obj.set_field("Psi", psi0) if obj is o else obj.upload("Psi", psi0)
+        obj.set_field("Q", rhs) if obj is o else obj.upload("Q", rhs)
+    it_o, _ = o.solve_poisson_cyl("Psi", "Q", acc, limit)
+    it_d = d.solve_poisson_cyl(0, acc, limit)
+    assert it_o == it_d
+    assert_fields(o, d, ["Psi"])
+    d.close()
+
+
 def test_two_runs_one_context_match_alone(ctx):
     """Runs sharing a context and interleaving their steps give the bits each gives alone."""
     g = gun_grid(modes=TMTE | abi.SPACE_CHARGE)
