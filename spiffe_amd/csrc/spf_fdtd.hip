@@ -6,12 +6,14 @@
 // then patches boundary rows (src/integrate.c:277-541).  Every sub-update is a
 // pure stencil of the previous sub-update's output, so the final value of each
 // array element is a closed function of the time-level-n arrays.  This kernel
-// evaluates that function per point: E and B are ping-ponged (read level n,
-// write level n+1), each thread recomputes the few neighbouring E values its
-// B update needs, and the reference's loop `break` after FL_LAST_ACTIVE_PT is
-// reproduced with per-row cut-offs (cut1 for loops starting at iz=1, cut0 for
-// iz=0).  Arithmetic repeats the reference's operation order; the library is
-// built with -ffp-contract=off so no FMA changes a rounding.
+// evaluates that function per point.  E and B are ping-ponged (read level n,
+// write level n+1).  Each workgroup owns a 64 x 16 tile: it first computes the
+// final Ez, Er and Ephi of the tile and a 2-point margin into LDS, then the B
+// updates and boundary patches of the tile from those values.  The reference's
+// loop `break` after FL_LAST_ACTIVE_PT is reproduced with per-row cut-offs
+// (cut1 for loops starting at iz=1, cut0 for iz=0).  Arithmetic repeats the
+// reference's operation order; the library is built with -ffp-contract=off so
+// no FMA changes a rounding.
 #include "spf_internal.h"
 
 namespace spf {
@@ -213,32 +215,175 @@ __device__ inline int xcd_band_block(int bid, int nblocks) {
   return logical;
 }
 
+// Tiled two-phase step: phase 1 evaluates the final Ez, Er, Ephi of every
+// point of the tile plus a 2-point margin into LDS (each once); phase 2 does
+// the B updates and boundary patches of the tile from those LDS values.  The
+// margin covers every neighbour a B update or patch refers to.
+constexpr int FTZ = 64, FTR = 16, FM = 2;
+constexpr int FW = FTZ + 2 * FM, FH = FTR + 2 * FM;
+
+struct FL {
+  const FdtdArgs &a;
+  const double (*ez)[FW], (*er)[FW], (*ephi)[FW];
+  int zb, rb;
+  __device__ int64_t ix(int iz, int ir) const { return fix(a.ldz, iz, ir); }
+  __device__ double ez_final(int iz, int ir) const { return ez[ir - rb][iz - zb]; }
+  __device__ double er_final(int iz, int ir) const { return er[ir - rb][iz - zb]; }
+  __device__ double ephi_final(int iz, int ir) const { return ephi[ir - rb][iz - zb]; }
+  __device__ bool act1(int iz, int ir, int16_t bit) const {
+    return !(a.flags[ix(iz, ir)] & bit) && iz <= a.cut1[ir];
+  }
+  __device__ bool act0(int iz, int ir, int16_t bit) const {
+    return !(a.flags[ix(iz, ir)] & bit) && iz <= a.cut0[ir];
+  }
+  __device__ void bphi_upd(int iz, int ir, double &b, double &b2) const {
+    int64_t p = ix(iz, ir);
+    b = a.Bphi[p];
+    b2 = a.Bphi2[p];
+    if (act1(iz, ir, BPHI_ZERO)) {
+      double dB = a.dti * ((ez_final(iz, ir) - ez_final(iz, ir - 1)) / a.dr - (er_final(iz, ir) - er_final(iz - 1, ir)) / a.dz);
+      b2 = b + dB / 2;
+      b = b + dB;
+    }
+  }
+  __device__ void bphi_inner(int iz, int ir, double &b, double &b2) const {
+    if (ir == 0) {
+      bphi_upd(iz, 1, b, b2);
+      b = -b;
+      b2 = -b2;
+    } else if (ir == a.nr - 1) {
+      bphi_upd(iz, a.nr - 2, b, b2);
+      b = b * a.top_factor;
+      b2 = b2 * a.top_factor;
+    } else if (ir == a.nr) {
+      int64_t p = ix(iz, ir);
+      b = a.Bphi[p];
+      b2 = a.Bphi2[p];
+    } else {
+      bphi_upd(iz, ir, b, b2);
+    }
+  }
+  __device__ void bphi_final(int iz, int ir, double &b, double &b2) const {
+    if (iz == 0) {
+      bphi_inner(1, ir, b, b2);
+      if (!a.left_neu) { b = -b; b2 = -b2; }
+    } else if (iz == a.nz) {
+      bphi_inner(a.nz - 1, ir, b, b2);
+      if (!a.right_neu) { b = -b; b2 = -b2; }
+    } else {
+      bphi_inner(iz, ir, b, b2);
+    }
+  }
+  __device__ void br_upd(int iz, int ir, double &b, double &b2) const {
+    int64_t p = ix(iz, ir);
+    b = a.Br[p];
+    b2 = a.Br2[p];
+    if (act1(iz, ir, BR_ZERO)) {
+      double dB = a.dti * (ephi_final(iz, ir) - ephi_final(iz - 1, ir)) / a.dz;
+      b2 = b + dB / 2;
+      b = b + dB;
+    }
+  }
+  __device__ void br_inner(int iz, int ir, double &b, double &b2) const {
+    if (ir == a.nr) {
+      int64_t p = ix(iz, ir);
+      b = a.Br[p];
+      b2 = a.Br2[p];
+    } else {
+      br_upd(iz, ir, b, b2);
+    }
+  }
+  __device__ void br_final(int iz, int ir, double &b, double &b2) const {
+    if (iz == a.nz) {
+      br_inner(a.nz - 1, ir, b, b2);
+      if (!a.right_neu) { b = -b; b2 = -b2; }
+    } else if (iz == 0) {
+      br_inner(1, ir, b, b2);
+      if (!a.left_neu) { b = -b; b2 = -b2; }
+    } else {
+      br_inner(iz, ir, b, b2);
+    }
+  }
+  __device__ void bz_upd(int iz, int ir, double &b, double &b2) const {
+    int64_t p = ix(iz, ir);
+    b = a.Bz[p];
+    b2 = a.Bz2[p];
+    if (act0(iz, ir, BZ_ZERO)) {
+      double dB = a.bzf[ir] * ((double)(ir + 1) * ephi_final(iz, ir) - (double)ir * ephi_final(iz, ir - 1));
+      b2 = b + dB / 2;
+      b = b + dB;
+    }
+  }
+  __device__ void bz_final(int iz, int ir, double &b, double &b2) const {
+    if ((a.right_neu && iz == a.nz - 1) || (a.left_neu && iz == 0)) {
+      b = 0.0;
+      b2 = 0.0;
+      return;
+    }
+    if (ir == 0) {
+      if (iz == a.nz) {
+        int64_t p = ix(iz, 1);
+        b = a.Bz[p];
+        b2 = a.Bz2[p];
+      } else {
+        bz_upd(iz, 1, b, b2);
+      }
+      return;
+    }
+    if (iz == a.nz || ir == a.nr) {
+      int64_t p = ix(iz, ir);
+      b = a.Bz[p];
+      b2 = a.Bz2[p];
+      return;
+    }
+    bz_upd(iz, ir, b, b2);
+  }
+};
+
 __global__ __launch_bounds__(256) void fdtd_kernel(FdtdArgs a, int tiles_z, int nblocks) {
+  __shared__ double sez[FH][FW], ser[FH][FW], sephi[FH][FW];
   int logical = xcd_band_block(blockIdx.x, nblocks);
   if (logical >= nblocks) return;
-  int tz = logical % tiles_z, tr = logical / tiles_z;
-  int iz = tz * 64 + (threadIdx.x & 63);
-  int ir = tr * 4 + (threadIdx.x >> 6);
-  if (iz > a.nz || ir > a.nr) return;
+  const int z0 = (logical % tiles_z) * FTZ, r0 = (logical / tiles_z) * FTR;
+  const int zb = z0 - FM, rb = r0 - FM;
   F f{a};
-  int64_t p = f.ix(iz, ir);
-  if (a.tm) {
-    a.Ezn[p] = f.ez_final(iz, ir);
-    a.Ern[p] = f.er_final(iz, ir);
-    double b, b2;
-    f.bphi_final(iz, ir, b, b2);
-    a.Bphin[p] = b;
-    a.Bphi2[p] = b2;
+  // phase 1: final E over the tile and its margin (points inside the array only)
+  for (int t = threadIdx.x; t < FW * FH; t += 256) {
+    int lr = t / FW, lz = t % FW;
+    int iz = zb + lz, ir = rb + lr;
+    if (iz < 0 || iz > a.nz || ir < 0 || ir > a.nr) continue;
+    if (a.tm) {
+      sez[lr][lz] = f.ez_final(iz, ir);
+      ser[lr][lz] = f.er_final(iz, ir);
+    }
+    if (a.te) sephi[lr][lz] = f.ephi_final(iz, ir);
   }
-  if (a.te) {
-    a.Ephin[p] = f.ephi_final(iz, ir);
-    double b, b2;
-    f.br_final(iz, ir, b, b2);
-    a.Brn[p] = b;
-    a.Br2[p] = b2;
-    f.bz_final(iz, ir, b, b2);
-    a.Bzn[p] = b;
-    a.Bz2[p] = b2;
+  __syncthreads();
+  FL g{a, sez, ser, sephi, zb, rb};
+  const int iz = z0 + (threadIdx.x & 63);
+  for (int k = threadIdx.x >> 6; k < FTR; k += 4) {
+    const int ir = r0 + k;
+    if (iz > a.nz || ir > a.nr) continue;
+    const int64_t p = fix(a.ldz, iz, ir);
+    const int lz = iz - zb, lr = ir - rb;
+    if (a.tm) {
+      a.Ezn[p] = sez[lr][lz];
+      a.Ern[p] = ser[lr][lz];
+      double b, b2;
+      g.bphi_final(iz, ir, b, b2);
+      a.Bphin[p] = b;
+      a.Bphi2[p] = b2;
+    }
+    if (a.te) {
+      a.Ephin[p] = sephi[lr][lz];
+      double b, b2;
+      g.br_final(iz, ir, b, b2);
+      a.Brn[p] = b;
+      a.Br2[p] = b2;
+      g.bz_final(iz, ir, b, b2);
+      a.Bzn[p] = b;
+      a.Bz2[p] = b2;
+    }
   }
 }
 
@@ -294,7 +439,7 @@ __global__ void add_line_kernel(double *J, int ldz, int direction, int64_t ipos,
 }  // namespace
 
 hipError_t launch_fdtd(const FdtdArgs &a, hipStream_t s) {
-  int tiles_z = (a.nz + 1 + 63) / 64, tiles_r = (a.nr + 1 + 3) / 4;
+  int tiles_z = (a.nz + 1 + FTZ - 1) / FTZ, tiles_r = (a.nr + 1 + FTR - 1) / FTR;
   int nblocks = tiles_z * tiles_r;
   int grid = ((nblocks + 7) / 8) * 8;
   hipLaunchKernelGGL(fdtd_kernel, dim3(grid), dim3(256), 0, s, a, tiles_z, nblocks);
