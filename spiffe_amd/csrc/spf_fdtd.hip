@@ -340,12 +340,120 @@ struct FL {
   }
 };
 
-__global__ __launch_bounds__(256) void fdtd_kernel(FdtdArgs a, int tiles_z, int nblocks) {
+// Interior fast path.  Inside the box z in [2, nz-2], r in [2, nr-3] no
+// boundary patch applies, so every final value is the plain loop body.  One
+// wavefront owns a strip of 63 columns (lane 0 is the z-1 halo) and walks it
+// upward in r one row at a time: the previous row's new Ez and Ephi stay in
+// registers, the z-1 neighbour's new Er and Ephi come from the lane below,
+// and the z+1 neighbour's old Bphi and Br from the lane above, so each array
+// element is read from memory once.
+constexpr int QZ = 63, QR = 32;
+
+struct Box {
+  int zi0, zi1, ri0, ri1;   // interior box [zi0, zi1) x [ri0, ri1)
+  int strips_z, nstrips;
+};
+
+__device__ inline double shfl_dn1(double v) { return __shfl_down(v, 1, 64); }
+__device__ inline double shfl_up1(double v) { return __shfl_up(v, 1, 64); }
+
+__global__ __launch_bounds__(256) void fdtd_interior_kernel(FdtdArgs a, Box bx, int nblocks) {
+  int logical = xcd_band_block(blockIdx.x, nblocks);
+  if (logical >= nblocks) return;
+  const int lane = threadIdx.x & 63;
+  const int strip = logical * 4 + (threadIdx.x >> 6);
+  if (strip >= bx.nstrips) return;
+  const int z0 = bx.zi0 + (strip % bx.strips_z) * QZ, r0 = bx.ri0 + (strip / bx.strips_z) * QR;
+  const int r1 = min(r0 + QR, bx.ri1);
+  const int iz = z0 - 1 + lane;                    // lane 0: halo column z0-1 >= 1
+  const bool out_col = lane > 0 && iz < bx.zi1;
+  const int izl = min(iz, a.nz - 1);               // load index (columns past the box are not written)
+  const double dti = a.dti, eps0 = a.eps0, dz = a.dz, dr = a.dr;
+  const int ldz = a.ldz;
+  double ez_prev = 0, ephi_prev = 0;
+  double bphi_lo = 0, bz_lo = 0;
+  if (a.tm) bphi_lo = a.Bphi[fix(ldz, izl, r0 - 1)];
+  if (a.te) bz_lo = a.Bz[fix(ldz, izl, r0 - 1)];
+  for (int ir = r0 - 1; ir < r1; ir++) {
+    const int64_t p = fix(ldz, izl, ir);
+    const int16_t fl = a.flags[p];
+    const int c0 = a.cut0[ir], c1 = a.cut1[ir];
+    const bool out = out_col && ir >= r0;
+    double ez_n = 0, er_n = 0, ephi_n = 0;
+    if (a.tm) {
+      const double bphi_c = bphi_lo, bphi_hi = a.Bphi[p + ldz];
+      double bphi_zp = shfl_dn1(bphi_c);
+      if (lane == 63) bphi_zp = a.Bphi[p + 1];
+      ez_n = a.Ez[p];
+      if (!(fl & EZ_ZERO) && izl <= c1)
+        ez_n += dti * (-a.Jz[p] / eps0 + (a.cp[ir] * bphi_hi - a.cm[ir] * bphi_c));
+      er_n = a.Er[p];
+      if (!(fl & ER_ZERO) && izl <= c0)
+        er_n += dti * (-a.Jr[p] / eps0 - a.er_factor * (bphi_zp - bphi_c));
+      const double er_zm = shfl_up1(er_n);
+      if (out) {
+        double b = bphi_c, b2;
+        if (!(fl & BPHI_ZERO) && izl <= c1) {
+          double dB = dti * ((ez_n - ez_prev) / dr - (er_n - er_zm) / dz);
+          b2 = b + dB / 2;
+          b = b + dB;
+        } else {
+          b2 = a.Bphi2[p];
+        }
+        a.Ezn[p] = ez_n;
+        a.Ern[p] = er_n;
+        a.Bphin[p] = b;
+        a.Bphi2[p] = b2;
+      }
+      bphi_lo = bphi_hi;
+    }
+    if (a.te) {
+      const double bz_c = bz_lo, bz_hi = a.Bz[p + ldz];
+      const double br_c = a.Br[p];
+      double br_zp = shfl_dn1(br_c);
+      if (lane == 63) br_zp = a.Br[p + 1];
+      ephi_n = a.Ephi[p];
+      if (!(fl & EPHI_ZERO) && izl <= c0)
+        ephi_n += dti * (-a.Jphi[p] / eps0 + a.ephi_factor * ((br_zp - br_c) / dz - (bz_hi - bz_c) / dr));
+      const double ephi_zm = shfl_up1(ephi_n);
+      if (out) {
+        double b = br_c, b2;
+        if (!(fl & BR_ZERO) && izl <= c1) {
+          double dB = dti * (ephi_n - ephi_zm) / dz;
+          b2 = b + dB / 2;
+          b = b + dB;
+        } else {
+          b2 = a.Br2[p];
+        }
+        a.Ephin[p] = ephi_n;
+        a.Brn[p] = b;
+        a.Br2[p] = b2;
+        double c = bz_c, c2;
+        if (!(fl & BZ_ZERO) && izl <= c0) {
+          double dB = a.bzf[ir] * ((double)(ir + 1) * ephi_n - (double)ir * ephi_prev);
+          c2 = c + dB / 2;
+          c = c + dB;
+        } else {
+          c2 = a.Bz2[p];
+        }
+        a.Bzn[p] = c;
+        a.Bz2[p] = c2;
+      }
+      bz_lo = bz_hi;
+    }
+    ez_prev = ez_n;
+    ephi_prev = ephi_n;
+  }
+}
+
+__global__ __launch_bounds__(256) void fdtd_kernel(FdtdArgs a, int tiles_z, int nblocks, Box bx) {
   __shared__ double sez[FH][FW], ser[FH][FW], sephi[FH][FW];
   int logical = xcd_band_block(blockIdx.x, nblocks);
   if (logical >= nblocks) return;
   const int z0 = (logical % tiles_z) * FTZ, r0 = (logical / tiles_z) * FTR;
   const int zb = z0 - FM, rb = r0 - FM;
+  // tiles entirely inside the interior box are done by fdtd_interior_kernel
+  if (z0 >= bx.zi0 && z0 + FTZ <= bx.zi1 && r0 >= bx.ri0 && r0 + FTR <= bx.ri1) return;
   F f{a};
   // phase 1: final E over the tile and its margin (points inside the array only)
   for (int t = threadIdx.x; t < FW * FH; t += 256) {
@@ -364,6 +472,7 @@ __global__ __launch_bounds__(256) void fdtd_kernel(FdtdArgs a, int tiles_z, int 
   for (int k = threadIdx.x >> 6; k < FTR; k += 4) {
     const int ir = r0 + k;
     if (iz > a.nz || ir > a.nr) continue;
+    if (iz >= bx.zi0 && iz < bx.zi1 && ir >= bx.ri0 && ir < bx.ri1) continue;
     const int64_t p = fix(a.ldz, iz, ir);
     const int lz = iz - zb, lr = ir - rb;
     if (a.tm) {
@@ -439,10 +548,25 @@ __global__ void add_line_kernel(double *J, int ldz, int direction, int64_t ipos,
 }  // namespace
 
 hipError_t launch_fdtd(const FdtdArgs &a, hipStream_t s) {
+  Box bx;
+  bx.zi0 = 2;
+  bx.zi1 = a.nz - 1;   // z in [2, nz-2]
+  bx.ri0 = 2;
+  bx.ri1 = a.nr - 2;   // r in [2, nr-3]
+  if (bx.zi1 <= bx.zi0 || bx.ri1 <= bx.ri0) bx.zi1 = bx.zi0 = bx.ri1 = bx.ri0 = 0;
+  bx.strips_z = (bx.zi1 - bx.zi0 + QZ - 1) / QZ;
+  bx.nstrips = bx.strips_z * ((bx.ri1 - bx.ri0 + QR - 1) / QR);
+  if (bx.nstrips > 0) {
+    int nb = (bx.nstrips + 3) / 4;
+    int grid = ((nb + 7) / 8) * 8;
+    hipLaunchKernelGGL(fdtd_interior_kernel, dim3(grid), dim3(256), 0, s, a, bx, nb);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   int tiles_z = (a.nz + 1 + FTZ - 1) / FTZ, tiles_r = (a.nr + 1 + FTR - 1) / FTR;
   int nblocks = tiles_z * tiles_r;
   int grid = ((nblocks + 7) / 8) * 8;
-  hipLaunchKernelGGL(fdtd_kernel, dim3(grid), dim3(256), 0, s, a, tiles_z, nblocks);
+  hipLaunchKernelGGL(fdtd_kernel, dim3(grid), dim3(256), 0, s, a, tiles_z, nblocks, bx);
   return hipGetLastError();
 }
 

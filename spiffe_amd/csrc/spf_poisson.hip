@@ -32,7 +32,21 @@ namespace {
 // each half-sweep of the batch in turn.
 constexpr int STZ = 128, STR = 16, SK = 8;
 constexpr int SLZ = STZ + 2 * SK, SLR = STR + 2 * SK;
-constexpr int SB = 256;
+constexpr int SB = 1024;   // SOR batch workgroup: 16 wavefronts share one LDS tile
+constexpr int CB = 256;    // check kernel
+
+// fixed-order sum over a workgroup: butterfly within each wavefront, then the
+// wavefront sums in order (the same operations every launch)
+__device__ double block_sum_det(double v, double *wsum) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) wsum[w] = v;
+  __syncthreads();
+  double t = 0;
+  for (int k = 0; k < nw; k++) t += wsum[k];
+  __syncthreads();
+  return t;
+}
 
 struct SorArgs {
   int nz, nr, ldz, left_dir, right_dir;
@@ -55,7 +69,7 @@ __global__ __launch_bounds__(SB) void sor_batch_kernel(SorArgs a) {
   __shared__ double P[SLR][SLZ];
   __shared__ double D[SLR][SLZ];
   __shared__ uint8_t M[SLR][SLZ];
-  __shared__ double red[SB];
+  __shared__ double wsum[SB / 64];
   if (!a.force && a.state[0]) return;
   const int tile = blockIdx.x;
   const int z0 = (tile % a.tiles_z) * STZ, r0 = (tile / a.tiles_z) * STR;
@@ -110,14 +124,8 @@ __global__ __launch_bounds__(SB) void sor_batch_kernel(SorArgs a) {
       if (iz >= z0 && iz < zo1 && ir >= r0 && ir < ro1) ares += fabs(residual);
     }
     // fixed-order block sum of the owned residuals of this half-sweep
-    red[threadIdx.x] = ares;
-    __syncthreads();
-    for (int off = SB / 2; off > 0; off >>= 1) {
-      if (threadIdx.x < off) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + off];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) a.partials[(int64_t)j * a.ntiles + tile] = red[0];
-    __syncthreads();
+    double tot = block_sum_det(ares, wsum);
+    if (threadIdx.x == 0) a.partials[(int64_t)j * a.ntiles + tile] = tot;
   }
   for (int t = threadIdx.x; t < (zo1 - z0) * (ro1 - r0); t += SB) {
     int w = zo1 - z0;
@@ -135,28 +143,26 @@ struct CheckArgs {
   double *sums;     // [0] drive sum, [1] residual of the last checked half-sweep
 };
 
-// applies the stopping rule after each half-sweep of a batch, in order
-__global__ __launch_bounds__(SB) void sor_check_kernel(CheckArgs a) {
-  __shared__ double red[SB];
+// applies the stopping rule after each half-sweep of a batch, in order: 32
+// lanes per half-sweep add the tile partials (strided, then a fixed butterfly)
+__global__ __launch_bounds__(CB) void sor_check_kernel(CheckArgs a) {
+  __shared__ double tot[CB / 32];
   if (a.state[0]) return;
-  for (int j = 0; j < a.k; j++) {
-    double s = 0;
-    for (int t = threadIdx.x; t < a.ntiles; t += SB) s += a.partials[(int64_t)j * a.ntiles + t];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = SB / 2; off > 0; off >>= 1) {
-      if (threadIdx.x < off) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + off];
-      __syncthreads();
-    }
-    double total = red[0];
-    __syncthreads();
-    int64_t n = a.n0 + j;
-    if (threadIdx.x == 0) a.sums[1] = total;
+  const int j = threadIdx.x >> 5, l = threadIdx.x & 31;
+  double s = 0;
+  if (j < a.k)
+    for (int t = l; t < a.ntiles; t += 32) s += a.partials[(int64_t)j * a.ntiles + t];
+  for (int off = 16; off > 0; off >>= 1) s += __shfl_xor(s, off, 32);
+  if (l == 0) tot[j] = s;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int jj = 0; jj < a.k; jj++) {
+    int64_t n = a.n0 + jj;
+    double total = tot[jj];
+    a.sums[1] = total;
     if (n > 0 && (total < a.accuracy * a.sums[0] || total == 0)) {
-      if (threadIdx.x == 0) {
-        a.state[0] = 1;
-        a.state[1] = n;
-      }
+      a.state[0] = 1;
+      a.state[1] = n;
       return;
     }
   }
@@ -350,7 +356,7 @@ hipError_t launch_sor_check(Run &R, int64_t n0, int k, double accuracy, hipStrea
   c.accuracy = accuracy;
   c.state = R.sor_state;
   c.sums = R.sor_sums;
-  hipLaunchKernelGGL(sor_check_kernel, dim3(1), dim3(SB), 0, s, c);
+  hipLaunchKernelGGL(sor_check_kernel, dim3(1), dim3(CB), 0, s, c);
   return hipGetLastError();
 }
 
