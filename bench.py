@@ -115,6 +115,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    from spiffe_amd import workloads as W
     from spiffe_amd.device import Context
     ctx = Context(0)
     g, f, run = build_run(ctx, args.particles)
@@ -136,36 +137,40 @@ def main():
     pushes = 0.5 * (np0 + np1) * args.steps
     ms_per_step = 1e3 * wall / args.steps
 
-    # per-phase kernel timing with HIP events on the library stream
+    # per-kernel timing with HIP events on the library stream
     reps = 50
-    t_fdtd = timer.time_ms(stream, run.advance_field_solutions, reps) / reps
     npk, _ = run.count()
-    t_push = timer.time_ms(stream, run.advance_particles, reps) / reps
+    t_fdtd = timer.time_ms(stream, run.advance_field_solutions, reps) / reps          # one fdtd_kernel launch
+    t_sor = timer.time_ms(stream, lambda: run.poisson_sweeps(0, 64, 8), reps) / reps   # one sor_batch_kernel launch
+    t_push = timer.time_ms(stream, run.advance_particles, reps) / reps                 # push + event compaction
     t_dep = timer.time_ms(stream, lambda: (run.zero_current_arrays(), run.add_particle_currents()), reps) / reps
     run.set_step(0)
     t_pc0 = time.perf_counter()
     it = run.add_poisson_correction(True)
     t_pc = (time.perf_counter() - t_pc0) * 1e3
-    per_step = {"fdtd_ms": t_fdtd, "push_ms": t_push, "deposit_ms": t_dep, "poisson_ms_per_step": t_pc / 10}
-    phases = {"fdtd": t_fdtd, "push": t_push, "deposit": t_dep, "poisson": t_pc / 10}
-    dominant = max(phases, key=phases.get)
-    if dominant == "fdtd":
-        ach = cells * B_CELL_TMTE / (t_fdtd * 1e-3) / 1e9
-        rl = {"bound": "hbm", "kernel": "fdtd_kernel", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-              "frac": ach / HBM_PEAK_GBS, "traffic": None,
-              "algorithmic_bytes_per_launch": cells * B_CELL_TMTE}
-    elif dominant == "push":
-        ach = npk * B_PUSH / (t_push * 1e-3) / 1e9
-        rl = {"bound": "hbm", "kernel": "push phase (push_kernel + event compaction)", "achieved": ach,
-              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
-              "algorithmic_bytes_per_launch": npk * B_PUSH}
-    else:
-        ach = None
-        rl = {"bound": "hbm", "kernel": dominant, "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-              "frac": None, "traffic": None}
-    # FDTD roofline is always reported beside the dominant one
+    half_sweeps = it + 1 if it >= 0 else W.GUN_POISSON["maximum_iterations"]
+    interval = W.GUN_POISSON["step_interval"]
+    per_step = {"fdtd_ms": t_fdtd, "push_ms": t_push, "deposit_ms": t_dep, "sor_batch_ms": t_sor,
+                "poisson_ms_per_correction": t_pc, "poisson_ms_per_step": t_pc / interval}
+    # time per step spent in each kernel
+    share = {"fdtd_kernel": t_fdtd, "push phase": t_push, "deposit phase": t_dep,
+             "sor_batch_kernel": t_sor * (half_sweeps / 8.0) / interval}
+    cells_all = (g.nz + 1) * (g.nr + 1)
+    algo = {  # algorithmic bytes per launch and measured ms per launch
+        "fdtd_kernel": (cells * B_CELL_TMTE, t_fdtd, "%d cells x %d B (TM+TE cell-update)" % (cells, B_CELL_TMTE)),
+        "sor_batch_kernel": (8 * cells_all * B_SOR, t_sor,
+                             "8 half-sweeps x %d points x %d B (SOR point of one half-sweep)" % (cells_all, B_SOR)),
+        "push phase": (npk * B_PUSH, t_push, "%d particles x %d B (particle-push)" % (npk, B_PUSH)),
+    }
+    dominant = max((k for k in share if k in algo), key=share.get)
+    byts, ms, what = algo[dominant]
+    ach = byts / (ms * 1e-3) / 1e9
+    rl = {"bound": "hbm", "kernel": dominant, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+          "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": byts,
+          "ms_per_launch": ms, "units": what}
     fdtd_gbs = cells * B_CELL_TMTE / (t_fdtd * 1e-3) / 1e9
     push_gbs = npk * B_PUSH / (t_push * 1e-3) / 1e9
+    sor_gbs = 8 * cells_all * B_SOR / (t_sor * 1e-3) / 1e9
 
     cpu = None
     if not args.no_cpu:
@@ -179,9 +184,10 @@ def main():
                                "TM+TE, Poisson every 10 steps to 1e-8, screen z=10 cm",
                    "grid_lines": [g.nz, g.nr], "cells": cells, "particles": args.particles, "dt": g.dt},
         "cell_updates_per_s": cells * args.steps / wall,
-        "phase_ms": per_step, "poisson_iterations": it,
+        "phase_ms": per_step, "poisson_iterations": it, "kernel_ms_per_step": share,
         "fdtd_gbs": fdtd_gbs, "fdtd_frac": fdtd_gbs / HBM_PEAK_GBS,
         "push_gbs": push_gbs, "push_frac": push_gbs / HBM_PEAK_GBS,
+        "sor_gbs": sor_gbs, "sor_frac": sor_gbs / HBM_PEAK_GBS,
         "roofline": rl, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

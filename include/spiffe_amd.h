@@ -198,6 +198,10 @@ int spf_poisson_correct(spf_ctx *ctx, int32_t run, int32_t space_charge, int64_t
  * rhs = Q) or PsiImposed (which=1, no rhs).  Synchronizing. */
 int spf_poisson_solve(spf_ctx *ctx, int32_t run, int32_t which, double accuracy, int64_t max_iterations,
                       int64_t *iterations);
+/* exactly `count` red-black half-sweeps (half-sweep indices n0 .. n0+count-1 of the
+ * omega schedule of src/poisson.c:121-124) with no stopping test: the loop body of
+ * solve_poisson_cyl as a fixed-count smoother.  Stream-ordered, not synchronizing. */
+int spf_poisson_sweeps(spf_ctx *ctx, int32_t run, int32_t which, int64_t n0, int64_t count);
 /* replaces compute_imposed_Efield, src/poisson_correction.c:456-562 */
 int spf_imposed_efield(spf_ctx *ctx, int32_t run, int32_t initialize, double ramp_time, double flat_top_time);
 

@@ -123,6 +123,28 @@ int spf_poisson_solve(spf_ctx *c, int32_t run, int32_t which, double accuracy, i
   return SPF_OK;
 }
 
+int spf_poisson_sweeps(spf_ctx *c, int32_t run, int32_t which, int64_t n0, int64_t count) {
+  Run *R = get_run(c, run);
+  if (!R || (which != 0 && which != 1) || n0 < 0 || count < 0) return fail(c, SPF_EINVAL, "bad sweep arguments");
+  if (count == 0) return SPF_OK;
+  int rc = ensure_omega(c, R, n0 + count);
+  if (rc) return rc;
+  double **Psi = which == 0 ? &R->Psi : &R->PsiImp;
+  const double *rhs = which == 0 ? R->Q : nullptr;
+  const int K = sor_batch();
+  double *buf[2] = {*Psi, R->PsiAlt};
+  int64_t nb = (count + K - 1) / K;
+  for (int64_t b = 0; b < nb; b++) {
+    int k = (int)std::min<int64_t>(K, count - b * K);
+    HIPCALL(c, launch_sor_batch(*R, buf[b % 2], buf[(b + 1) % 2], rhs, n0 + b * K, k, 1, c->stream));
+  }
+  if (nb % 2) {
+    R->PsiAlt = *Psi;
+    *Psi = buf[1];
+  }
+  return SPF_OK;
+}
+
 int spf_poisson_correct(spf_ctx *c, int32_t run, int32_t space_charge, int64_t *iterations) {
   Run *R = get_run(c, run);
   if (!R) return fail(c, SPF_EINVAL, "bad run id");

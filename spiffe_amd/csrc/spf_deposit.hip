@@ -11,9 +11,8 @@
 //  2. a stable radix sort of the (key, value) pairs: inputs are laid out in
 //     particle index order per component, so each cell's contributions come
 //     out contiguous and still in index order;
-//  3. each cell segment is folded sequentially onto the cell's current value:
-//     short segments by one thread, long ones by one wavefront that loads 64
-//     values at a time and chains the additions through readlane.
+//  3. each cell segment is folded sequentially onto the cell's current value
+//     by one thread, adding from register blocks of 16 values.
 // Every cell has exactly one writer, so the result is the reference's bits.
 // Contributions that are exactly zero in the reference (ir >= nr-1 for Jz and
 // Jphi, ir == 0 and ir == nr-1 for Jr) are left out: J starts each step at +0
@@ -30,7 +29,6 @@ namespace spf {
 namespace {
 
 constexpr int COMP_JZ = 0, COMP_JR = 1, COMP_JPHI = 2, COMP_Q = 3;
-constexpr int LONG_SEG = 48;   // segments longer than this are folded by a whole wavefront
 
 struct KeyArgs {
   int nz, nr, ncomp, ldz;
@@ -148,7 +146,6 @@ struct FoldArgs {
   const int32_t *seg;
   int64_t *counters;
   double *target[4];
-  int32_t *long_list;
 };
 
 __device__ inline double *target_cell(const FoldArgs &a, uint32_t key) {
@@ -163,56 +160,37 @@ __device__ inline void seg_range(const FoldArgs &a, int64_t S, int64_t g, int64_
   e = (g + 1 < S) ? a.seg[g + 1] : a.counters[6];
 }
 
-// one thread per segment; long segments are queued for the wavefront kernel
-__global__ __launch_bounds__(256) void fold_short_kernel(FoldArgs a) {
+// One thread per segment: the additions of a segment form one dependent
+// chain, so a thread adds from registers, 16 values per block, with the
+// next block's loads already in flight.
+constexpr int FB = 16;
+
+__global__ __launch_bounds__(256) void fold_kernel(FoldArgs a) {
   const int64_t S = a.counters[5];
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S; g += (int64_t)gridDim.x * blockDim.x) {
     int64_t s0, e0;
     seg_range(a, S, g, s0, e0);
-    if (e0 - s0 > LONG_SEG) {
-      int slot = atomicAdd((unsigned long long *)(a.counters + 9), 1ull);
-      a.long_list[slot] = (int32_t)g;
-      continue;
-    }
     double *t = target_cell(a, a.sk[s0]);
     double s = *t;
-    for (int64_t i = s0; i < e0; i++) s += a.sv[i];
-    *t = s;
-  }
-}
-
-__device__ inline double readlane_d(double v, int lane) {
-  int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
-}
-
-// one wavefront per long segment: 64 values per load, added in order
-__global__ __launch_bounds__(256) void fold_long_kernel(FoldArgs a) {
-  const int64_t S = a.counters[5];
-  const int64_t L = a.counters[9];
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t w = wave; w < L; w += nwaves) {
-    int64_t g = a.long_list[w];
-    int64_t s0, e0;
-    seg_range(a, S, g, s0, e0);
-    double *t = target_cell(a, a.sk[s0]);
-    double s = *t;
-    double nxt = (s0 + lane < e0) ? a.sv[s0 + lane] : 0.0;
-    for (int64_t b = s0; b < e0; b += 64) {
-      double cur = nxt;
-      nxt = (b + 64 + lane < e0) ? a.sv[b + 64 + lane] : 0.0;   // next chunk in flight
-      int64_t m = e0 - b;
-      if (m >= 64) {
+    int64_t i = s0;
+    if (e0 - i >= FB) {
+      double cur[FB], nxt[FB];
 #pragma unroll
-        for (int k = 0; k < 64; k++) s += readlane_d(cur, k);
-      } else {
-        for (int k = 0; k < (int)m; k++) s += readlane_d(cur, k);
+      for (int k = 0; k < FB; k++) cur[k] = a.sv[i + k];
+      for (; i + 2 * FB <= e0; i += FB) {
+#pragma unroll
+        for (int k = 0; k < FB; k++) nxt[k] = a.sv[i + FB + k];
+#pragma unroll
+        for (int k = 0; k < FB; k++) s += cur[k];
+#pragma unroll
+        for (int k = 0; k < FB; k++) cur[k] = nxt[k];
       }
+#pragma unroll
+      for (int k = 0; k < FB; k++) s += cur[k];
+      i += FB;
     }
-    if (lane == 0) *t = s;
+    for (; i < e0; i++) s += a.sv[i];
+    *t = s;
   }
 }
 
@@ -297,10 +275,7 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   f.seg = R.gstart;
   f.counters = R.counters;
   for (int i = 0; i < 4; i++) f.target[i] = i < ncomp ? targets[i] : nullptr;
-  f.long_list = R.vals;
-  hipLaunchKernelGGL(fold_short_kernel, dim3(grid_for(n, 4096)), dim3(256), 0, s, f);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(fold_long_kernel, dim3(1024), dim3(256), 0, s, f);
+  hipLaunchKernelGGL(fold_kernel, dim3(grid_for(n, 4096)), dim3(256), 0, s, f);
   return hipGetLastError();
 }
 

@@ -7,9 +7,8 @@
 // pure stencil of the previous sub-update's output, so the final value of each
 // array element is a closed function of the time-level-n arrays.  This kernel
 // evaluates that function per point.  E and B are ping-ponged (read level n,
-// write level n+1).  Each workgroup owns a 64 x 16 tile: it first computes the
-// final Ez, Er and Ephi of the tile and a 2-point margin into LDS, then the B
-// updates and boundary patches of the tile from those values.  The reference's
+// write level n+1).  Each thread computes its point's final E; the B update
+// reads the neighbouring new E from LDS or the next lane.  The reference's
 // loop `break` after FL_LAST_ACTIVE_PT is reproduced with per-row cut-offs
 // (cut1 for loops starting at iz=1, cut0 for iz=0).  Arithmetic repeats the
 // reference's operation order; the library is built with -ffp-contract=off so
@@ -215,284 +214,119 @@ __device__ inline int xcd_band_block(int bid, int nblocks) {
   return logical;
 }
 
-// Tiled two-phase step: phase 1 evaluates the final Ez, Er, Ephi of every
-// point of the tile plus a 2-point margin into LDS (each once); phase 2 does
-// the B updates and boundary patches of the tile from those LDS values.  The
-// margin covers every neighbour a B update or patch refers to.
-constexpr int FTZ = 64, FTR = 16, FM = 2;
-constexpr int FW = FTZ + 2 * FM, FH = FTR + 2 * FM;
+// One point per thread.  A block is 9 wavefronts x 64 lanes: wavefront w is
+// row r0-1+w and lane l column z0-1+l, so row r0-1 and column z0-1 are halo
+// and the block outputs a 63 x 8 tile.  Every thread computes its point's
+// final Ez, Er, Ephi into LDS; then each output thread computes its B updates.
+// Points in the box z in [1, nz-2], r in [1, nr-2] use the plain loop bodies
+// (no boundary patch reaches them); all other points go through F, which
+// applies the patches.  The B updates of points in z in [2, nz-2],
+// r in [2, nr-3] read their neighbours' new E from LDS; the others use F too.
+constexpr int QZ = 63, QR = 8, QW = QR + 1;
 
-struct FL {
-  const FdtdArgs &a;
-  const double (*ez)[FW], (*er)[FW], (*ephi)[FW];
-  int zb, rb;
-  __device__ int64_t ix(int iz, int ir) const { return fix(a.ldz, iz, ir); }
-  __device__ double ez_final(int iz, int ir) const { return ez[ir - rb][iz - zb]; }
-  __device__ double er_final(int iz, int ir) const { return er[ir - rb][iz - zb]; }
-  __device__ double ephi_final(int iz, int ir) const { return ephi[ir - rb][iz - zb]; }
-  __device__ bool act1(int iz, int ir, int16_t bit) const {
-    return !(a.flags[ix(iz, ir)] & bit) && iz <= a.cut1[ir];
-  }
-  __device__ bool act0(int iz, int ir, int16_t bit) const {
-    return !(a.flags[ix(iz, ir)] & bit) && iz <= a.cut0[ir];
-  }
-  __device__ void bphi_upd(int iz, int ir, double &b, double &b2) const {
-    int64_t p = ix(iz, ir);
-    b = a.Bphi[p];
-    b2 = a.Bphi2[p];
-    if (act1(iz, ir, BPHI_ZERO)) {
-      double dB = a.dti * ((ez_final(iz, ir) - ez_final(iz, ir - 1)) / a.dr - (er_final(iz, ir) - er_final(iz - 1, ir)) / a.dz);
-      b2 = b + dB / 2;
-      b = b + dB;
-    }
-  }
-  __device__ void bphi_inner(int iz, int ir, double &b, double &b2) const {
-    if (ir == 0) {
-      bphi_upd(iz, 1, b, b2);
-      b = -b;
-      b2 = -b2;
-    } else if (ir == a.nr - 1) {
-      bphi_upd(iz, a.nr - 2, b, b2);
-      b = b * a.top_factor;
-      b2 = b2 * a.top_factor;
-    } else if (ir == a.nr) {
-      int64_t p = ix(iz, ir);
-      b = a.Bphi[p];
-      b2 = a.Bphi2[p];
-    } else {
-      bphi_upd(iz, ir, b, b2);
-    }
-  }
-  __device__ void bphi_final(int iz, int ir, double &b, double &b2) const {
-    if (iz == 0) {
-      bphi_inner(1, ir, b, b2);
-      if (!a.left_neu) { b = -b; b2 = -b2; }
-    } else if (iz == a.nz) {
-      bphi_inner(a.nz - 1, ir, b, b2);
-      if (!a.right_neu) { b = -b; b2 = -b2; }
-    } else {
-      bphi_inner(iz, ir, b, b2);
-    }
-  }
-  __device__ void br_upd(int iz, int ir, double &b, double &b2) const {
-    int64_t p = ix(iz, ir);
-    b = a.Br[p];
-    b2 = a.Br2[p];
-    if (act1(iz, ir, BR_ZERO)) {
-      double dB = a.dti * (ephi_final(iz, ir) - ephi_final(iz - 1, ir)) / a.dz;
-      b2 = b + dB / 2;
-      b = b + dB;
-    }
-  }
-  __device__ void br_inner(int iz, int ir, double &b, double &b2) const {
-    if (ir == a.nr) {
-      int64_t p = ix(iz, ir);
-      b = a.Br[p];
-      b2 = a.Br2[p];
-    } else {
-      br_upd(iz, ir, b, b2);
-    }
-  }
-  __device__ void br_final(int iz, int ir, double &b, double &b2) const {
-    if (iz == a.nz) {
-      br_inner(a.nz - 1, ir, b, b2);
-      if (!a.right_neu) { b = -b; b2 = -b2; }
-    } else if (iz == 0) {
-      br_inner(1, ir, b, b2);
-      if (!a.left_neu) { b = -b; b2 = -b2; }
-    } else {
-      br_inner(iz, ir, b, b2);
-    }
-  }
-  __device__ void bz_upd(int iz, int ir, double &b, double &b2) const {
-    int64_t p = ix(iz, ir);
-    b = a.Bz[p];
-    b2 = a.Bz2[p];
-    if (act0(iz, ir, BZ_ZERO)) {
-      double dB = a.bzf[ir] * ((double)(ir + 1) * ephi_final(iz, ir) - (double)ir * ephi_final(iz, ir - 1));
-      b2 = b + dB / 2;
-      b = b + dB;
-    }
-  }
-  __device__ void bz_final(int iz, int ir, double &b, double &b2) const {
-    if ((a.right_neu && iz == a.nz - 1) || (a.left_neu && iz == 0)) {
-      b = 0.0;
-      b2 = 0.0;
-      return;
-    }
-    if (ir == 0) {
-      if (iz == a.nz) {
-        int64_t p = ix(iz, 1);
-        b = a.Bz[p];
-        b2 = a.Bz2[p];
-      } else {
-        bz_upd(iz, 1, b, b2);
-      }
-      return;
-    }
-    if (iz == a.nz || ir == a.nr) {
-      int64_t p = ix(iz, ir);
-      b = a.Bz[p];
-      b2 = a.Bz2[p];
-      return;
-    }
-    bz_upd(iz, ir, b, b2);
-  }
-};
-
-// Interior fast path.  Inside the box z in [2, nz-2], r in [2, nr-3] no
-// boundary patch applies, so every final value is the plain loop body.  One
-// wavefront owns a strip of 63 columns (lane 0 is the z-1 halo) and walks it
-// upward in r one row at a time: the previous row's new Ez and Ephi stay in
-// registers, the z-1 neighbour's new Er and Ephi come from the lane below,
-// and the z+1 neighbour's old Bphi and Br from the lane above, so each array
-// element is read from memory once.
-constexpr int QZ = 63, QR = 32;
-
-struct Box {
-  int zi0, zi1, ri0, ri1;   // interior box [zi0, zi1) x [ri0, ri1)
-  int strips_z, nstrips;
-};
-
-__device__ inline double shfl_dn1(double v) { return __shfl_down(v, 1, 64); }
-__device__ inline double shfl_up1(double v) { return __shfl_up(v, 1, 64); }
-
-__global__ __launch_bounds__(256) void fdtd_interior_kernel(FdtdArgs a, Box bx, int nblocks) {
+__global__ __launch_bounds__(64 * QW) void fdtd_kernel(FdtdArgs a, int tiles_z, int nblocks) {
+  __shared__ double sez[QW][64], ser[QW][64], sephi[QW][64];
   int logical = xcd_band_block(blockIdx.x, nblocks);
   if (logical >= nblocks) return;
-  const int lane = threadIdx.x & 63;
-  const int strip = logical * 4 + (threadIdx.x >> 6);
-  if (strip >= bx.nstrips) return;
-  const int z0 = bx.zi0 + (strip % bx.strips_z) * QZ, r0 = bx.ri0 + (strip / bx.strips_z) * QR;
-  const int r1 = min(r0 + QR, bx.ri1);
-  const int iz = z0 - 1 + lane;                    // lane 0: halo column z0-1 >= 1
-  const bool out_col = lane > 0 && iz < bx.zi1;
-  const int izl = min(iz, a.nz - 1);               // load index (columns past the box are not written)
-  const double dti = a.dti, eps0 = a.eps0, dz = a.dz, dr = a.dr;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int z0 = (logical % tiles_z) * QZ, r0 = (logical / tiles_z) * QR;
+  const int iz = z0 - 1 + lane, ir = r0 - 1 + w;
+  const bool inside = iz >= 0 && iz <= a.nz && ir >= 0 && ir <= a.nr;
   const int ldz = a.ldz;
-  double ez_prev = 0, ephi_prev = 0;
-  double bphi_lo = 0, bz_lo = 0;
-  if (a.tm) bphi_lo = a.Bphi[fix(ldz, izl, r0 - 1)];
-  if (a.te) bz_lo = a.Bz[fix(ldz, izl, r0 - 1)];
-  for (int ir = r0 - 1; ir < r1; ir++) {
-    const int64_t p = fix(ldz, izl, ir);
-    const int16_t fl = a.flags[p];
-    const int c0 = a.cut0[ir], c1 = a.cut1[ir];
-    const bool out = out_col && ir >= r0;
-    double ez_n = 0, er_n = 0, ephi_n = 0;
-    if (a.tm) {
-      const double bphi_c = bphi_lo, bphi_hi = a.Bphi[p + ldz];
-      double bphi_zp = shfl_dn1(bphi_c);
-      if (lane == 63) bphi_zp = a.Bphi[p + 1];
-      ez_n = a.Ez[p];
-      if (!(fl & EZ_ZERO) && izl <= c1)
-        ez_n += dti * (-a.Jz[p] / eps0 + (a.cp[ir] * bphi_hi - a.cm[ir] * bphi_c));
-      er_n = a.Er[p];
-      if (!(fl & ER_ZERO) && izl <= c0)
-        er_n += dti * (-a.Jr[p] / eps0 - a.er_factor * (bphi_zp - bphi_c));
-      const double er_zm = shfl_up1(er_n);
-      if (out) {
-        double b = bphi_c, b2;
-        if (!(fl & BPHI_ZERO) && izl <= c1) {
-          double dB = dti * ((ez_n - ez_prev) / dr - (er_n - er_zm) / dz);
-          b2 = b + dB / 2;
-          b = b + dB;
-        } else {
-          b2 = a.Bphi2[p];
-        }
-        a.Ezn[p] = ez_n;
-        a.Ern[p] = er_n;
-        a.Bphin[p] = b;
-        a.Bphi2[p] = b2;
-      }
-      bphi_lo = bphi_hi;
-    }
-    if (a.te) {
-      const double bz_c = bz_lo, bz_hi = a.Bz[p + ldz];
-      const double br_c = a.Br[p];
-      double br_zp = shfl_dn1(br_c);
-      if (lane == 63) br_zp = a.Br[p + 1];
-      ephi_n = a.Ephi[p];
-      if (!(fl & EPHI_ZERO) && izl <= c0)
-        ephi_n += dti * (-a.Jphi[p] / eps0 + a.ephi_factor * ((br_zp - br_c) / dz - (bz_hi - bz_c) / dr));
-      const double ephi_zm = shfl_up1(ephi_n);
-      if (out) {
-        double b = br_c, b2;
-        if (!(fl & BR_ZERO) && izl <= c1) {
-          double dB = dti * (ephi_n - ephi_zm) / dz;
-          b2 = b + dB / 2;
-          b = b + dB;
-        } else {
-          b2 = a.Br2[p];
-        }
-        a.Ephin[p] = ephi_n;
-        a.Brn[p] = b;
-        a.Br2[p] = b2;
-        double c = bz_c, c2;
-        if (!(fl & BZ_ZERO) && izl <= c0) {
-          double dB = a.bzf[ir] * ((double)(ir + 1) * ephi_n - (double)ir * ephi_prev);
-          c2 = c + dB / 2;
-          c = c + dB;
-        } else {
-          c2 = a.Bz2[p];
-        }
-        a.Bzn[p] = c;
-        a.Bz2[p] = c2;
-      }
-      bz_lo = bz_hi;
-    }
-    ez_prev = ez_n;
-    ephi_prev = ephi_n;
-  }
-}
-
-__global__ __launch_bounds__(256) void fdtd_kernel(FdtdArgs a, int tiles_z, int nblocks, Box bx) {
-  __shared__ double sez[FH][FW], ser[FH][FW], sephi[FH][FW];
-  int logical = xcd_band_block(blockIdx.x, nblocks);
-  if (logical >= nblocks) return;
-  const int z0 = (logical % tiles_z) * FTZ, r0 = (logical / tiles_z) * FTR;
-  const int zb = z0 - FM, rb = r0 - FM;
-  // tiles entirely inside the interior box are done by fdtd_interior_kernel
-  if (z0 >= bx.zi0 && z0 + FTZ <= bx.zi1 && r0 >= bx.ri0 && r0 + FTR <= bx.ri1) return;
+  const int izc = min(max(iz, 0), a.nz), irc = min(max(ir, 0), a.nr);
+  const int64_t p = fix(ldz, izc, irc);
   F f{a};
-  // phase 1: final E over the tile and its margin (points inside the array only)
-  for (int t = threadIdx.x; t < FW * FH; t += 256) {
-    int lr = t / FW, lz = t % FW;
-    int iz = zb + lz, ir = rb + lr;
-    if (iz < 0 || iz > a.nz || ir < 0 || ir > a.nr) continue;
-    if (a.tm) {
-      sez[lr][lz] = f.ez_final(iz, ir);
-      ser[lr][lz] = f.er_final(iz, ir);
-    }
-    if (a.te) sephi[lr][lz] = f.ephi_final(iz, ir);
+  const bool efast = iz >= 1 && iz <= a.nz - 2 && ir >= 1 && ir <= a.nr - 2;
+  const int16_t fl = a.flags[p];
+  // neighbours z+1 through the lane above (lane 63 loads its own)
+  double bphi_c = 0, br_c = 0;
+  if (a.tm) bphi_c = a.Bphi[p];
+  if (a.te) br_c = a.Br[p];
+  double bphi_zp = __shfl_down(bphi_c, 1, 64), br_zp = __shfl_down(br_c, 1, 64);
+  if (lane == 63 && efast) {
+    if (a.tm) bphi_zp = a.Bphi[p + 1];
+    if (a.te) br_zp = a.Br[p + 1];
   }
+  double ez_n = 0, er_n = 0, ephi_n = 0;
+  if (inside) {
+    if (efast) {
+      const int c0 = a.cut0[ir], c1 = a.cut1[ir];
+      if (a.tm) {
+        const double bphi_hi = a.Bphi[p + ldz];
+        ez_n = a.Ez[p];
+        if (!(fl & EZ_ZERO) && iz <= c1)
+          ez_n += a.dti * (-a.Jz[p] / a.eps0 + (a.cp[ir] * bphi_hi - a.cm[ir] * bphi_c));
+        er_n = a.Er[p];
+        if (!(fl & ER_ZERO) && iz <= c0)
+          er_n += a.dti * (-a.Jr[p] / a.eps0 - a.er_factor * (bphi_zp - bphi_c));
+      }
+      if (a.te) {
+        const double bz_c = a.Bz[p], bz_hi = a.Bz[p + ldz];
+        ephi_n = a.Ephi[p];
+        if (!(fl & EPHI_ZERO) && iz <= c0)
+          ephi_n += a.dti * (-a.Jphi[p] / a.eps0 + a.ephi_factor * ((br_zp - br_c) / a.dz - (bz_hi - bz_c) / a.dr));
+      }
+    } else {
+      if (a.tm) {
+        ez_n = f.ez_final(iz, ir);
+        er_n = f.er_final(iz, ir);
+      }
+      if (a.te) ephi_n = f.ephi_final(iz, ir);
+    }
+  }
+  sez[w][lane] = ez_n;
+  ser[w][lane] = er_n;
+  sephi[w][lane] = ephi_n;
   __syncthreads();
-  FL g{a, sez, ser, sephi, zb, rb};
-  const int iz = z0 + (threadIdx.x & 63);
-  for (int k = threadIdx.x >> 6; k < FTR; k += 4) {
-    const int ir = r0 + k;
-    if (iz > a.nz || ir > a.nr) continue;
-    if (iz >= bx.zi0 && iz < bx.zi1 && ir >= bx.ri0 && ir < bx.ri1) continue;
-    const int64_t p = fix(a.ldz, iz, ir);
-    const int lz = iz - zb, lr = ir - rb;
-    if (a.tm) {
-      a.Ezn[p] = sez[lr][lz];
-      a.Ern[p] = ser[lr][lz];
-      double b, b2;
-      g.bphi_final(iz, ir, b, b2);
-      a.Bphin[p] = b;
-      a.Bphi2[p] = b2;
+  if (!inside || w == 0 || lane == 0 || iz >= z0 + QZ) return;
+  const bool bfast = iz >= 2 && iz <= a.nz - 2 && ir >= 2 && ir <= a.nr - 3;
+  if (a.tm) {
+    double b, b2;
+    if (bfast) {
+      b = bphi_c;
+      if (!(fl & BPHI_ZERO) && iz <= a.cut1[ir]) {
+        double dB = a.dti * ((ez_n - sez[w - 1][lane]) / a.dr - (er_n - ser[w][lane - 1]) / a.dz);
+        b2 = b + dB / 2;
+        b = b + dB;
+      } else {
+        b2 = a.Bphi2[p];
+      }
+    } else {
+      f.bphi_final(iz, ir, b, b2);
     }
-    if (a.te) {
-      a.Ephin[p] = sephi[lr][lz];
-      double b, b2;
-      g.br_final(iz, ir, b, b2);
-      a.Brn[p] = b;
-      a.Br2[p] = b2;
-      g.bz_final(iz, ir, b, b2);
-      a.Bzn[p] = b;
-      a.Bz2[p] = b2;
+    a.Ezn[p] = ez_n;
+    a.Ern[p] = er_n;
+    a.Bphin[p] = b;
+    a.Bphi2[p] = b2;
+  }
+  if (a.te) {
+    double b, b2, c, c2;
+    if (bfast) {
+      b = br_c;
+      if (!(fl & BR_ZERO) && iz <= a.cut1[ir]) {
+        double dB = a.dti * (ephi_n - sephi[w][lane - 1]) / a.dz;
+        b2 = b + dB / 2;
+        b = b + dB;
+      } else {
+        b2 = a.Br2[p];
+      }
+      c = a.Bz[p];
+      if (!(fl & BZ_ZERO) && iz <= a.cut0[ir]) {
+        double dB = a.bzf[ir] * ((double)(ir + 1) * ephi_n - (double)ir * sephi[w - 1][lane]);
+        c2 = c + dB / 2;
+        c = c + dB;
+      } else {
+        c2 = a.Bz2[p];
+      }
+    } else {
+      f.br_final(iz, ir, b, b2);
+      f.bz_final(iz, ir, c, c2);
     }
+    a.Ephin[p] = ephi_n;
+    a.Brn[p] = b;
+    a.Br2[p] = b2;
+    a.Bzn[p] = c;
+    a.Bz2[p] = c2;
   }
 }
 
@@ -548,25 +382,10 @@ __global__ void add_line_kernel(double *J, int ldz, int direction, int64_t ipos,
 }  // namespace
 
 hipError_t launch_fdtd(const FdtdArgs &a, hipStream_t s) {
-  Box bx;
-  bx.zi0 = 2;
-  bx.zi1 = a.nz - 1;   // z in [2, nz-2]
-  bx.ri0 = 2;
-  bx.ri1 = a.nr - 2;   // r in [2, nr-3]
-  if (bx.zi1 <= bx.zi0 || bx.ri1 <= bx.ri0) bx.zi1 = bx.zi0 = bx.ri1 = bx.ri0 = 0;
-  bx.strips_z = (bx.zi1 - bx.zi0 + QZ - 1) / QZ;
-  bx.nstrips = bx.strips_z * ((bx.ri1 - bx.ri0 + QR - 1) / QR);
-  if (bx.nstrips > 0) {
-    int nb = (bx.nstrips + 3) / 4;
-    int grid = ((nb + 7) / 8) * 8;
-    hipLaunchKernelGGL(fdtd_interior_kernel, dim3(grid), dim3(256), 0, s, a, bx, nb);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  int tiles_z = (a.nz + 1 + FTZ - 1) / FTZ, tiles_r = (a.nr + 1 + FTR - 1) / FTR;
+  int tiles_z = (a.nz + 1 + QZ - 1) / QZ, tiles_r = (a.nr + 1 + QR - 1) / QR;
   int nblocks = tiles_z * tiles_r;
   int grid = ((nblocks + 7) / 8) * 8;
-  hipLaunchKernelGGL(fdtd_kernel, dim3(grid), dim3(256), 0, s, a, tiles_z, nblocks, bx);
+  hipLaunchKernelGGL(fdtd_kernel, dim3(grid), dim3(64 * QW), 0, s, a, tiles_z, nblocks);
   return hipGetLastError();
 }
 

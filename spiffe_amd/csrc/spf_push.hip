@@ -248,7 +248,8 @@ __global__ __launch_bounds__(256) void push_kernel(PushArgs a, int64_t n_ub) {
   double dt = a.dt, dz = a.dz, dr = a.dr, zmin = a.zmin, zmax = a.zmax, rmax = a.rmax;
   double z = a.z[ip], r = a.r[ip];
   const double zold = z, rold = r;
-  double zmx = a.zmax_p[ip], zmn = a.zmin_p[ip], rmx = a.rmax_p[ip], rmn = a.rmin_p[ip];
+  const double zmax0 = a.zmax_p[ip], zmin0 = a.zmin_p[ip], rmax0 = a.rmax_p[ip], rmin0 = a.rmin_p[ip];
+  double zmx = zmax0, zmn = zmin0, rmx = rmax0, rmn = rmin0;
   if (z > zmx) zmx = z;
   if (z < zmn) zmn = z;
   if (r > rmx) rmx = r;
@@ -256,7 +257,8 @@ __global__ __launch_bounds__(256) void push_kernel(PushArgs a, int64_t n_ub) {
   double vz = a.vz[ip], vr = a.vr[ip];
   z += vz * dt;
   r += vr * dt;
-  uint32_t st = a.status[ip];
+  const uint32_t st0 = a.status[ip];
+  uint32_t st = st0;
   double adjustFactor = 1;
   if (!(st & ACTIVE)) {
     long long iz = trunc_l((z - zmin) / dz);
@@ -340,12 +342,13 @@ __global__ __launch_bounds__(256) void push_kernel(PushArgs a, int64_t n_ub) {
   a.vz[ip] = nvz;
   a.vr[ip] = nvr;
   a.vphi[ip] = nvphi;
-  a.zmax_p[ip] = zmx;
-  a.zmin_p[ip] = zmn;
-  a.rmax_p[ip] = rmx;
-  a.rmin_p[ip] = rmn;
+  // extents and status change only now and then: store them only when they do
+  if (zmx != zmax0) a.zmax_p[ip] = zmx;
+  if (zmn != zmin0) a.zmin_p[ip] = zmn;
+  if (rmx != rmax0) a.rmax_p[ip] = rmx;
+  if (rmn != rmin0) a.rmin_p[ip] = rmn;
   if (lost) st = (st & ~ACTIVE) | LOST;
-  a.status[ip] = st;
+  if (st != st0) a.status[ip] = st;
   // screen crossings, tested as in :370 and :399
   uint32_t ev = (lost ? EV_LOST : 0u) | (rneg ? EV_RNEG : 0u);
   for (int s = 0; s < a.nscreens; s++) {

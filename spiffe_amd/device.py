@@ -248,6 +248,10 @@ class DeviceRun:
                                                   int(maximum_iterations), ctypes.byref(it)))
         return it.value
 
+    def poisson_sweeps(self, which, n0, count):
+        """`count` half-sweeps with no stopping test (stream-ordered)."""
+        self.ctx.check(self.lib.spf_poisson_sweeps(self.ctx.h, self.id, int(which), int(n0), int(count)))
+
     def compute_imposed_Efield(self, initialize, ramp_time=0.0, flat_top_time=0.0):
         self.ctx.check(self.lib.spf_imposed_efield(self.ctx.h, self.id, 1 if initialize else 0, float(ramp_time),
                                                    float(flat_top_time)))
