@@ -33,14 +33,14 @@ void free_run(Run *R) {
   };
   for (int k = 0; k < 6; k++)
     for (int j = 0; j < 2; j++) F(R->E2[k][j]);
-  F(R->Bphi2); F(R->Br2); F(R->Bz2); F(R->Jz); F(R->Jr); F(R->Jphi); F(R->Psi); F(R->Q); F(R->PsiAlt);
+  F(R->Bphi2); F(R->Br2); F(R->Bz2); F(R->Jz); F(R->Jr); F(R->Jphi); F(R->Psi); F(R->Q); for (int k = 0; k < 4; k++) F(R->PsiSpare[k]);
   F(R->EzImp); F(R->ErImp); F(R->PsiImp); F(R->BzImp); F(R->BrImp); F(R->flags); F(R->cut0); F(R->cut1);
   F(R->cp); F(R->cm); F(R->bzf); F(R->sor_c); F(R->sor_d); F(R->volume);
   for (int k = 0; k < 16; k++) F(R->P[k]);
   F(R->pid); F(R->gen); F(R->status); F(R->counters); F(R->evt); F(R->evflag);
   for (int k = 0; k < 7; k++) F(R->ev[k]);
   F(R->last_old); F(R->evlist); F(R->move_src); F(R->move_dst); F(R->srows); F(R->lrows);
-  F(R->keys); F(R->skeys); F(R->vals); F(R->cz); F(R->scz); F(R->gstart); F(R->gflag); F(R->sort_tmp);
+  F(R->keys); F(R->skeys); F(R->vals); F(R->cz); F(R->scz); F(R->gstart); F(R->long_list); F(R->gflag); F(R->sort_tmp);
   F(R->bcount); F(R->boff); F(R->stage); F(R->append_stage); F(R->Pdev); F(R->omega); F(R->red);
   F(R->red_counter); F(R->sor_state); F(R->sor_sums);
   R->live = false;
@@ -107,7 +107,7 @@ void *spf_stream(spf_ctx *c) { return c ? (void *)c->stream : nullptr; }
 int64_t spf_run_footprint(const spf_run_desc *d) {
   if (!d || d->nz < 3 || d->nr < 3) return -1;
   int64_t cells = ldz_for(d->nz) * (int64_t)(d->nr + 1);
-  int64_t field_bytes = cells * 8 * 29 + cells * 2;                      // 29 double grids + flags
+  int64_t field_bytes = cells * 8 * 32 + cells * 2;                      // 32 double grids + flags
   int64_t n = d->max_particles > 0 ? d->max_particles : 1;
   int64_t part_bytes = n * (16 * 8 + 4 + 2 + 4);                         // particle SoA
   int64_t event_bytes = n * (4 + 1 + 7 * 8 + 4 + 8);                     // staging, lists
@@ -156,7 +156,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
     for (int j = 0; j < 2; j++) ALLOC(R->E2[k][j], cells);
   ALLOC(R->Bphi2, cells); ALLOC(R->Br2, cells); ALLOC(R->Bz2, cells);
   ALLOC(R->Jz, cells); ALLOC(R->Jr, cells); ALLOC(R->Jphi, cells);
-  ALLOC(R->Psi, cells); ALLOC(R->Q, cells); ALLOC(R->PsiAlt, cells);
+  ALLOC(R->Psi, cells); ALLOC(R->Q, cells); for (int k = 0; k < 4; k++) ALLOC(R->PsiSpare[k], cells);
   ALLOC(R->EzImp, cells); ALLOC(R->ErImp, cells); ALLOC(R->PsiImp, cells);
   ALLOC(R->BzImp, cells); ALLOC(R->BrImp, cells);
   ALLOC(R->flags, cells);
@@ -177,7 +177,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   // ordered scatter-add: three components (Jz, Jr, Jphi) per particle
   const int64_t cap3 = 3 * cap;
   ALLOC(R->keys, cap3); ALLOC(R->skeys, cap3); ALLOC(R->cz, cap3); ALLOC(R->scz, cap3);
-  ALLOC(R->vals, cap3); ALLOC(R->gstart, cap3); ALLOC(R->gflag, cap3);
+  ALLOC(R->vals, cap3); ALLOC(R->gstart, cap3); ALLOC(R->long_list, cap3 / 256 + 1); ALLOC(R->gflag, cap3);
   R->sort_tmp_bytes = deposit_sort_bytes(cap3, deposit_cell_bits(nz, nr) + 2);
   {
     uint8_t *t = nullptr;
@@ -186,7 +186,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   }
   R->nblk_cap = compact_blocks(cap3) + 1;
   ALLOC(R->bcount, R->nblk_cap); ALLOC(R->boff, R->nblk_cap);
-  R->red_n = std::max<int64_t>(sor_tiles(*R) * sor_batch(), 512) + 16;
+  R->red_n = std::max<int64_t>(sor_tiles(*R) * sor_batch() * 4, 512) + 16;
   ALLOC(R->red, R->red_n);
   ALLOC(R->red_counter, 4); ALLOC(R->sor_state, 4); ALLOC(R->sor_sums, 8);
   ALLOC(R->Pdev, 16);
@@ -275,6 +275,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
     up(R->cm, cm.data(), cm.size() * 8);
     up(R->bzf, bzf.data(), bzf.size() * 8);
     up(R->sor_c, sc.data(), sc.size() * 8);
+    R->sor_c_host0 = sc[0];
     up(R->sor_d, sd.data(), sd.size() * 8);
     up(R->volume, vol.data(), vol.size() * 8);
     up(R->Pdev, R->P, sizeof(R->P));

@@ -13,6 +13,8 @@ namespace {
 
 constexpr double PI_D = 3.141592653589793238462643;
 constexpr int64_t kSorCheckBatches = 16;   // batches queued between host checks of the done flag
+constexpr int kCheck = 4;                  // batches per stopping-rule check
+constexpr int kRing = kCheck + 1;          // Psi buffers the batches rotate through
 
 // omega for half-sweep k: src/poisson.c:47-49 and :121-124
 int ensure_omega(spf_ctx *c, Run *R, int64_t n) {
@@ -38,7 +40,13 @@ int ensure_omega(spf_ctx *c, Run *R, int64_t n) {
 }
 
 // solve_poisson_cyl, src/poisson.c:17-130, on *Psi (a run-owned buffer that
-// may be swapped with the run's spare buffer); *iters = n on convergence, -1 otherwise
+// may be swapped with one of the run's spare buffers); *iters = n on
+// convergence, -1 otherwise.  Batches rotate through a ring of kRing
+// buffers: batch b reads ring[b % kRing] and writes ring[(b+1) % kRing].
+// The stopping rule is applied every kCheck batches by one check launch over
+// their half-sweeps; queued batches return at once after a stop, and the
+// batch that contains the stopping half-sweep still has its input in the ring
+// (kRing > kCheck), so it is re-run for exactly the half-sweeps needed.
 int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accuracy, int64_t limit, int64_t *iters) {
   if (accuracy <= 0) return fail(c, SPF_EINVAL, "accuracy is <= 0 in solve_poisson_cyl");
   if (limit <= 1) return fail(c, SPF_EINVAL, "iteration_limit is <= 1 in solve_poisson_cyl");
@@ -46,20 +54,22 @@ int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accura
   if (rc) return rc;
   hipStream_t s = c->stream;
   const int K = sor_batch();
-  double *buf[2] = {*Psi, R->PsiAlt};
+  double *ring[kRing] = {*Psi, R->PsiSpare[0], R->PsiSpare[1], R->PsiSpare[2], R->PsiSpare[3]};
   HIPCALL(c, launch_sor_reset(*R, s));
   if (rhs)
     HIPCALL(c, launch_sum_abs(rhs, R->ldz, R->d.nz, R->d.nr - 1, R->red, R->red_n, R->sor_sums, s));
-  // batches alternate buffers: batch b reads buf[b%2] and writes buf[(b+1)%2];
-  // once converged, queued batches return at once and leave their input intact
   int64_t nb = (limit + K - 1) / K, b = 0, converged = -1;
   while (b < nb) {
-    int64_t stop = std::min<int64_t>(nb, b + kSorCheckBatches);
-    for (; b < stop; b++) {
-      int64_t n0 = b * K;
-      int k = (int)std::min<int64_t>(K, limit - n0);
-      HIPCALL(c, launch_sor_batch(*R, buf[b % 2], buf[(b + 1) % 2], rhs, n0, k, 0, s));
-      HIPCALL(c, launch_sor_check(*R, n0, k, accuracy, s));
+    int64_t win_end = std::min<int64_t>(nb, b + kSorCheckBatches);
+    while (b < win_end) {
+      int64_t g0 = b, g1 = std::min<int64_t>(win_end, b + kCheck);
+      for (; b < g1; b++) {
+        int64_t n0 = b * K;
+        int k = (int)std::min<int64_t>(K, limit - n0);
+        HIPCALL(c, launch_sor_batch(*R, ring[b % kRing], ring[(b + 1) % kRing], rhs, n0, k,
+                                    (int)((b - g0) * K), 0, s));
+      }
+      HIPCALL(c, launch_sor_check(*R, g0 * K, (int)(std::min<int64_t>(b * K, limit) - g0 * K), accuracy, s));
     }
     HIPCALL(c, hipMemcpyAsync(c->pinned, R->sor_state, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
     HIPCALL(c, hipStreamSynchronize(s));
@@ -72,17 +82,20 @@ int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accura
   if (converged >= 0) {
     int64_t bc = converged / K, n0 = bc * K;
     int k = (int)(converged - n0 + 1);
-    out = (int)((bc + 1) % 2);
-    if (k != std::min<int64_t>(K, limit - n0))   // stopped inside a batch: redo it exactly
-      HIPCALL(c, launch_sor_batch(*R, buf[bc % 2], buf[out], rhs, n0, k, 1, s));
+    out = (int)((bc + 1) % kRing);
+    // the batches after bc may have run before the check saw the stop: redo bc from its input
+    HIPCALL(c, launch_sor_batch(*R, ring[bc % kRing], ring[out], rhs, n0, k, 0, 1, s));
     *iters = converged;
   } else {
-    out = (int)(nb % 2);
+    out = (int)(nb % kRing);
     *iters = -1;
   }
-  if (buf[out] != *Psi) {
-    R->PsiAlt = *Psi;
-    *Psi = buf[out];
+  if (out != 0) {
+    double *res = ring[out];
+    int si = 0;
+    for (int i = 0; i < kRing; i++)
+      if (i != out) R->PsiSpare[si++] = ring[i];
+    *Psi = res;   // the four buffers other than the result become the spares again
   }
   return SPF_OK;
 }
@@ -132,14 +145,14 @@ int spf_poisson_sweeps(spf_ctx *c, int32_t run, int32_t which, int64_t n0, int64
   double **Psi = which == 0 ? &R->Psi : &R->PsiImp;
   const double *rhs = which == 0 ? R->Q : nullptr;
   const int K = sor_batch();
-  double *buf[2] = {*Psi, R->PsiAlt};
+  double *buf[2] = {*Psi, R->PsiSpare[0]};
   int64_t nb = (count + K - 1) / K;
   for (int64_t b = 0; b < nb; b++) {
     int k = (int)std::min<int64_t>(K, count - b * K);
-    HIPCALL(c, launch_sor_batch(*R, buf[b % 2], buf[(b + 1) % 2], rhs, n0 + b * K, k, 1, c->stream));
+    HIPCALL(c, launch_sor_batch(*R, buf[b % 2], buf[(b + 1) % 2], rhs, n0 + b * K, k, 0, 1, c->stream));
   }
   if (nb % 2) {
-    R->PsiAlt = *Psi;
+    R->PsiSpare[0] = *Psi;
     *Psi = buf[1];
   }
   return SPF_OK;

@@ -145,6 +145,7 @@ struct FoldArgs {
   const double *sv;
   const int32_t *seg;
   int64_t *counters;
+  int32_t *long_list;   // ids of segments longer than LONG_SEG
   double *target[4];
 };
 
@@ -160,16 +161,23 @@ __device__ inline void seg_range(const FoldArgs &a, int64_t S, int64_t g, int64_
   e = (g + 1 < S) ? a.seg[g + 1] : a.counters[6];
 }
 
-// One thread per segment: the additions of a segment form one dependent
-// chain, so a thread adds from registers, 16 values per block, with the
-// next block's loads already in flight.
+// The additions of a segment form one dependent chain.  Segments of up to
+// LONG_SEG values are folded by one thread each, from register blocks of 16
+// values with the next block's loads already in flight; longer segments are
+// listed and folded by one wavefront each (fold_long_kernel).
 constexpr int FB = 16;
+constexpr int64_t LONG_SEG = 256;
 
 __global__ __launch_bounds__(256) void fold_kernel(FoldArgs a) {
   const int64_t S = a.counters[5];
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S; g += (int64_t)gridDim.x * blockDim.x) {
     int64_t s0, e0;
     seg_range(a, S, g, s0, e0);
+    if (e0 - s0 > LONG_SEG) {
+      unsigned long long slot = atomicAdd((unsigned long long *)(a.counters + 9), 1ull);
+      a.long_list[slot] = (int32_t)g;
+      continue;
+    }
     double *t = target_cell(a, a.sk[s0]);
     double s = *t;
     int64_t i = s0;
@@ -191,6 +199,74 @@ __global__ __launch_bounds__(256) void fold_kernel(FoldArgs a) {
     }
     for (; i < e0; i++) s += a.sv[i];
     *t = s;
+  }
+}
+
+// One wavefront per long segment.  The wavefront loads the segment in
+// coalesced chunks of LCH values into registers, one chunk ahead, and parks
+// each chunk in LDS; lane 0 adds the parked chunk in order while the next
+// chunk's loads are in flight.
+constexpr int LU = 16;            // values per lane per chunk
+constexpr int LCH = 64 * LU;      // values per chunk
+
+__global__ __launch_bounds__(64) void fold_long_kernel(FoldArgs a) {
+  __shared__ __align__(16) double buf[LCH];
+  const int lane = threadIdx.x;
+  const int64_t S = a.counters[5], nl = a.counters[9];
+  for (int64_t q = blockIdx.x; q < nl; q += gridDim.x) {
+    const int64_t g = a.long_list[q];
+    int64_t s0, e0;
+    seg_range(a, S, g, s0, e0);
+    double *t = target_cell(a, a.sk[s0]);
+    double s = lane == 0 ? *t : 0.0;
+    double v[LU];
+#pragma unroll
+    for (int u = 0; u < LU; u++) {
+      int64_t i = s0 + u * 64 + lane;
+      v[u] = i < e0 ? a.sv[i] : 0.0;
+    }
+    for (int64_t c0 = s0; c0 < e0; c0 += LCH) {
+      __syncthreads();   // lane 0 has finished with the previous chunk
+#pragma unroll
+      for (int u = 0; u < LU; u++) buf[u * 64 + lane] = v[u];
+      __syncthreads();
+      const int64_t c1 = c0 + LCH;
+#pragma unroll
+      for (int u = 0; u < LU; u++) {
+        int64_t i = c1 + u * 64 + lane;
+        v[u] = i < e0 ? a.sv[i] : 0.0;
+      }
+      if (lane == 0) {
+        // 16 parked values per step, the next 16 already being read from LDS
+        const int m = (int)(e0 - c0 < LCH ? e0 - c0 : LCH);
+        const double2 *b2 = reinterpret_cast<const double2 *>(buf);
+        int i = 0;
+        if (m >= 16) {
+          double2 cur[8], nxt[8];
+#pragma unroll
+          for (int k = 0; k < 8; k++) cur[k] = b2[k];
+          for (; i + 32 <= m; i += 16) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) nxt[k] = b2[(i + 16) / 2 + k];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+              s += cur[k].x;
+              s += cur[k].y;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) cur[k] = nxt[k];
+          }
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            s += cur[k].x;
+            s += cur[k].y;
+          }
+          i += 16;
+        }
+        for (; i < m; i++) s += buf[i];
+      }
+    }
+    if (lane == 0) *t = s;
   }
 }
 
@@ -274,8 +350,13 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   f.sv = R.scz;
   f.seg = R.gstart;
   f.counters = R.counters;
+  f.long_list = R.long_list;
   for (int i = 0; i < 4; i++) f.target[i] = i < ncomp ? targets[i] : nullptr;
   hipLaunchKernelGGL(fold_kernel, dim3(grid_for(n, 4096)), dim3(256), 0, s, f);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  // at most n / LONG_SEG long segments
+  int64_t nlong = n / LONG_SEG + 1;
+  hipLaunchKernelGGL(fold_long_kernel, dim3((unsigned)std::min<int64_t>(nlong, 8192)), dim3(64), 0, s, f);
   return hipGetLastError();
 }
 

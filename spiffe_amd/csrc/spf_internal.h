@@ -96,12 +96,13 @@ struct Run {
   double *Bphi2 = nullptr, *Br2 = nullptr, *Bz2 = nullptr;
   double *Jz = nullptr, *Jr = nullptr, *Jphi = nullptr;
   double *Psi = nullptr, *Q = nullptr;
-  double *PsiAlt = nullptr;   // second buffer of the ping-ponged SOR solves
+  double *PsiSpare[4] = {};   // spare buffers the SOR batches rotate through
   double *EzImp = nullptr, *ErImp = nullptr, *PsiImp = nullptr, *BzImp = nullptr, *BrImp = nullptr;
   int16_t *flags = nullptr;
   int32_t *cut0 = nullptr, *cut1 = nullptr;
   double *cp = nullptr, *cm = nullptr, *bzf = nullptr;
   double *sor_c = nullptr, *sor_d = nullptr, *volume = nullptr;
+  double sor_c_host0 = 0;   // sor_c[0] on the host (axis-row reciprocal)
   // particles
   int64_t cap = 0;
   int64_t np_ub = 0;        // host-side upper bound of the device particle count
@@ -127,6 +128,7 @@ struct Run {
   double *cz = nullptr, *scz = nullptr;   // contributions, unsorted and sorted
   int32_t *vals = nullptr;                // list of long segments
   int32_t *gstart = nullptr;              // segment starts
+  int32_t *long_list = nullptr;           // long segments, folded one per wavefront
   uint8_t *gflag = nullptr;
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
@@ -189,8 +191,8 @@ hipError_t launch_sum_plain(const double *a, int ldz, int iz_n, int ir_n, double
 hipError_t launch_sor_reset(Run &R, hipStream_t s);
 int64_t sor_tiles(const Run &R);
 int sor_batch();
-hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const double *rhs, int64_t n0, int k, int force,
-                            hipStream_t s);
+hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const double *rhs, int64_t n0, int k,
+                            int partial_row, int force, hipStream_t s);
 hipError_t launch_sor_check(Run &R, int64_t n0, int k, double accuracy, hipStream_t s);
 hipError_t launch_field_from_psi(Run &R, int imposed, double field_factor, hipStream_t s);
 hipError_t launch_zero_psi(Run &R, hipStream_t s);

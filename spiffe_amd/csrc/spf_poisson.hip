@@ -4,14 +4,16 @@
 //
 // Red-black SOR: within one half-sweep every updated point reads only points
 // of the other colour, so the serial loop order of the reference does not
-// affect Psi; the device updates one colour per launch and gets the same
-// bits.  The convergence test needs sum |residual| after every half-sweep:
-// each block writes a partial sum, the last block to finish adds the partials
-// in block order (fixed, so deterministic) and sets a done flag that makes
-// the remaining queued half-sweeps return at once.  The host checks the flag
-// once per batch of launches.  The sum's rounding differs from the serial
-// sum, so the iteration count can differ from the reference only when the
-// residual lands within rounding of the threshold.
+// affect Psi; the device gets the same bits with any schedule that keeps the
+// colour order (here: several half-sweeps per launch on LDS tiles).  The
+// convergence test needs sum |residual| after every half-sweep: each tile
+// writes one partial per half-sweep and a one-workgroup check kernel adds
+// them in tile order (fixed, so deterministic) and sets a done flag that makes
+// the queued batches return at once.  The sum's rounding differs from the
+// serial sum, so the iteration count can differ from the reference only when
+// the residual lands within rounding of the threshold.
+#include <cmath>
+
 #include "spf_internal.h"
 
 namespace spf {
@@ -20,20 +22,31 @@ namespace {
 
 // ---- temporally blocked red-black SOR ----
 // A batch launch performs up to SK consecutive half-sweeps.  Each workgroup
-// owns an STZ x STR tile of points, loads it with an SK-point halo into LDS
-// (clipped at the array edges), and updates colour n%2 for n = n0 .. n0+k-1
-// in LDS.  At local step j only points at least j+1 away from a halo edge
-// that is not an array edge are updated, so after the batch every owned
-// point holds exactly the value the one-colour-per-sweep order produces.
-// Psi is read from one buffer and the owned points written to the other.
-// Each workgroup also sums |residual| over its owned points for every step
-// (fixed tree order) into partials[j][tile]; sor_check_kernel then adds the
-// tiles in order and applies the stopping rule of src/poisson.c:125-126 for
-// each half-sweep of the batch in turn.
-constexpr int STZ = 128, STR = 16, SK = 8;
-constexpr int SLZ = STZ + 2 * SK, SLR = STR + 2 * SK;
-constexpr int SB = 1024;   // SOR batch workgroup: 16 wavefronts share one LDS tile
-constexpr int CB = 256;    // check kernel
+// owns an OZ x OR tile of points and holds it with an SK-point halo (clipped
+// at the array edges) in registers: lane l of wavefront w keeps columns
+// zlo+2l and zlo+2l+1 of tile rows w*RS .. w*RS+RS-1, with their right-hand
+// side and update mask.  A half-sweep updates colour n%2, which in every row
+// is one of the lane's two columns; the other-colour neighbours come from the
+// lane itself, its neighbour lanes (DPP shifts) and, for the first and last
+// rows, the adjacent wavefronts through LDS.  At local step j only points at
+// least j+1 away from a halo edge that is not an array edge are updated, so
+// after the batch every owned point holds exactly the value the
+// one-colour-per-sweep order produces.  Psi is read from one buffer and the
+// owned points written to the other.  Each workgroup also sums |residual|
+// over its owned points for every step (fixed order) into partials[j][tile];
+// sor_check_kernel then adds the tiles in order and applies the stopping rule
+// of src/poisson.c:125-126 for each half-sweep in turn.
+constexpr int SK = 8;                   // half-sweeps per batch = halo width
+constexpr int TZ = 128;                 // tile columns: 64 lanes x 2
+constexpr int OZ = TZ - 2 * SK;         // owned columns per tile: 112
+constexpr int RS = 6;                   // rows per lane
+constexpr int SW = 6;                   // wavefronts per tile
+constexpr int TRW = RS * SW;            // tile rows: 36
+constexpr int OR = TRW - 2 * SK;        // owned rows per tile: 20
+constexpr int SB = SW * 64;             // SOR batch workgroup
+constexpr int CB = 1024;                // check kernel: 32 lanes per half-sweep, 32 half-sweeps
+static_assert(OZ % 2 == 0 && OR % 2 == 0 && SK % 2 == 0 && RS % 2 == 0,
+              "tile origins and lane rows must be even so each row's colour column is fixed");
 
 // fixed-order sum over a workgroup: butterfly within each wavefront, then the
 // wavefront sums in order (the same operations every launch)
@@ -48,16 +61,50 @@ __device__ double block_sum_det(double v, double *wsum) {
   return t;
 }
 
+// lane l receives lane l-1's value (wave_shr:1); lane 0 receives 0
+__device__ inline double from_left(double v) {
+  int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// lane l receives lane l+1's value (wave_shl:1); lane 63 receives 0
+__device__ inline double from_right(double v) {
+  int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// x / d, correctly rounded, for |x| in (1e-280, 1e280) and a host-checked
+// y = RN(1/d): q0 = RN(x*y) and two remainder corrections
+// q <- RN(q + RN(x - d*q)*y) give RN(x/d) (Markstein's correction theorem:
+// the first step makes q faithful, the second rounds it correctly; nothing
+// underflows in that range).  Callers divide directly outside it.
+__device__ inline double div_fast(double x, double d, double y) {
+  double q = x * y;
+  double r = __builtin_fma(-q, d, x);
+  q = __builtin_fma(r, y, q);
+  r = __builtin_fma(-q, d, x);
+  return __builtin_fma(r, y, q);
+}
+
+__device__ inline bool div_fast_ok(double x) {
+  double ax = fabs(x);
+  return ax > 1e-280 && ax < 1e280;
+}
+
 struct SorArgs {
   int nz, nr, ldz, left_dir, right_dir;
   double a, e;
+  double y_e, y_axis;    // RN(1/e), RN(1/(e - 2*c[0]))
+  int use_y;             // 1: both reciprocals are safe for div_fast
   const double *c, *d;   // per-row stencil coefficients
   const double *omega;   // per-half-sweep omega
   const double *rhs;     // may be null
   const double *Pin;
   double *Pout;
   const int16_t *flags;
-  double *partials;      // [SK][ntiles]
+  double *partials;      // [SK][ntiles] of this batch
   const int64_t *state;  // [0] done
   int64_t n0;
   int k;                 // half-sweeps in this batch
@@ -65,72 +112,190 @@ struct SorArgs {
   int tiles_z, ntiles;
 };
 
+// Per-lane constants of one tile.  Point (i, col) of a lane is bit 2*i+col
+// of the masks.  A point is updated at local step j when its update-mask bit
+// is set and it lies at least j+1 points from every shrinking halo edge:
+// zdepth[col] > j and rdepth[i] > j.
+struct SorLane {
+  unsigned upd;     // update mask: in the array interior, not metal (src/poisson.c:63-70)
+  unsigned own;     // point belongs to the tile's owned region
+  int zdepth[2];    // distance of each column from the nearest shrinking z edge
+  int rdepth[RS];   // distance of each row from the nearest shrinking r edge (wave-uniform)
+  bool l_mirror;    // column 0 is iz == 0: L takes R's value
+  bool r_mirror[2]; // column is iz == nz-1: R takes L's value
+};
+
+// One half-sweep of colour C on the registers of this lane.  Row i of the
+// lane is tile row w*RS + i, whose parity is the parity of i; its colour-C
+// point is column (i + C) & 1 of the lane's pair.
+template <int C>
+__device__ inline double sor_half_sweep(const SorArgs &a, const SorLane &ln, double (&P)[RS][2],
+                                        const double (&D)[RS][2], const double2 up_row, const double2 dn_row,
+                                        const double (&cr)[RS], const double (&dr)[RS], int j, double omega,
+                                        bool axis_wave) {
+  const double A = a.a, E = a.e;
+  unsigned okb = ln.upd & ((ln.zdepth[0] > j ? 0x555u : 0u) | (ln.zdepth[1] > j ? 0xAAAu : 0u));
+#pragma unroll
+  for (int i = 0; i < RS; i++)
+    if (ln.rdepth[i] <= j) okb &= ~(3u << (2 * i));
+  double q[RS], x[RS], res[RS];
+  bool slow = false;
+#pragma unroll
+  for (int i = 0; i < RS; i++) {
+    const int col = (i + C) & 1;
+    double L, R;
+    if (col == 0) {
+      R = P[i][1];
+      L = from_left(R);
+      L = ln.l_mirror ? R : L;
+      R = ln.r_mirror[0] ? L : R;
+    } else {
+      L = P[i][0];
+      R = from_right(L);
+      R = ln.r_mirror[1] ? L : R;
+    }
+    const double zs = R + L;
+    const double up = i < RS - 1 ? P[i + 1 < RS ? i + 1 : i][col] : (col ? up_row.y : up_row.x);
+    const double down = i > 0 ? P[i > 0 ? i - 1 : 0][col] : (col ? dn_row.y : dn_row.x);
+    const double c = cr[i];
+    double residual, diag, y;
+    if (i == 0) {
+      // the axis row (src/poisson.c:84-90) has no "below" term, added as
+      // -0.0 (an exact no-op), a 4*c "above" weight and diagonal e - 2*c
+      const double cu = axis_wave ? 4 * c : c;
+      const double dn = axis_wave ? -0.0 : dr[i] * down;
+      diag = axis_wave ? E - 2 * c : E;
+      y = axis_wave ? a.y_axis : a.y_e;
+      residual = A * zs + cu * up + dn + diag * P[i][col] - D[i][col];
+    } else {
+      diag = E;
+      y = a.y_e;
+      residual = A * zs + c * up + dr[i] * down + E * P[i][col] - D[i][col];
+    }
+    res[i] = residual;
+    x[i] = omega * residual;
+    q[i] = div_fast(x[i], diag, y);
+    slow |= !div_fast_ok(x[i]);
+  }
+  if (slow || !a.use_y) {
+#pragma unroll
+    for (int i = 0; i < RS; i++) {
+      const double diag = (i == 0 && axis_wave) ? E - 2 * cr[0] : E;
+      if (!a.use_y || !div_fast_ok(x[i])) q[i] = x[i] / diag;
+    }
+  }
+  double acc = 0;
+#pragma unroll
+  for (int i = 0; i < RS; i++) {
+    const int col = (i + C) & 1;
+    const bool ok = (okb >> (2 * i + col)) & 1;
+    const double nv = P[i][col] - q[i];
+    P[i][col] = ok ? nv : P[i][col];
+    acc += (ok && ((ln.own >> (2 * i + col)) & 1)) ? fabs(res[i]) : 0.0;
+  }
+  return acc;
+}
+
 __global__ __launch_bounds__(SB) void sor_batch_kernel(SorArgs a) {
-  __shared__ double P[SLR][SLZ];
-  __shared__ double D[SLR][SLZ];
-  __shared__ uint8_t M[SLR][SLZ];
-  __shared__ double wsum[SB / 64];
-  if (!a.force && a.state[0]) return;
+  __shared__ double2 xrow[2][SW][2][64];   // [parity][wave][first,last row][lane]
+  __shared__ double wsum[SK][SW];
+  const int64_t done = a.force ? 0 : a.state[0];
   const int tile = blockIdx.x;
-  const int z0 = (tile % a.tiles_z) * STZ, r0 = (tile / a.tiles_z) * STR;
-  // owned region covers the whole array [0, nz] x [0, nr] so every value is carried over
-  const int zo1 = min(z0 + STZ, a.nz + 1), ro1 = min(r0 + STR, a.nr + 1);
-  const int zlo = max(z0 - SK, 0), zhi = min(z0 + STZ + SK, a.nz + 1);
-  const int rlo = max(r0 - SK, 0), rhi = min(r0 + STR + SK, a.nr + 1);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int z0 = (tile % a.tiles_z) * OZ, r0 = (tile / a.tiles_z) * OR;
+  const int zo1 = min(z0 + OZ, a.nz + 1), ro1 = min(r0 + OR, a.nr + 1);
+  const int zlo = max(z0 - SK, 0), rlo = max(r0 - SK, 0);
+  const int zhi = min(z0 + OZ + SK, a.nz + 1), rhi = min(r0 + OR + SK, a.nr + 1);
   const int LZ = zhi - zlo, LR = rhi - rlo;
-  for (int t = threadIdx.x; t < LZ * LR; t += SB) {
-    int lr = t / LZ, lz = t % LZ;
-    int iz = zlo + lz, ir = rlo + lr;
-    int64_t p = fix(a.ldz, iz, ir);
-    P[lr][lz] = a.Pin[p];
-    bool in_range = (iz >= 1 && iz <= a.nz - 2) || (iz == 0 && a.left_dir) || (iz == a.nz - 1 && a.right_dir);
-    bool upd = ir <= a.nr - 2 && iz <= a.nz - 1 && in_range && !(a.flags[p] & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE));
-    M[lr][lz] = upd ? 1 : 0;
-    D[lr][lz] = (upd && a.rhs) ? a.rhs[p] : 0.0;
+  // load this lane's points, right-hand side, flags and row coefficients;
+  // every address is clamped into the array so all loads issue together
+  double P[RS][2], D[RS][2], cr[RS], dr[RS];
+  int16_t fl[RS][2];
+  const int iz_e = zlo + 2 * lane;
+  const double *rhs = a.rhs ? a.rhs : a.Pin;
+#pragma unroll
+  for (int i = 0; i < RS; i++) {
+    const int irc = min(rlo + w * RS + i, a.nr);
+    cr[i] = a.c[irc];
+    dr[i] = a.d[irc];
+#pragma unroll
+    for (int col = 0; col < 2; col++) {
+      const int64_t p = fix(a.ldz, min(iz_e + col, a.nz), irc);
+      P[i][col] = a.Pin[p];
+      D[i][col] = rhs[p];
+      fl[i][col] = a.flags[p];
+    }
+  }
+  if (done) return;
+  SorLane ln;
+  ln.upd = 0;
+  ln.own = 0;
+  const bool sz_lo = zlo > 0, sz_hi = zhi < a.nz + 1, sr_lo = rlo > 0, sr_hi = rhi < a.nr + 1;
+#pragma unroll
+  for (int col = 0; col < 2; col++) {
+    const int lz = 2 * lane + col;
+    ln.zdepth[col] = min(sz_lo ? lz : SK + 1, sz_hi ? LZ - 1 - lz : SK + 1);
+    ln.r_mirror[col] = iz_e + col == a.nz - 1;
+  }
+  ln.l_mirror = iz_e == 0;
+#pragma unroll
+  for (int i = 0; i < RS; i++) {
+    const int lr = w * RS + i, ir = rlo + lr;
+    ln.rdepth[i] = min(sr_lo ? lr : SK + 1, sr_hi ? LR - 1 - lr : SK + 1);
+#pragma unroll
+    for (int col = 0; col < 2; col++) {
+      const int iz = iz_e + col;
+      const bool in = lr < LR && 2 * lane + col < LZ;
+      const bool in_range =
+          (iz >= 1 && iz <= a.nz - 2) || (iz == 0 && a.left_dir) || (iz == a.nz - 1 && a.right_dir);
+      const bool upd = in && ir <= a.nr - 2 && iz <= a.nz - 1 && in_range &&
+                       !(fl[i][col] & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE));
+      if (upd) ln.upd |= 1u << (2 * i + col);
+      if (iz >= z0 && iz < zo1 && ir >= r0 && ir < ro1) ln.own |= 1u << (2 * i + col);
+      D[i][col] = (upd && a.rhs) ? D[i][col] : 0.0;
+    }
+  }
+  const bool axis_wave = rlo == 0 && w == 0;
+  double ares[SK];
+#pragma unroll
+  for (int j = 0; j < SK; j++) ares[j] = 0;
+  for (int j = 0; j < a.k; j++) {
+    const int par = j & 1;
+    // publish the first and last rows, then read the neighbouring wavefronts' rows
+    xrow[par][w][0][lane] = make_double2(P[0][0], P[0][1]);
+    xrow[par][w][1][lane] = make_double2(P[RS - 1][0], P[RS - 1][1]);
+    __syncthreads();
+    const double2 up_row = w + 1 < SW ? xrow[par][w + 1][0][lane] : make_double2(0.0, 0.0);
+    const double2 dn_row = w > 0 ? xrow[par][w - 1][1][lane] : make_double2(0.0, 0.0);
+    const double omega = a.omega[a.n0 + j];
+    double acc;
+    if (((a.n0 + j) & 1) == 0)
+      acc = sor_half_sweep<0>(a, ln, P, D, up_row, dn_row, cr, dr, j, omega, axis_wave);
+    else
+      acc = sor_half_sweep<1>(a, ln, P, D, up_row, dn_row, cr, dr, j, omega, axis_wave);
+#pragma unroll
+    for (int jj = 0; jj < SK; jj++)
+      if (jj == j) ares[jj] = acc;
+  }
+  // fixed-order sums of the owned residuals, one per half-sweep: butterflies
+  // within each wavefront, then the wavefronts in order
+#pragma unroll
+  for (int j = 0; j < SK; j++) {
+    double v = ares[j];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) wsum[j][w] = v;
   }
   __syncthreads();
-  const double A = a.a, E = a.e;
-  for (int j = 0; j < a.k; j++) {
-    const int64_t n = a.n0 + j;
-    const int color = (int)(n & 1);
-    const double omega = a.omega[n];
-    // shrink only at halo edges that are not array edges
-    const int sz0 = zlo > 0 ? j + 1 : 0, sz1 = zhi < a.nz + 1 ? j + 1 : 0;
-    const int sr0 = rlo > 0 ? j + 1 : 0, sr1 = rhi < a.nr + 1 ? j + 1 : 0;
-    double ares = 0;
-    const int half = LZ / 2 + 1;   // covers both parities of [zlo, zhi)
-    for (int t = threadIdx.x; t < half * LR; t += SB) {
-      int lr = t / half, kk = t % half;
-      int ir = rlo + lr;
-      int iz = 2 * (zlo / 2 + kk) + ((color + ir) & 1);   // (iz + ir) % 2 == n % 2
-      int lz = iz - zlo;
-      if (lz < sz0 || lz >= LZ - sz1 || lr < sr0 || lr >= LR - sr1) continue;
-      if (!M[lr][lz]) continue;
-      double zs;
-      if (iz == 0)
-        zs = P[lr][lz + 1] + P[lr][lz + 1];
-      else if (iz == a.nz - 1)
-        zs = P[lr][lz - 1] + P[lr][lz - 1];
-      else
-        zs = P[lr][lz + 1] + P[lr][lz - 1];
-      double up = P[lr + 1][lz], p0 = P[lr][lz], c = a.c[ir], residual;
-      if (ir == 0) {
-        residual = A * zs + 4 * c * up + (E - 2 * c) * p0 - D[lr][lz];
-        P[lr][lz] = p0 - omega * residual / (E - 2 * c);
-      } else {
-        residual = A * zs + c * up + a.d[ir] * P[lr - 1][lz] + E * p0 - D[lr][lz];
-        P[lr][lz] = p0 - omega * residual / E;
-      }
-      if (iz >= z0 && iz < zo1 && ir >= r0 && ir < ro1) ares += fabs(residual);
-    }
-    // fixed-order block sum of the owned residuals of this half-sweep
-    double tot = block_sum_det(ares, wsum);
-    if (threadIdx.x == 0) a.partials[(int64_t)j * a.ntiles + tile] = tot;
+  if (threadIdx.x < a.k) {
+    double tot = 0;
+    for (int k = 0; k < SW; k++) tot += wsum[threadIdx.x][k];
+    a.partials[(int64_t)threadIdx.x * a.ntiles + tile] = tot;
   }
-  for (int t = threadIdx.x; t < (zo1 - z0) * (ro1 - r0); t += SB) {
-    int w = zo1 - z0;
-    int iz = z0 + t % w, ir = r0 + t / w;
-    a.Pout[fix(a.ldz, iz, ir)] = P[ir - rlo][iz - zlo];
+#pragma unroll
+  for (int i = 0; i < RS; i++) {
+#pragma unroll
+    for (int col = 0; col < 2; col++)
+      if ((ln.own >> (2 * i + col)) & 1) a.Pout[fix(a.ldz, iz_e + col, rlo + w * RS + i)] = P[i][col];
   }
 }
 
@@ -150,8 +315,20 @@ __global__ __launch_bounds__(CB) void sor_check_kernel(CheckArgs a) {
   if (a.state[0]) return;
   const int j = threadIdx.x >> 5, l = threadIdx.x & 31;
   double s = 0;
-  if (j < a.k)
-    for (int t = l; t < a.ntiles; t += 32) s += a.partials[(int64_t)j * a.ntiles + t];
+  if (j < a.k) {
+    const double *row = a.partials + (int64_t)j * a.ntiles;
+    // 8 independent loads in flight per lane, then added in order
+    for (int t0 = l; t0 < a.ntiles; t0 += 32 * 8) {
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        int t = t0 + 32 * u;
+        x[u] = t < a.ntiles ? row[t] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += x[u];
+    }
+  }
   for (int off = 16; off > 0; off >>= 1) s += __shfl_xor(s, off, 32);
   if (l == 0) tot[j] = s;
   __syncthreads();
@@ -312,14 +489,14 @@ hipError_t launch_sor_reset(Run &R, hipStream_t s) {
 }
 
 int64_t sor_tiles(const Run &R) {
-  int64_t tz = (R.d.nz + 1 + STZ - 1) / STZ, tr = (R.d.nr + 1 + STR - 1) / STR;
+  int64_t tz = (R.d.nz + 1 + OZ - 1) / OZ, tr = (R.d.nr + 1 + OR - 1) / OR;
   return tz * tr;
 }
 
 int sor_batch() { return SK; }
 
-hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const double *rhs, int64_t n0, int k, int force,
-                            hipStream_t s) {
+hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const double *rhs, int64_t n0, int k,
+                            int partial_row, int force, hipStream_t s) {
   SorArgs a;
   a.nz = R.d.nz;
   a.nr = R.d.nr;
@@ -329,6 +506,12 @@ hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const doubl
   double dz = R.d.dz, dr = R.d.dr;
   a.a = 1 / (dz * dz);
   a.e = -2 * (1 / (dz * dz) + 1 / (dr * dr));
+  // reciprocals for div_fast, computed with the device's operands
+  const double diag0 = a.e - 2 * R.sor_c_host0;
+  a.y_e = 1 / a.e;
+  a.y_axis = 1 / diag0;
+  auto safe = [](double v) { double m = std::fabs(v); return m > 1e-200 && m < 1e200; };
+  a.use_y = safe(a.e) && safe(diag0) && safe(a.y_e) && safe(a.y_axis);
   a.c = R.sor_c;
   a.d = R.sor_d;
   a.omega = R.omega;
@@ -336,12 +519,12 @@ hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const doubl
   a.Pin = Pin;
   a.Pout = Pout;
   a.flags = R.flags;
-  a.partials = R.red;
+  a.partials = R.red + (int64_t)partial_row * sor_tiles(R);
   a.state = R.sor_state;
   a.n0 = n0;
   a.k = k;
   a.force = force;
-  a.tiles_z = (R.d.nz + 1 + STZ - 1) / STZ;
+  a.tiles_z = (R.d.nz + 1 + OZ - 1) / OZ;
   a.ntiles = (int)sor_tiles(R);
   hipLaunchKernelGGL(sor_batch_kernel, dim3((unsigned)a.ntiles), dim3(SB), 0, s, a);
   return hipGetLastError();
