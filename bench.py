@@ -31,6 +31,10 @@ HBM_PEAK_GBS = 8000.0
 B_CELL_TMTE = 146.0      # TM+TE cell-update: reads Ez Er Bphi Jz Jr Ephi Br Bz Jphi + flag, writes 9 arrays
 B_PUSH = 196.0           # particle-push: 12 doubles + status read, 12 doubles written
 B_SOR = 22.0             # SOR point of one half-sweep: Psi rhs flag read, half of points written
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE on this workload, scaled by
+# the counters' reading for a plain copy of known size (profiles/); None where not measured
+TRAFFIC = {}
+TIMER_EVERY = 8
 
 
 class HipTimer:
@@ -92,12 +96,13 @@ def cpu_baseline(g, f, n_particles, budget_s):
         pushes += int(o.b.np)
         o.step(screens=W.GUN_SCREENS, compat=False)
         steps += 1
-        if time.perf_counter() - t0 > budget_s and steps >= 11:
+        if time.perf_counter() - t0 > budget_s and steps % 10 == 0:
             break
     dt = time.perf_counter() - t0
     return {"value": pushes / dt, "unit": "particle-pushes/s", "cores": 1, "kind": "port",
             "sample": "%d whole steps (incl. %d Poisson corrections) of the same 1M-particle gun workload, "
                       "%.1f s, serial restatement built -O2 -ffp-contract=off" % (steps, (steps + 9) // 10, dt)}
+
 
 
 def main():
@@ -106,9 +111,20 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--particles", type=int, default=1_000_000)
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 512 MiB device copy (5 times), the reference for HBM byte counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8, "copies": 5}))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU: independent gun runs shard across GPUs with no exchange, "
               "and this build measures the single-GPU path only (--gpus %d requested)" % args.gpus,
@@ -127,50 +143,54 @@ def main():
         run.step()
     ctx.synchronize()
     np0, _ = run.count()
-    # timed region: K whole steps
+    # timed region: K whole steps; the library's HIP-event timers on its stream sample one
+    # call in TIMER_EVERY of each phase (event markers cost a few microseconds each)
+    ctx.timing(TIMER_EVERY)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         run.step()
     ctx.synchronize()
     wall = time.perf_counter() - t0
+    tm = ctx.timings()
+    ctx.timing(0)
     np1, _ = run.count()
     pushes = 0.5 * (np0 + np1) * args.steps
     ms_per_step = 1e3 * wall / args.steps
 
-    # per-kernel timing with HIP events on the library stream
-    reps = 50
-    npk, _ = run.count()
-    t_fdtd = timer.time_ms(stream, run.advance_field_solutions, reps) / reps          # one fdtd_kernel launch
-    t_sor = timer.time_ms(stream, lambda: run.poisson_sweeps(0, 64, 8), reps) / reps   # one sor_batch_kernel launch
-    t_push = timer.time_ms(stream, run.advance_particles, reps) / reps                 # push + event compaction
-    t_dep = timer.time_ms(stream, lambda: (run.zero_current_arrays(), run.add_particle_currents()), reps) / reps
-    run.set_step(0)
-    t_pc0 = time.perf_counter()
-    it = run.add_poisson_correction(True)
-    t_pc = (time.perf_counter() - t_pc0) * 1e3
-    half_sweeps = it + 1 if it >= 0 else W.GUN_POISSON["maximum_iterations"]
-    interval = W.GUN_POISSON["step_interval"]
-    per_step = {"fdtd_ms": t_fdtd, "push_ms": t_push, "deposit_ms": t_dep, "sor_batch_ms": t_sor,
-                "poisson_ms_per_correction": t_pc, "poisson_ms_per_step": t_pc / interval}
-    # time per step spent in each kernel
-    share = {"fdtd_kernel": t_fdtd, "push phase": t_push, "deposit phase": t_dep,
-             "sor_batch_kernel": t_sor * (half_sweeps / 8.0) / interval}
+    def avg(name):   # mean device ms per launch over the sampled launches
+        ms, n, _ = tm[name]
+        return ms / n if n else None
+
+    def per_step(name):   # sampled mean per launch x launches per step
+        ms, n, launched = tm[name]
+        return ms / n * launched / args.steps if n else 0.0
+
+    t_fdtd, t_push, t_dep, t_sor = avg("fdtd"), avg("push"), avg("deposit"), avg("sor_batch")
+    phase = {"fdtd_ms": t_fdtd, "push_ms": t_push, "deposit_ms": t_dep,
+             "sor_batch_ms": t_sor, "sor_batches": tm["sor_batch"][2], "sor_batches_timed": tm["sor_batch"][1],
+             "poisson_ms_per_correction": avg("poisson"), "poisson_corrections": tm["poisson"][2],
+             "poisson_ms_per_step": per_step("poisson")}
+    # device time per step spent in each kernel or phase
+    share = {"fdtd_kernel": per_step("fdtd"), "push phase": per_step("push"), "deposit phase": per_step("deposit"),
+             "sor_batch_kernel": per_step("sor_batch")}
     cells_all = (g.nz + 1) * (g.nr + 1)
+    npk = 0.5 * (np0 + np1)
     algo = {  # algorithmic bytes per launch and measured ms per launch
         "fdtd_kernel": (cells * B_CELL_TMTE, t_fdtd, "%d cells x %d B (TM+TE cell-update)" % (cells, B_CELL_TMTE)),
         "sor_batch_kernel": (8 * cells_all * B_SOR, t_sor,
                              "8 half-sweeps x %d points x %d B (SOR point of one half-sweep)" % (cells_all, B_SOR)),
         "push phase": (npk * B_PUSH, t_push, "%d particles x %d B (particle-push)" % (npk, B_PUSH)),
     }
-    dominant = max((k for k in share if k in algo), key=share.get)
+    dominant = max((k for k in share if k in algo and algo[k][1]), key=share.get)
     byts, ms, what = algo[dominant]
     ach = byts / (ms * 1e-3) / 1e9
     rl = {"bound": "hbm", "kernel": dominant, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-          "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": byts,
+          "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC.get(dominant), "algorithmic_bytes_per_launch": byts,
           "ms_per_launch": ms, "units": what}
-    fdtd_gbs = cells * B_CELL_TMTE / (t_fdtd * 1e-3) / 1e9
-    push_gbs = npk * B_PUSH / (t_push * 1e-3) / 1e9
-    sor_gbs = 8 * cells_all * B_SOR / (t_sor * 1e-3) / 1e9
+    gbs = {k: v[0] / (v[1] * 1e-3) / 1e9 for k, v in algo.items() if v[1]}
+    # iteration count of one correction (outside the timed region)
+    run.set_step(0)
+    it = run.add_poisson_correction(True)
 
     cpu = None
     if not args.no_cpu:
@@ -184,10 +204,8 @@ def main():
                                "TM+TE, Poisson every 10 steps to 1e-8, screen z=10 cm",
                    "grid_lines": [g.nz, g.nr], "cells": cells, "particles": args.particles, "dt": g.dt},
         "cell_updates_per_s": cells * args.steps / wall,
-        "phase_ms": per_step, "poisson_iterations": it, "kernel_ms_per_step": share,
-        "fdtd_gbs": fdtd_gbs, "fdtd_frac": fdtd_gbs / HBM_PEAK_GBS,
-        "push_gbs": push_gbs, "push_frac": push_gbs / HBM_PEAK_GBS,
-        "sor_gbs": sor_gbs, "sor_frac": sor_gbs / HBM_PEAK_GBS,
+        "phase_ms": phase, "poisson_iterations": it, "kernel_ms_per_step": share,
+        "hbm_frac": {k: v / HBM_PEAK_GBS for k, v in gbs.items()},
         "roofline": rl, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -205,6 +205,28 @@ int spf_poisson_sweeps(spf_ctx *ctx, int32_t run, int32_t which, int64_t n0, int
 /* replaces compute_imposed_Efield, src/poisson_correction.c:456-562 */
 int spf_imposed_efield(spf_ctx *ctx, int32_t run, int32_t initialize, double ramp_time, double flat_top_time);
 
+/* ---- phase timers ----
+ * No counterpart in the reference: telemetry for benchmarks and serving loops.
+ * When enabled, a pair of HIP events on the context's stream brackets one call
+ * in every `every` calls of each phase below; a timer accumulates the device
+ * time between its events and the number of launches they cover.  Event
+ * markers add a few microseconds between kernels, so sparse sampling keeps the
+ * measured run close to an untimed one. */
+#define SPF_T_STEP 0       /* spf_step */
+#define SPF_T_DEPOSIT 1    /* spf_currents_deposit */
+#define SPF_T_FDTD 2       /* spf_fields_advance */
+#define SPF_T_PUSH 3       /* spf_particles_push: push, compaction and event rows */
+#define SPF_T_POISSON 4    /* spf_poisson_correct, when it solves */
+#define SPF_T_SOR_BATCH 5  /* SOR batch kernel launches (up to 8 half-sweeps each), timed per
+                              group of back-to-back launches between stopping-rule checks */
+#define SPF_T_N 6
+/* every > 0 starts timing one call in `every` of each phase and clears the
+ * accumulators; 0 stops it */
+int spf_timing_enable(spf_ctx *ctx, int32_t every);
+/* synchronizing: total milliseconds over the timed launches, the number of timed
+ * launches, and (may be NULL) all launches since timing started; arrays of SPF_T_N */
+int spf_timing_read(spf_ctx *ctx, double *ms, int64_t *timed, int64_t *launched);
+
 /* ---- one whole step of the loop body, src/integrate.c:134-194 ----
  * time <- step*dt+start; J reset; deposit (if SPF_SPACE_CHARGE); n_lines line
  * currents; Poisson correction (if set up); FDTD; push; step++. */

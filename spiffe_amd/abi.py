@@ -27,6 +27,9 @@ FIELDS = ["Ez", "Er", "Bphi", "Bphi2", "Ephi", "Br", "Br2", "Bz", "Bz2", "Jz", "
           "EzImposed", "ErImposed", "PsiImposed", "BzImposed", "BrImposed"]
 FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 
+# phase timers
+TIMERS = ["step", "deposit", "fdtd", "push", "poisson", "sor_batch"]
+
 PARTICLE_DOUBLES = ["Q", "z", "r", "pz", "pr", "r_pphi", "vz", "vr", "vphi",
                     "r0", "z0", "t0", "rmax", "rmin", "zmax", "zmin"]
 
@@ -119,6 +122,8 @@ SIGNATURES = {
     "spf_poisson_solve": (ctypes.c_int, [_vp, _i32, _i32, _d, _i64, _pi64]),
     "spf_poisson_sweeps": (ctypes.c_int, [_vp, _i32, _i32, _i64, _i64]),
     "spf_imposed_efield": (ctypes.c_int, [_vp, _i32, _i32, _d, _d]),
+    "spf_timing_enable": (ctypes.c_int, [_vp, _i32]),
+    "spf_timing_read": (ctypes.c_int, [_vp, _dp, _pi64, _pi64]),
     "spf_step": (ctypes.c_int, [_vp, _i32, _i32, _pi32, _pi64, _pi64, _pi64, _dp]),
 }
 

@@ -56,7 +56,87 @@ int64_t default_lrow_cap(const spf_run_desc *d) {
 
 }  // namespace
 
+PhaseTimer::PhaseTimer(spf_ctx *ctx, int timer_id, int64_t launches) : c(ctx), id(timer_id), n(launches) {
+  if (!c || c->timing_every <= 0 || n <= 0) return;
+  c->t_launched[id] += n;
+  if (c->t_calls[id]++ % c->timing_every) return;
+  if (!c->ev_free.empty()) {
+    e0 = c->ev_free.back();
+    c->ev_free.pop_back();
+  } else if (hipEventCreate(&e0) != hipSuccess) {
+    e0 = nullptr;
+    return;
+  }
+  if (hipEventRecord(e0, c->stream) != hipSuccess) {
+    c->ev_free.push_back(e0);
+    e0 = nullptr;
+  }
+}
+
+PhaseTimer::~PhaseTimer() {
+  if (!e0) return;
+  hipEvent_t e1 = nullptr;
+  if (!c->ev_free.empty()) {
+    e1 = c->ev_free.back();
+    c->ev_free.pop_back();
+  } else if (hipEventCreate(&e1) != hipSuccess) {
+    c->ev_free.push_back(e0);
+    return;
+  }
+  if (hipEventRecord(e1, c->stream) != hipSuccess) {
+    c->ev_free.push_back(e0);
+    c->ev_free.push_back(e1);
+    return;
+  }
+  c->ev_pending.push_back({id, n, e0, e1});
+  if (c->ev_pending.size() >= 4096) resolve_timers(c, false);
+}
+
+// adds the elapsed times of completed event pairs, oldest first; with wait,
+// the stream has been synchronized and every pair is complete
+void spf::resolve_timers(spf_ctx *c, bool wait) {
+  size_t k = 0;
+  for (; k < c->ev_pending.size(); k++) {
+    spf_ctx::Pending &p = c->ev_pending[k];
+    if (!wait && hipEventQuery(p.e1) != hipSuccess) break;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, p.e0, p.e1) == hipSuccess) {
+      c->t_ms[p.id] += ms;
+      c->t_count[p.id] += p.n;
+    }
+    c->ev_free.push_back(p.e0);
+    c->ev_free.push_back(p.e1);
+  }
+  c->ev_pending.erase(c->ev_pending.begin(), c->ev_pending.begin() + k);
+}
+
 extern "C" {
+
+int spf_timing_enable(spf_ctx *c, int32_t enable) {
+  if (!c) return SPF_EINVAL;
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  resolve_timers(c, true);
+  for (int k = 0; k < SPF_T_N; k++) {
+    c->t_ms[k] = 0;
+    c->t_count[k] = 0;
+    c->t_calls[k] = 0;
+    c->t_launched[k] = 0;
+  }
+  c->timing_every = enable > 0 ? enable : 0;
+  return SPF_OK;
+}
+
+int spf_timing_read(spf_ctx *c, double *ms, int64_t *timed, int64_t *launched) {
+  if (!c || !ms || !timed) return SPF_EINVAL;
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  resolve_timers(c, true);
+  for (int k = 0; k < SPF_T_N; k++) {
+    ms[k] = c->t_ms[k];
+    timed[k] = c->t_count[k];
+    if (launched) launched[k] = c->t_launched[k];
+  }
+  return SPF_OK;
+}
 
 int spf_ctx_create(int32_t device, spf_ctx **out) {
   if (!out) return SPF_EINVAL;
@@ -89,6 +169,11 @@ int spf_ctx_destroy(spf_ctx *c) {
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->hstage) (void)hipHostFree(c->hstage);
   if (c->stage_done) (void)hipEventDestroy(c->stage_done);
+  for (auto &p : c->ev_pending) {
+    (void)hipEventDestroy(p.e0);
+    (void)hipEventDestroy(p.e1);
+  }
+  for (hipEvent_t e : c->ev_free) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return SPF_OK;
@@ -417,6 +502,7 @@ int spf_fields_advance(spf_ctx *c, int32_t run) {
   a.cp = R->cp;
   a.cm = R->cm;
   a.bzf = R->bzf;
+  PhaseTimer pt(c, SPF_T_FDTD);
   HIPCALL(c, launch_fdtd(a, c->stream));
   if (a.tm)
     for (int k = 0; k < 3; k++) R->cur[k] = 1 - R->cur[k];
@@ -468,6 +554,7 @@ int spf_currents_add_line(spf_ctx *c, int32_t run, int32_t direction, int64_t ip
 int spf_currents_deposit(spf_ctx *c, int32_t run) {
   Run *R = get_run(c, run);
   if (!R) return fail(c, SPF_EINVAL, "bad run id");
+  PhaseTimer pt(c, SPF_T_DEPOSIT);
   HIPCALL(c, deposit_currents(*R, c->stream));
   return SPF_OK;
 }
@@ -611,6 +698,7 @@ int spf_particles_push(spf_ctx *c, int32_t run, double zForcesStart) {
     a.sdir[s] = R->screen_dir[s];
   }
   RowsArgs w;
+  PhaseTimer pt(c, SPF_T_PUSH);
   HIPCALL(c, launch_set_i64(R->counters + 7, 0, c->stream));
   HIPCALL(c, launch_push(a, R->np_ub, c->stream));
   HIPCALL(c, ordered_compact(R->evflag, R->np_ub, R->evlist, R->counters + 2, R->bcount, R->boff, c->stream));
@@ -654,6 +742,7 @@ int spf_particles_push(spf_ctx *c, int32_t run, double zForcesStart) {
   w.lrow_cap = R->lrow_cap;
   w.move_src = R->move_src;
   w.move_dst = R->move_dst;
+  w.tflag = R->gflag;
   w.overflow = (int32_t *)(R->counters + 8);
   HIPCALL(c, launch_rows(w, c->stream));
   return SPF_OK;

@@ -63,11 +63,14 @@ int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accura
     int64_t win_end = std::min<int64_t>(nb, b + kSorCheckBatches);
     while (b < win_end) {
       int64_t g0 = b, g1 = std::min<int64_t>(win_end, b + kCheck);
-      for (; b < g1; b++) {
-        int64_t n0 = b * K;
-        int k = (int)std::min<int64_t>(K, limit - n0);
-        HIPCALL(c, launch_sor_batch(*R, ring[b % kRing], ring[(b + 1) % kRing], rhs, n0, k,
-                                    (int)((b - g0) * K), 0, s));
+      {
+        PhaseTimer bt(c, SPF_T_SOR_BATCH, g1 - b);   // the group's batch launches, back to back
+        for (; b < g1; b++) {
+          int64_t n0 = b * K;
+          int k = (int)std::min<int64_t>(K, limit - n0);
+          HIPCALL(c, launch_sor_batch(*R, ring[b % kRing], ring[(b + 1) % kRing], rhs, n0, k,
+                                      (int)((b - g0) * K), 0, s));
+        }
       }
       HIPCALL(c, launch_sor_check(*R, g0 * K, (int)(std::min<int64_t>(b * K, limit) - g0 * K), accuracy, s));
     }
@@ -147,9 +150,12 @@ int spf_poisson_sweeps(spf_ctx *c, int32_t run, int32_t which, int64_t n0, int64
   const int K = sor_batch();
   double *buf[2] = {*Psi, R->PsiSpare[0]};
   int64_t nb = (count + K - 1) / K;
-  for (int64_t b = 0; b < nb; b++) {
-    int k = (int)std::min<int64_t>(K, count - b * K);
-    HIPCALL(c, launch_sor_batch(*R, buf[b % 2], buf[(b + 1) % 2], rhs, n0 + b * K, k, 0, 1, c->stream));
+  {
+    PhaseTimer bt(c, SPF_T_SOR_BATCH, nb);
+    for (int64_t b = 0; b < nb; b++) {
+      int k = (int)std::min<int64_t>(K, count - b * K);
+      HIPCALL(c, launch_sor_batch(*R, buf[b % 2], buf[(b + 1) % 2], rhs, n0 + b * K, k, 0, 1, c->stream));
+    }
   }
   if (nb % 2) {
     R->PsiSpare[0] = *Psi;
@@ -167,6 +173,7 @@ int spf_poisson_correct(spf_ctx *c, int32_t run, int32_t space_charge, int64_t *
   if (R->pc_start_step == -1) R->pc_start_step = (int64_t)((p.start_time - R->d.start_time) / R->d.dt);
   if (p.step_interval <= 0 || R->time_step % p.step_interval != 0 || R->pc_start_step > R->time_step) return SPF_OK;
   hipStream_t s = c->stream;
+  PhaseTimer pt(c, SPF_T_POISSON);
   HIPCALL(c, launch_gauss_q(*R, s));
   if (space_charge) HIPCALL(c, launch_q_subtract(*R, s));
   if (p.error_charge_threshold > 0) {
@@ -230,6 +237,7 @@ int spf_step(spf_ctx *c, int32_t run, int32_t n_lines, const int32_t *line_dir, 
   if (n_lines < 0 || (n_lines > 0 && (!line_dir || !line_pos || !line_start || !line_end || !line_J)))
     return fail(c, SPF_EINVAL, "bad line-current arguments");
   int rc;
+  PhaseTimer pt(c, SPF_T_STEP);
   // src/integrate.c:136-138: the ramped imposed field uses the previous step's time
   if (R->ramp_time > 0 && (rc = spf_imposed_efield(c, run, 0, R->ramp_time, R->flat_top_time))) return rc;
   R->time = R->time_step * R->d.dt + R->d.start_time;

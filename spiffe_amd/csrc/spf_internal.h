@@ -80,6 +80,7 @@ struct RowsArgs {
   int64_t *n_lrows;
   int64_t lrow_cap;
   int32_t *move_src, *move_dst;     // scratch, capacity = particle capacity
+  uint8_t *tflag;                   // scratch bytes, capacity >= particle capacity
   int32_t *overflow;
 };
 
@@ -209,6 +210,19 @@ struct spf_ctx {
   void *hstage = nullptr;      // pinned staging for appends
   size_t hstage_bytes = 0;
   hipEvent_t stage_done = nullptr;
+  // phase timers (SPF_T_*): event pairs recorded on the stream, resolved lazily
+  int64_t timing_every = 0;    // 0: off; N: one call in N of each phase is timed
+  int64_t t_calls[SPF_T_N] = {};
+  int64_t t_launched[SPF_T_N] = {};   // launches of each phase while timing, sampled or not
+  struct Pending {
+    int id;
+    int64_t n;                 // launches covered by the event pair
+    hipEvent_t e0, e1;
+  };
+  std::vector<hipEvent_t> ev_free;
+  std::vector<Pending> ev_pending;
+  double t_ms[SPF_T_N] = {};
+  int64_t t_count[SPF_T_N] = {};
 };
 
 namespace spf {
@@ -217,6 +231,22 @@ inline int fail(spf_ctx *c, int code, const std::string &msg) {
   if (c) c->err = msg;
   return code;
 }
+
+// Brackets one phase (covering `launches` launches) with a pair of HIP events
+// on the context's stream when timing is enabled and this call is sampled;
+// the elapsed device time is added to timer `id` later.
+struct PhaseTimer {
+  spf_ctx *c;
+  int id;
+  int64_t n;
+  hipEvent_t e0 = nullptr;
+  PhaseTimer(spf_ctx *ctx, int timer_id, int64_t launches = 1);
+  ~PhaseTimer();
+  PhaseTimer(const PhaseTimer &) = delete;
+  PhaseTimer &operator=(const PhaseTimer &) = delete;
+};
+
+void resolve_timers(spf_ctx *c, bool wait);
 
 inline Run *get_run(spf_ctx *c, int32_t run) {
   if (!c || run < 0 || run >= (int32_t)c->runs.size() || !c->runs[run] || !c->runs[run]->live) return nullptr;

@@ -594,21 +594,38 @@ __global__ __launch_bounds__(ROWS_THREADS) void rows_kernel(RowsArgs a) {
     return;
   }
   // swap-remove with the flag moved too: the k-th lost slot below the new np
-  // receives the k-th survivor counted from the end
+  // receives the k-th survivor counted from the end.  The tail [new_np, np)
+  // holds nlost slots; its survivors, taken in descending order, are the
+  // sources, one per lost slot below new_np.
   int64_t new_np = np - nlost;
+  // m = lost slots below new_np (move_dst is ascending)
+  int cnt = 0;
+  for (int64_t k = threadIdx.x; k < nlost; k += blockDim.x) cnt += a.move_dst[k] < new_np;
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = cnt;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    int64_t m = 0;
-    while (m < nlost && a.move_dst[m] < new_np) m++;
-    int64_t t = nlost - 1, j = np - 1;
-    for (int64_t k = 0; k < m; k++) {
-      while (t >= 0 && a.move_dst[t] == j) {
-        t--;
-        j--;
-      }
-      a.move_src[k] = (int32_t)j;
-      j--;
+    long long t = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) t += wsum[k];
+    sh_first_lost = t;
+  }
+  __syncthreads();
+  {
+    const int64_t m0 = sh_first_lost;
+    for (int64_t t = threadIdx.x; t < nlost; t += blockDim.x) a.tflag[t] = 0;
+    __syncthreads();
+    for (int64_t k = m0 + threadIdx.x; k < nlost; k += blockDim.x) a.tflag[a.move_dst[k] - new_np] = 1;
+    __syncthreads();
+    // survivor of ascending rank q in the tail is the source of lost slot m0-1-q
+    int64_t running = 0;
+    for (int64_t c = 0; c < nlost; c += blockDim.x) {
+      const int64_t t = c + threadIdx.x;
+      const bool surv = t < nlost && !a.tflag[t];
+      int excl;
+      const int tot = block_flag_scan(surv, wsum, excl);
+      if (surv) a.move_src[m0 - 1 - (running + excl)] = (int32_t)(new_np + t);
+      running += tot;
     }
-    sh_first_lost = m;
   }
   __syncthreads();
   int64_t m = sh_first_lost;

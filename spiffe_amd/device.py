@@ -81,6 +81,19 @@ class Context:
     def stream(self):
         return self.lib.spf_stream(self.h)
 
+    def timing(self, every=1):
+        """Starts (and clears) the per-phase HIP-event timers, timing one call in `every`
+        of each phase; every=0 stops them."""
+        self.check(self.lib.spf_timing_enable(self.h, int(every)))
+
+    def timings(self):
+        """{phase: (ms over the timed launches, launches timed, all launches)}; synchronizing."""
+        ms = (ctypes.c_double * len(abi.TIMERS))()
+        cnt = (ctypes.c_int64 * len(abi.TIMERS))()
+        tot = (ctypes.c_int64 * len(abi.TIMERS))()
+        self.check(self.lib.spf_timing_read(self.h, ms, cnt, tot))
+        return {name: (ms[i], cnt[i], tot[i]) for i, name in enumerate(abi.TIMERS)}
+
     def close(self):
         if self.h:
             self.lib.spf_ctx_destroy(self.h)

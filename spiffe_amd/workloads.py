@@ -36,11 +36,12 @@ def gun_flags(grid):
 
 
 def gun_fields(grid, E0=120e6):
-    """A standing-wave Ez pattern, pi-mode-like between the two cells, zero inside metal."""
+    """A standing-wave Ez pattern, pi-mode-like between the two cells, in the
+    phase that accelerates electrons off the cathode (Ez < 0 in the half cell)."""
     z = grid.zmin + grid.dz * (np.arange(grid.nz + 1) - 0.5)
     r = grid.dr * np.arange(grid.nr + 1)
     from scipy.special import j0
-    ez_z = np.where(z < 0.035, 1.0, -1.0) * np.abs(np.cos(np.pi * (z - 0.035) / 0.07))
+    ez_z = np.where(z < 0.035, -1.0, 1.0) * np.abs(np.cos(np.pi * (z - 0.035) / 0.07))
     Ez = E0 * np.outer(ez_z, j0(2.404825557695773 * np.minimum(r, 0.042) / 0.042))
     return {"Ez": Ez}
 

@@ -191,6 +191,65 @@ def test_push_single_step_rows_and_compaction(ctx, compat):
     d.close()
 
 
+@pytest.mark.parametrize("compat", [True, False])
+def test_push_mass_loss_compaction(ctx, compat):
+    """One step where ~40% of 60 000 particles leave the box: lost slots on both sides of
+    the new np and many 1024-particle chunks of the ordered compaction."""
+    g = gun_grid(modes=TMTE)
+    f = gun_flags(g)
+    n = 60000
+    o, d = pair(ctx, g, f, cap=n, compat=compat)
+    fl = random_fields(g, 61, scale=1e6)
+    load_fields(o, d, fl)
+    p = random_particles(g, n, 62, flags=f, pz_scale=0.5)
+    rng = np.random.default_rng(63)
+    out = rng.random(n) < 0.4
+    p["z"][out] = g.zmax - 0.05 * g.dz
+    p["pz"][out] = 5.0
+    gm = np.sqrt(p["pz"] ** 2 + p["pr"] ** 2 + 1)
+    p["vz"] = 299792458.0 * p["pz"] / gm
+    p["vr"] = 299792458.0 * p["pr"] / gm
+    o.append(p)
+    d.append(p)
+    for step in range(2):
+        o.f.time_step = step
+        o.f.time = step * g.dt + g.start_time
+        d.set_step(step)
+        ro, lo = o.advance_particles((), compat)
+        d.advance_particles()
+        rd, ld = d.drain()
+        if step == 0:
+            assert len(lo) > 0.3 * n
+        assert_rows(ro, rd, lo, ld)
+        assert_particles(o, d)
+    d.close()
+
+
+def test_full_size_gun_steps(ctx):
+    """The benchmark workload at full size (1051 x 421 lines, 1M particles, TM+TE, space
+    charge, a screen): three whole steps including one Poisson correction, bit for bit."""
+    from spiffe_amd import workloads as W
+    g = W.gun_grid()
+    f = W.gun_flags(g)
+    n = 1_000_000
+    o, d = pair(ctx, g, f, cap=n, screens=W.GUN_SCREENS, compat=False)
+    load_fields(o, d, W.gun_fields(g))
+    p = W.gun_bunch(g, n)
+    o.append(p)
+    d.append(p)
+    pc = dict(W.GUN_POISSON, maximum_iterations=400)
+    for obj in (o, d):
+        obj.setup_poisson_correction(**pc)
+    for step in range(3):
+        ro, lo = o.step(screens=W.GUN_SCREENS, compat=False)
+        d.step()
+        rd, ld = d.drain()
+        assert_rows(ro, rd, lo, ld)
+    assert_fields(o, d, TM_NAMES + TE_NAMES + J_NAMES + ["Psi", "Q"])
+    assert_particles(o, d)
+    d.close()
+
+
 def test_full_step_pic_gun_100_steps(ctx):
     """Whole step (reset, deposit, line current, FDTD, push) on the gun geometry, TM+TE."""
     g = gun_grid(modes=TMTE | abi.SPACE_CHARGE)
