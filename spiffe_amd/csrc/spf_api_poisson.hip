@@ -39,25 +39,37 @@ int ensure_omega(spf_ctx *c, Run *R, int64_t n) {
   return SPF_OK;
 }
 
+// Makes `out` the solve's result buffer (*Psi) and the other ring buffers
+// the run's spares.
+void adopt_result(Run *R, double **Psi, double *const *ring, int nring, int out) {
+  double *res = ring[out];
+  double *spare[4];
+  int si = 0;
+  for (int i = 0; i < nring; i++)
+    if (i != out) spare[si++] = ring[i];
+  for (int i = 0; i < 4; i++) R->PsiSpare[i] = spare[i];
+  *Psi = res;
+}
+
 // solve_poisson_cyl, src/poisson.c:17-130, on *Psi (a run-owned buffer that
 // may be swapped with one of the run's spare buffers); *iters = n on
 // convergence, -1 otherwise.  Batches rotate through a ring of kRing
-// buffers: batch b reads ring[b % kRing] and writes ring[(b+1) % kRing].
-// The stopping rule is applied every kCheck batches by one check launch over
-// their half-sweeps; queued batches return at once after a stop, and the
-// batch that contains the stopping half-sweep still has its input in the ring
-// (kRing > kCheck), so it is re-run for exactly the half-sweeps needed.
+// buffers: batch b reads ring[b % kRing] and writes ring[(b+1) % kRing].  The stopping rule is applied every kCheck batches by
+// one check launch over their half-sweeps; queued batches return at once
+// after a stop, and the batch that contains the stopping half-sweep still has
+// its input in the ring (kRing > kCheck), so it is re-run for exactly the
+// half-sweeps needed.
 int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accuracy, int64_t limit, int64_t *iters) {
   if (accuracy <= 0) return fail(c, SPF_EINVAL, "accuracy is <= 0 in solve_poisson_cyl");
   if (limit <= 1) return fail(c, SPF_EINVAL, "iteration_limit is <= 1 in solve_poisson_cyl");
   int rc = ensure_omega(c, R, limit);
   if (rc) return rc;
   hipStream_t s = c->stream;
-  const int K = sor_batch();
-  double *ring[kRing] = {*Psi, R->PsiSpare[0], R->PsiSpare[1], R->PsiSpare[2], R->PsiSpare[3]};
   HIPCALL(c, launch_sor_reset(*R, s));
   if (rhs)
     HIPCALL(c, launch_sum_abs(rhs, R->ldz, R->d.nz, R->d.nr - 1, R->red, R->red_n, R->sor_sums, s));
+  const int K = sor_batch();
+  double *ring[kRing] = {*Psi, R->PsiSpare[0], R->PsiSpare[1], R->PsiSpare[2], R->PsiSpare[3]};
   int64_t nb = (limit + K - 1) / K, b = 0, converged = -1;
   while (b < nb) {
     int64_t win_end = std::min<int64_t>(nb, b + kSorCheckBatches);
@@ -93,13 +105,7 @@ int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accura
     out = (int)(nb % kRing);
     *iters = -1;
   }
-  if (out != 0) {
-    double *res = ring[out];
-    int si = 0;
-    for (int i = 0; i < kRing; i++)
-      if (i != out) R->PsiSpare[si++] = ring[i];
-    *Psi = res;   // the four buffers other than the result become the spares again
-  }
+  adopt_result(R, Psi, ring, kRing, out);
   return SPF_OK;
 }
 

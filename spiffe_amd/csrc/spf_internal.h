@@ -192,6 +192,7 @@ hipError_t launch_sum_plain(const double *a, int ldz, int iz_n, int ir_n, double
 hipError_t launch_sor_reset(Run &R, hipStream_t s);
 int64_t sor_tiles(const Run &R);
 int sor_batch();
+
 hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const double *rhs, int64_t n0, int k,
                             int partial_row, int force, hipStream_t s);
 hipError_t launch_sor_check(Run &R, int64_t n0, int k, double accuracy, hipStream_t s);

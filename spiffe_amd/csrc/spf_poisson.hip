@@ -21,32 +21,41 @@ namespace spf {
 namespace {
 
 // ---- temporally blocked red-black SOR ----
-// A batch launch performs up to SK consecutive half-sweeps.  Each workgroup
-// owns an OZ x OR tile of points and holds it with an SK-point halo (clipped
-// at the array edges) in registers: lane l of wavefront w keeps columns
-// zlo+2l and zlo+2l+1 of tile rows w*RS .. w*RS+RS-1, with their right-hand
-// side and update mask.  A half-sweep updates colour n%2, which in every row
-// is one of the lane's two columns; the other-colour neighbours come from the
-// lane itself, its neighbour lanes (DPP shifts) and, for the first and last
-// rows, the adjacent wavefronts through LDS.  At local step j only points at
-// least j+1 away from a halo edge that is not an array edge are updated, so
-// after the batch every owned point holds exactly the value the
-// one-colour-per-sweep order produces.  Psi is read from one buffer and the
-// owned points written to the other.  Each workgroup also sums |residual|
-// over its owned points for every step (fixed order) into partials[j][tile];
-// sor_check_kernel then adds the tiles in order and applies the stopping rule
-// of src/poisson.c:125-126 for each half-sweep in turn.
-constexpr int SK = 8;                   // half-sweeps per batch = halo width
-constexpr int TZ = 128;                 // tile columns: 64 lanes x 2
-constexpr int OZ = TZ - 2 * SK;         // owned columns per tile: 112
-constexpr int RS = 6;                   // rows per lane
-constexpr int SW = 6;                   // wavefronts per tile
-constexpr int TRW = RS * SW;            // tile rows: 36
-constexpr int OR = TRW - 2 * SK;        // owned rows per tile: 20
-constexpr int SB = SW * 64;             // SOR batch workgroup
-constexpr int CB = 1024;                // check kernel: 32 lanes per half-sweep, 32 half-sweeps
-static_assert(OZ % 2 == 0 && OR % 2 == 0 && SK % 2 == 0 && RS % 2 == 0,
-              "tile origins and lane rows must be even so each row's colour column is fixed");
+// A batch performs up to SK consecutive half-sweeps.  Each workgroup owns an
+// OZ x OR tile of points and holds it with an SK-point halo (clipped at the
+// array edges) in registers: lane l of wavefront w keeps columns zlo+2l and
+// zlo+2l+1 of tile rows w*RS .. w*RS+RS-1, with their right-hand side and
+// update mask.  A half-sweep updates colour n%2, which in every row is one of
+// the lane's two columns; the other-colour neighbours come from the lane
+// itself, its neighbour lanes (DPP shifts) and, for the first and last rows,
+// the adjacent wavefronts through LDS.  At local step j only points at least
+// j+1 away from a halo edge that is not an array edge are updated, so after
+// the batch every owned point holds exactly the value the
+// one-colour-per-sweep order produces.  Each workgroup also sums |residual|
+// over its owned points for every step in a fixed order.
+// sor_batch_kernel runs one batch per launch, reading Psi from one buffer
+// and writing the owned points to another; sor_check_kernel applies the
+// stopping rule.
+template <int SK_, int RS_, int SW_>
+struct SorCfg {
+  static constexpr int SK = SK_;            // half-sweeps per batch = halo width
+  static constexpr int RS = RS_;            // rows per lane
+  static constexpr int SW = SW_;            // wavefronts per tile
+  static constexpr int TZ = 128;            // tile columns: 64 lanes x 2
+  static constexpr int OZ = TZ - 2 * SK;    // owned columns per tile
+  static constexpr int TRW = RS * SW;       // tile rows
+  static constexpr int OR = TRW - 2 * SK;   // owned rows per tile
+  static constexpr int NT = SW * 64;        // workgroup size
+  static_assert(OZ % 2 == 0 && OR % 2 == 0 && SK % 2 == 0 && RS % 2 == 0,
+                "tile origins and lane rows must be even so each row's colour column is fixed");
+  static_assert(OR >= SK && OZ >= SK, "a halo must come from the adjacent tiles only");
+  static_assert(2 * RS <= 32, "one mask bit per point");
+  // mask bits of the lane's even (column 0) and odd (column 1) points
+  static constexpr unsigned COL0 = 0x55555555u & ((1u << (2 * RS)) - 1u);
+  static constexpr unsigned COL1 = 0xAAAAAAAAu & ((1u << (2 * RS)) - 1u);
+};
+using BatchCfg = SorCfg<8, 6, 6>;     // 112 x 20 owned points, 6 wavefronts
+constexpr int CB = 1024;              // check kernel: 32 lanes per half-sweep, 32 half-sweeps
 
 // fixed-order sum over a workgroup: butterfly within each wavefront, then the
 // wavefront sums in order (the same operations every launch)
@@ -61,17 +70,17 @@ __device__ double block_sum_det(double v, double *wsum) {
   return t;
 }
 
-// lane l receives lane l-1's value (wave_shr:1); lane 0 receives 0
-__device__ inline double from_left(double v) {
-  int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xF, 0xF, false);
-  int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xF, 0xF, false);
+// lane l receives lane l-1's value (wave_shr:1); lane 0 keeps `edge`
+__device__ inline double from_left(double v, double edge) {
+  int lo = __builtin_amdgcn_update_dpp(__double2loint(edge), __double2loint(v), 0x138, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_update_dpp(__double2hiint(edge), __double2hiint(v), 0x138, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
 
-// lane l receives lane l+1's value (wave_shl:1); lane 63 receives 0
-__device__ inline double from_right(double v) {
-  int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xF, 0xF, false);
-  int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xF, 0xF, false);
+// lane l receives lane l+1's value (wave_shl:1); lane 63 keeps `edge`
+__device__ inline double from_right(double v, double edge) {
+  int lo = __builtin_amdgcn_update_dpp(__double2loint(edge), __double2loint(v), 0x130, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_update_dpp(__double2hiint(edge), __double2hiint(v), 0x130, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
 
@@ -93,7 +102,7 @@ __device__ inline bool div_fast_ok(double x) {
   return ax > 1e-280 && ax < 1e280;
 }
 
-struct SorArgs {
+struct SorCommon {
   int nz, nr, ldz, left_dir, right_dir;
   double a, e;
   double y_e, y_axis;    // RN(1/e), RN(1/(e - 2*c[0]))
@@ -101,76 +110,147 @@ struct SorArgs {
   const double *c, *d;   // per-row stencil coefficients
   const double *omega;   // per-half-sweep omega
   const double *rhs;     // may be null
-  const double *Pin;
-  double *Pout;
   const int16_t *flags;
-  double *partials;      // [SK][ntiles] of this batch
-  const int64_t *state;  // [0] done
-  int64_t n0;
-  int k;                 // half-sweeps in this batch
-  int force;             // run even if done is set (the exact re-run after convergence)
   int tiles_z, ntiles;
 };
 
-// Per-lane constants of one tile.  Point (i, col) of a lane is bit 2*i+col
-// of the masks.  A point is updated at local step j when its update-mask bit
-// is set and it lies at least j+1 points from every shrinking halo edge:
-// zdepth[col] > j and rdepth[i] > j.
-struct SorLane {
-  unsigned upd;     // update mask: in the array interior, not metal (src/poisson.c:63-70)
-  unsigned own;     // point belongs to the tile's owned region
-  int zdepth[2];    // distance of each column from the nearest shrinking z edge
-  int rdepth[RS];   // distance of each row from the nearest shrinking r edge (wave-uniform)
-  bool l_mirror;    // column 0 is iz == 0: L takes R's value
-  bool r_mirror[2]; // column is iz == nz-1: R takes L's value
+template <class G>
+struct SorTile {
+  int z0, r0, zo1, ro1, zlo, rlo, LZ, LR;
+  bool sz_lo, sz_hi, sr_lo, sr_hi, redge;
+  __device__ SorTile(const SorCommon &a, int tile) {
+    z0 = (tile % a.tiles_z) * G::OZ;
+    r0 = (tile / a.tiles_z) * G::OR;
+    zo1 = min(z0 + G::OZ, a.nz + 1);
+    ro1 = min(r0 + G::OR, a.nr + 1);
+    zlo = max(z0 - G::SK, 0);
+    rlo = max(r0 - G::SK, 0);
+    const int zhi = min(z0 + G::OZ + G::SK, a.nz + 1), rhi = min(r0 + G::OR + G::SK, a.nr + 1);
+    LZ = zhi - zlo;
+    LR = rhi - rlo;
+    // the halo shrinks only at edges that are not array edges
+    sz_lo = zlo > 0;
+    sz_hi = zhi < a.nz + 1;
+    sr_lo = rlo > 0;
+    sr_hi = rhi < a.nr + 1;
+    redge = zhi >= a.nz;   // the tile holds iz = nz-1
+  }
 };
 
-// One half-sweep of colour C on the registers of this lane.  Row i of the
-// lane is tile row w*RS + i, whose parity is the parity of i; its colour-C
-// point is column (i + C) & 1 of the lane's pair.
-template <int C>
-__device__ inline double sor_half_sweep(const SorArgs &a, const SorLane &ln, double (&P)[RS][2],
-                                        const double (&D)[RS][2], const double2 up_row, const double2 dn_row,
-                                        const double (&cr)[RS], const double (&dr)[RS], int j, double omega,
-                                        bool axis_wave) {
-  const double A = a.a, E = a.e;
-  unsigned okb = ln.upd & ((ln.zdepth[0] > j ? 0x555u : 0u) | (ln.zdepth[1] > j ? 0xAAAu : 0u));
+// Per-lane state.  Point (i, col) of a lane is bit 2*i+col of the masks.  A
+// point is updated at local step j when its update-mask bit is set and it
+// lies at least j+1 points from every shrinking halo edge.
+template <class G>
+struct SorLane {
+  double P[G::RS][2], D[G::RS][2];
+  unsigned upd;      // in the array interior and not metal (src/poisson.c:58-120)
+  unsigned own;      // in the tile's owned region
+  int zdepth[2];     // distance of each column from the nearest shrinking z edge
+  bool r_mirror[2];  // column is iz == nz-1: its right neighbour mirrors the left one
+};
+
+// this lane's points from src; every address is clamped into the array so
+// all loads issue together (points outside the tile are never updated)
+template <class G>
+__device__ inline void sor_load_P(const SorCommon &a, const SorTile<G> &t, int w, int lane, const double *src,
+                                  double (&P)[G::RS][2]) {
+  const int iz_e = t.zlo + 2 * lane;
 #pragma unroll
-  for (int i = 0; i < RS; i++)
-    if (ln.rdepth[i] <= j) okb &= ~(3u << (2 * i));
-  double q[RS], x[RS], res[RS];
+  for (int i = 0; i < G::RS; i++) {
+    const int irc = min(t.rlo + w * G::RS + i, a.nr);
+#pragma unroll
+    for (int col = 0; col < 2; col++) P[i][col] = src[fix(a.ldz, min(iz_e + col, a.nz), irc)];
+  }
+}
+
+// right-hand side, masks and row coefficients (cr/dr in LDS, TRW each)
+template <class G>
+__device__ inline void sor_setup(const SorCommon &a, const SorTile<G> &t, int w, int lane, SorLane<G> &ln,
+                                 double *cr, double *dr) {
+  const int iz_e = t.zlo + 2 * lane;
+  const double *rhs = a.rhs ? a.rhs : a.omega;   // any valid address when there is no rhs
+  int16_t fl[G::RS][2];
+#pragma unroll
+  for (int i = 0; i < G::RS; i++) {
+    const int irc = min(t.rlo + w * G::RS + i, a.nr);
+#pragma unroll
+    for (int col = 0; col < 2; col++) {
+      const int64_t p = fix(a.ldz, min(iz_e + col, a.nz), irc);
+      ln.D[i][col] = a.rhs ? rhs[p] : 0.0;
+      fl[i][col] = a.flags[p];
+    }
+  }
+  for (int k = threadIdx.x; k < G::TRW; k += G::NT) {
+    const int irc = min(t.rlo + k, a.nr);
+    cr[k] = a.c[irc];
+    dr[k] = a.d[irc];
+  }
+  ln.upd = 0;
+  ln.own = 0;
+#pragma unroll
+  for (int col = 0; col < 2; col++) {
+    const int lz = 2 * lane + col;
+    ln.zdepth[col] = min(t.sz_lo ? lz : G::SK + 1, t.sz_hi ? t.LZ - 1 - lz : G::SK + 1);
+    ln.r_mirror[col] = iz_e + col == a.nz - 1;
+  }
+#pragma unroll
+  for (int i = 0; i < G::RS; i++) {
+    const int lr = w * G::RS + i, ir = t.rlo + lr;
+#pragma unroll
+    for (int col = 0; col < 2; col++) {
+      const int iz = iz_e + col;
+      const bool in = lr < t.LR && 2 * lane + col < t.LZ;
+      const bool in_range =
+          (iz >= 1 && iz <= a.nz - 2) || (iz == 0 && a.left_dir) || (iz == a.nz - 1 && a.right_dir);
+      const bool upd = in && ir <= a.nr - 2 && iz <= a.nz - 1 && in_range &&
+                       !(fl[i][col] & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE));
+      if (upd) ln.upd |= 1u << (2 * i + col);
+      if (iz >= t.z0 && iz < t.zo1 && ir >= t.r0 && ir < t.ro1) ln.own |= 1u << (2 * i + col);
+      if (!upd) ln.D[i][col] = 0.0;
+    }
+  }
+}
+
+// One half-sweep of colour C on the registers of this lane; AX: this
+// wavefront holds the axis row (tile row 0 at ir = 0).  Row i of the lane is
+// tile row w*RS + i, whose parity is the parity of i (rlo is even); its
+// colour-C point is column (i + C) & 1 of the lane's pair.  Returns the sum of
+// |residual| over the lane's updated owned points.
+template <class G, int C, bool AX>
+__device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsigned okb, const double2 up_row,
+                                        const double2 dn_row, const double *cr, const double *dr, int w,
+                                        double omega, bool redge) {
+  const double A = a.a, E = a.e;
+  double q[G::RS], x[G::RS], res[G::RS];
   bool slow = false;
 #pragma unroll
-  for (int i = 0; i < RS; i++) {
+  for (int i = 0; i < G::RS; i++) {
     const int col = (i + C) & 1;
+    const int lr = w * G::RS + i;
     double L, R;
     if (col == 0) {
-      R = P[i][1];
-      L = from_left(R);
-      L = ln.l_mirror ? R : L;
-      R = ln.r_mirror[0] ? L : R;
+      R = ln.P[i][1];
+      L = from_left(R, R);   // lane 0 holds iz == 0 when the tile starts at the array edge: mirror
+      if (redge) R = ln.r_mirror[0] ? L : R;
     } else {
-      L = P[i][0];
-      R = from_right(L);
-      R = ln.r_mirror[1] ? L : R;
+      L = ln.P[i][0];
+      R = from_right(L, L);  // lane 63 at iz == nz-1 mirrors; elsewhere it is never updated
+      if (redge) R = ln.r_mirror[1] ? L : R;
     }
     const double zs = R + L;
-    const double up = i < RS - 1 ? P[i + 1 < RS ? i + 1 : i][col] : (col ? up_row.y : up_row.x);
-    const double down = i > 0 ? P[i > 0 ? i - 1 : 0][col] : (col ? dn_row.y : dn_row.x);
-    const double c = cr[i];
+    const double up = i < G::RS - 1 ? ln.P[i + 1 < G::RS ? i + 1 : i][col] : (col ? up_row.y : up_row.x);
+    const double c = cr[lr];
     double residual, diag, y;
-    if (i == 0) {
-      // the axis row (src/poisson.c:84-90) has no "below" term, added as
-      // -0.0 (an exact no-op), a 4*c "above" weight and diagonal e - 2*c
-      const double cu = axis_wave ? 4 * c : c;
-      const double dn = axis_wave ? -0.0 : dr[i] * down;
-      diag = axis_wave ? E - 2 * c : E;
-      y = axis_wave ? a.y_axis : a.y_e;
-      residual = A * zs + cu * up + dn + diag * P[i][col] - D[i][col];
+    if (AX && i == 0) {
+      // axis row, src/poisson.c:84-90: no "below" term, a 4*c "above" weight, diagonal e - 2*c
+      diag = E - 2 * c;
+      y = a.y_axis;
+      residual = A * zs + 4 * c * up + diag * ln.P[i][col] - ln.D[i][col];
     } else {
+      const double down = i > 0 ? ln.P[i > 0 ? i - 1 : 0][col] : (col ? dn_row.y : dn_row.x);
       diag = E;
       y = a.y_e;
-      residual = A * zs + c * up + dr[i] * down + E * P[i][col] - D[i][col];
+      residual = A * zs + c * up + dr[lr] * down + E * ln.P[i][col] - ln.D[i][col];
     }
     res[i] = residual;
     x[i] = omega * residual;
@@ -179,124 +259,118 @@ __device__ inline double sor_half_sweep(const SorArgs &a, const SorLane &ln, dou
   }
   if (slow || !a.use_y) {
 #pragma unroll
-    for (int i = 0; i < RS; i++) {
-      const double diag = (i == 0 && axis_wave) ? E - 2 * cr[0] : E;
+    for (int i = 0; i < G::RS; i++) {
+      const double diag = (AX && i == 0) ? E - 2 * cr[w * G::RS] : E;
       if (!a.use_y || !div_fast_ok(x[i])) q[i] = x[i] / diag;
     }
   }
   double acc = 0;
 #pragma unroll
-  for (int i = 0; i < RS; i++) {
+  for (int i = 0; i < G::RS; i++) {
     const int col = (i + C) & 1;
     const bool ok = (okb >> (2 * i + col)) & 1;
-    const double nv = P[i][col] - q[i];
-    P[i][col] = ok ? nv : P[i][col];
+    const double nv = ln.P[i][col] - q[i];
+    ln.P[i][col] = ok ? nv : ln.P[i][col];
     acc += (ok && ((ln.own >> (2 * i + col)) & 1)) ? fabs(res[i]) : 0.0;
   }
   return acc;
 }
 
-__global__ __launch_bounds__(SB) void sor_batch_kernel(SorArgs a) {
-  __shared__ double2 xrow[2][SW][2][64];   // [parity][wave][first,last row][lane]
-  __shared__ double wsum[SK][SW];
-  const int64_t done = a.force ? 0 : a.state[0];
-  const int tile = blockIdx.x;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int z0 = (tile % a.tiles_z) * OZ, r0 = (tile / a.tiles_z) * OR;
-  const int zo1 = min(z0 + OZ, a.nz + 1), ro1 = min(r0 + OR, a.nr + 1);
-  const int zlo = max(z0 - SK, 0), rlo = max(r0 - SK, 0);
-  const int zhi = min(z0 + OZ + SK, a.nz + 1), rhi = min(r0 + OR + SK, a.nr + 1);
-  const int LZ = zhi - zlo, LR = rhi - rlo;
-  // load this lane's points, right-hand side, flags and row coefficients;
-  // every address is clamped into the array so all loads issue together
-  double P[RS][2], D[RS][2], cr[RS], dr[RS];
-  int16_t fl[RS][2];
-  const int iz_e = zlo + 2 * lane;
-  const double *rhs = a.rhs ? a.rhs : a.Pin;
+// k <= SK half-sweeps n0 .. n0+k-1; ares[j] gets the lane's owned residual sum of step j
+template <class G>
+__device__ inline void sor_sweeps(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane,
+                                  int64_t n0, int k, double2 (*xrow)[G::SW][2][64], const double *cr,
+                                  const double *dr, double (&ares)[G::SK]) {
+  const bool axis_wave = t.rlo == 0 && w == 0;
 #pragma unroll
-  for (int i = 0; i < RS; i++) {
-    const int irc = min(rlo + w * RS + i, a.nr);
-    cr[i] = a.c[irc];
-    dr[i] = a.d[irc];
-#pragma unroll
-    for (int col = 0; col < 2; col++) {
-      const int64_t p = fix(a.ldz, min(iz_e + col, a.nz), irc);
-      P[i][col] = a.Pin[p];
-      D[i][col] = rhs[p];
-      fl[i][col] = a.flags[p];
-    }
-  }
-  if (done) return;
-  SorLane ln;
-  ln.upd = 0;
-  ln.own = 0;
-  const bool sz_lo = zlo > 0, sz_hi = zhi < a.nz + 1, sr_lo = rlo > 0, sr_hi = rhi < a.nr + 1;
-#pragma unroll
-  for (int col = 0; col < 2; col++) {
-    const int lz = 2 * lane + col;
-    ln.zdepth[col] = min(sz_lo ? lz : SK + 1, sz_hi ? LZ - 1 - lz : SK + 1);
-    ln.r_mirror[col] = iz_e + col == a.nz - 1;
-  }
-  ln.l_mirror = iz_e == 0;
-#pragma unroll
-  for (int i = 0; i < RS; i++) {
-    const int lr = w * RS + i, ir = rlo + lr;
-    ln.rdepth[i] = min(sr_lo ? lr : SK + 1, sr_hi ? LR - 1 - lr : SK + 1);
-#pragma unroll
-    for (int col = 0; col < 2; col++) {
-      const int iz = iz_e + col;
-      const bool in = lr < LR && 2 * lane + col < LZ;
-      const bool in_range =
-          (iz >= 1 && iz <= a.nz - 2) || (iz == 0 && a.left_dir) || (iz == a.nz - 1 && a.right_dir);
-      const bool upd = in && ir <= a.nr - 2 && iz <= a.nz - 1 && in_range &&
-                       !(fl[i][col] & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE));
-      if (upd) ln.upd |= 1u << (2 * i + col);
-      if (iz >= z0 && iz < zo1 && ir >= r0 && ir < ro1) ln.own |= 1u << (2 * i + col);
-      D[i][col] = (upd && a.rhs) ? D[i][col] : 0.0;
-    }
-  }
-  const bool axis_wave = rlo == 0 && w == 0;
-  double ares[SK];
-#pragma unroll
-  for (int j = 0; j < SK; j++) ares[j] = 0;
-  for (int j = 0; j < a.k; j++) {
+  for (int j = 0; j < G::SK; j++) ares[j] = 0;
+  for (int j = 0; j < k; j++) {
     const int par = j & 1;
     // publish the first and last rows, then read the neighbouring wavefronts' rows
-    xrow[par][w][0][lane] = make_double2(P[0][0], P[0][1]);
-    xrow[par][w][1][lane] = make_double2(P[RS - 1][0], P[RS - 1][1]);
+    xrow[par][w][0][lane] = make_double2(ln.P[0][0], ln.P[0][1]);
+    xrow[par][w][1][lane] = make_double2(ln.P[G::RS - 1][0], ln.P[G::RS - 1][1]);
     __syncthreads();
-    const double2 up_row = w + 1 < SW ? xrow[par][w + 1][0][lane] : make_double2(0.0, 0.0);
+    const double2 up_row = w + 1 < G::SW ? xrow[par][w + 1][0][lane] : make_double2(0.0, 0.0);
     const double2 dn_row = w > 0 ? xrow[par][w - 1][1][lane] : make_double2(0.0, 0.0);
-    const double omega = a.omega[a.n0 + j];
+    const double omega = a.omega[n0 + j];
+    // points valid at this step: column depth and the wavefront's share of the valid rows
+    const int vr0 = t.sr_lo ? j + 1 : 0, vr1 = t.LR - (t.sr_hi ? j + 1 : 0);
+    const int lo = min(max(vr0 - w * G::RS, 0), G::RS), hi = min(max(vr1 - w * G::RS, 0), G::RS);
+    const unsigned rmask = hi > lo ? (((1u << (2 * hi)) - 1u) & ~((1u << (2 * lo)) - 1u)) : 0u;
+    const unsigned okb =
+        ln.upd & rmask & ((ln.zdepth[0] > j ? G::COL0 : 0u) | (ln.zdepth[1] > j ? G::COL1 : 0u));
     double acc;
-    if (((a.n0 + j) & 1) == 0)
-      acc = sor_half_sweep<0>(a, ln, P, D, up_row, dn_row, cr, dr, j, omega, axis_wave);
+    const bool odd = ((n0 + j) & 1) != 0;
+    if (axis_wave)
+      acc = odd ? sor_half_sweep<G, 1, true>(a, ln, okb, up_row, dn_row, cr, dr, w, omega, t.redge)
+                : sor_half_sweep<G, 0, true>(a, ln, okb, up_row, dn_row, cr, dr, w, omega, t.redge);
     else
-      acc = sor_half_sweep<1>(a, ln, P, D, up_row, dn_row, cr, dr, j, omega, axis_wave);
+      acc = odd ? sor_half_sweep<G, 1, false>(a, ln, okb, up_row, dn_row, cr, dr, w, omega, t.redge)
+                : sor_half_sweep<G, 0, false>(a, ln, okb, up_row, dn_row, cr, dr, w, omega, t.redge);
 #pragma unroll
-    for (int jj = 0; jj < SK; jj++)
+    for (int jj = 0; jj < G::SK; jj++)
       if (jj == j) ares[jj] = acc;
   }
-  // fixed-order sums of the owned residuals, one per half-sweep: butterflies
-  // within each wavefront, then the wavefronts in order
+}
+
+// the tile's owned residual sum of each step j < k into out[j * stride]:
+// butterflies within each wavefront, then the wavefronts in order
+template <class G>
+__device__ inline void sor_block_sums(const double (&ares)[G::SK], int k, int w, int lane, double (*wsum)[G::SW],
+                                      double *out, int64_t stride) {
 #pragma unroll
-  for (int j = 0; j < SK; j++) {
+  for (int j = 0; j < G::SK; j++) {
     double v = ares[j];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if (lane == 0) wsum[j][w] = v;
   }
   __syncthreads();
-  if (threadIdx.x < a.k) {
+  if (threadIdx.x < k) {
     double tot = 0;
-    for (int k = 0; k < SW; k++) tot += wsum[threadIdx.x][k];
-    a.partials[(int64_t)threadIdx.x * a.ntiles + tile] = tot;
+    for (int q = 0; q < G::SW; q++) tot += wsum[threadIdx.x][q];
+    out[threadIdx.x * stride] = tot;
   }
+}
+
+template <class G>
+__device__ inline void sor_write_owned(const SorCommon &a, const SorTile<G> &t, const SorLane<G> &ln, int w,
+                                       int lane, double *dst) {
+  const int iz_e = t.zlo + 2 * lane;
 #pragma unroll
-  for (int i = 0; i < RS; i++) {
+  for (int i = 0; i < G::RS; i++) {
 #pragma unroll
     for (int col = 0; col < 2; col++)
-      if ((ln.own >> (2 * i + col)) & 1) a.Pout[fix(a.ldz, iz_e + col, rlo + w * RS + i)] = P[i][col];
+      if ((ln.own >> (2 * i + col)) & 1) dst[fix(a.ldz, iz_e + col, t.rlo + w * G::RS + i)] = ln.P[i][col];
   }
+}
+
+struct SorArgs {
+  SorCommon m;
+  const double *Pin;
+  double *Pout;
+  double *partials;      // [SK][ntiles] of this batch
+  const int64_t *state;  // [0] done
+  int64_t n0;
+  int k;                 // half-sweeps in this batch
+  int force;             // run even if done is set (the exact re-run after convergence)
+};
+
+__global__ __launch_bounds__(BatchCfg::NT) void sor_batch_kernel(SorArgs a) {
+  using G = BatchCfg;
+  __shared__ double2 xrow[2][G::SW][2][64];   // [parity][wave][first,last row][lane]
+  __shared__ double wsum[G::SK][G::SW];
+  __shared__ double cr[G::TRW], dr[G::TRW];
+  const int64_t done = a.force ? 0 : a.state[0];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const SorTile<G> t(a.m, blockIdx.x);
+  SorLane<G> ln;
+  sor_load_P<G>(a.m, t, w, lane, a.Pin, ln.P);
+  sor_setup<G>(a.m, t, w, lane, ln, cr, dr);
+  if (done) return;
+  double ares[G::SK];
+  sor_sweeps<G>(a.m, t, ln, w, lane, a.n0, a.k, xrow, cr, dr, ares);
+  sor_block_sums<G>(ares, a.k, w, lane, wsum, a.partials + blockIdx.x, a.m.ntiles);
+  sor_write_owned<G>(a.m, t, ln, w, lane, a.Pout);
 }
 
 struct CheckArgs {
@@ -488,45 +562,58 @@ hipError_t launch_sor_reset(Run &R, hipStream_t s) {
   return hipGetLastError();
 }
 
-int64_t sor_tiles(const Run &R) {
-  int64_t tz = (R.d.nz + 1 + OZ - 1) / OZ, tr = (R.d.nr + 1 + OR - 1) / OR;
-  return tz * tr;
-}
+namespace {
 
-int sor_batch() { return SK; }
-
-hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const double *rhs, int64_t n0, int k,
-                            int partial_row, int force, hipStream_t s) {
-  SorArgs a;
+SorCommon sor_common(Run &R, const double *rhs, int oz, int orr) {
+  SorCommon a;
   a.nz = R.d.nz;
   a.nr = R.d.nr;
   a.ldz = R.ldz;
   a.left_dir = R.d.left_bc == SPF_DIRICHLET;
   a.right_dir = R.d.right_bc == SPF_DIRICHLET;
-  double dz = R.d.dz, dr = R.d.dr;
+  const double dz = R.d.dz, dr = R.d.dr;
   a.a = 1 / (dz * dz);
   a.e = -2 * (1 / (dz * dz) + 1 / (dr * dr));
   // reciprocals for div_fast, computed with the device's operands
   const double diag0 = a.e - 2 * R.sor_c_host0;
   a.y_e = 1 / a.e;
   a.y_axis = 1 / diag0;
-  auto safe = [](double v) { double m = std::fabs(v); return m > 1e-200 && m < 1e200; };
+  auto safe = [](double v) {
+    double m = std::fabs(v);
+    return m > 1e-200 && m < 1e200;
+  };
   a.use_y = safe(a.e) && safe(diag0) && safe(a.y_e) && safe(a.y_axis);
   a.c = R.sor_c;
   a.d = R.sor_d;
   a.omega = R.omega;
   a.rhs = rhs;
+  a.flags = R.flags;
+  a.tiles_z = (R.d.nz + 1 + oz - 1) / oz;
+  a.ntiles = a.tiles_z * ((R.d.nr + 1 + orr - 1) / orr);
+  return a;
+}
+
+}  // namespace
+
+int64_t sor_tiles(const Run &R) {
+  int64_t tz = (R.d.nz + 1 + BatchCfg::OZ - 1) / BatchCfg::OZ, tr = (R.d.nr + 1 + BatchCfg::OR - 1) / BatchCfg::OR;
+  return tz * tr;
+}
+
+int sor_batch() { return BatchCfg::SK; }
+
+hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const double *rhs, int64_t n0, int k,
+                            int partial_row, int force, hipStream_t s) {
+  SorArgs a;
+  a.m = sor_common(R, rhs, BatchCfg::OZ, BatchCfg::OR);
   a.Pin = Pin;
   a.Pout = Pout;
-  a.flags = R.flags;
-  a.partials = R.red + (int64_t)partial_row * sor_tiles(R);
+  a.partials = R.red + (int64_t)partial_row * a.m.ntiles;
   a.state = R.sor_state;
   a.n0 = n0;
   a.k = k;
   a.force = force;
-  a.tiles_z = (R.d.nz + 1 + OZ - 1) / OZ;
-  a.ntiles = (int)sor_tiles(R);
-  hipLaunchKernelGGL(sor_batch_kernel, dim3((unsigned)a.ntiles), dim3(SB), 0, s, a);
+  hipLaunchKernelGGL(sor_batch_kernel, dim3((unsigned)a.m.ntiles), dim3(BatchCfg::NT), 0, s, a);
   return hipGetLastError();
 }
 
