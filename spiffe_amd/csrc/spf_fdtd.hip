@@ -22,6 +22,10 @@ namespace {
 constexpr int16_t EZ_ZERO = SPF_FL_EZ_ZERO, ER_ZERO = SPF_FL_ER_ZERO, BPHI_ZERO = SPF_FL_BPHI_ZERO,
                   EPHI_ZERO = SPF_FL_EPHI_ZERO, BZ_ZERO = SPF_FL_BZ_ZERO, BR_ZERO = SPF_FL_BR_ZERO;
 
+// F evaluates the final value of any point with the reference's loop bodies
+// and boundary patches.  With NEWE the B updates read the new E from the
+// output arrays (already complete) instead of re-evaluating it.
+template <bool NEWE>
 struct F {
   const FdtdArgs &a;
   __device__ int64_t ix(int iz, int ir) const { return fix(a.ldz, iz, ir); }
@@ -33,13 +37,15 @@ struct F {
   }
 
   // ---- TM ----
-  // Ez loop body, src/integrate.c:341-347 (1 <= iz <= nz-1, 0 <= ir <= nr-1)
+  // Ez loop body, src/integrate.c:341-347 (1 <= iz <= nz-1, 0 <= ir <= nr-1).
+  // Every loop body here loads its operands whether or not the point is
+  // active, so the loads of a whole call tree issue together; an inactive
+  // point returns its old value unchanged.
   __device__ double ez_upd(int iz, int ir) const {
     int64_t p = ix(iz, ir);
     double v = a.Ez[p];
-    if (act1(iz, ir, EZ_ZERO))
-      v += a.dti * (-a.Jz[p] / a.eps0 + (a.cp[ir] * a.Bphi[ix(iz, ir + 1)] - a.cm[ir] * a.Bphi[p]));
-    return v;
+    double inc = a.dti * (-a.Jz[p] / a.eps0 + (a.cp[ir] * a.Bphi[ix(iz, ir + 1)] - a.cm[ir] * a.Bphi[p]));
+    return act1(iz, ir, EZ_ZERO) ? v + inc : v;
   }
   // Ez after its loop and the boundary patches of :348-359
   __device__ double ez_final(int iz, int ir) const {
@@ -59,9 +65,8 @@ struct F {
   __device__ double er_upd(int iz, int ir) const {
     int64_t p = ix(iz, ir);
     double v = a.Er[p];
-    if (act0(iz, ir, ER_ZERO))
-      v += a.dti * (-a.Jr[p] / a.eps0 - a.er_factor * (a.Bphi[ix(iz + 1, ir)] - a.Bphi[p]));
-    return v;
+    double inc = a.dti * (-a.Jr[p] / a.eps0 - a.er_factor * (a.Bphi[ix(iz + 1, ir)] - a.Bphi[p]));
+    return act0(iz, ir, ER_ZERO) ? v + inc : v;
   }
   // Er after :373-380
   __device__ double er_final(int iz, int ir) const {
@@ -75,13 +80,11 @@ struct F {
   // Bphi loop body, :385-399 (1 <= iz <= nz-1, 1 <= ir <= nr-1), from the final E
   __device__ void bphi_upd(int iz, int ir, double &b, double &b2) const {
     int64_t p = ix(iz, ir);
-    b = a.Bphi[p];
-    b2 = a.Bphi2[p];
-    if (act1(iz, ir, BPHI_ZERO)) {
-      double dB = a.dti * ((ez_final(iz, ir) - ez_final(iz, ir - 1)) / a.dr - (er_final(iz, ir) - er_final(iz - 1, ir)) / a.dz);
-      b2 = b + dB / 2;
-      b = b + dB;
-    }
+    const double b0 = a.Bphi[p], b20 = a.Bphi2[p];
+    double dB = a.dti * ((ez_at(iz, ir) - ez_at(iz, ir - 1)) / a.dr - (er_at(iz, ir) - er_at(iz - 1, ir)) / a.dz);
+    const bool act = act1(iz, ir, BPHI_ZERO);
+    b2 = act ? b0 + dB / 2 : b20;
+    b = act ? b0 + dB : b0;
   }
   // Bphi after the row patches of :400-408 (1 <= iz <= nz-1, 0 <= ir <= nr)
   __device__ void bphi_inner(int iz, int ir, double &b, double &b2) const {
@@ -119,10 +122,9 @@ struct F {
   __device__ double ephi_upd(int iz, int ir) const {
     int64_t p = ix(iz, ir);
     double v = a.Ephi[p];
-    if (act0(iz, ir, EPHI_ZERO))
-      v += a.dti * (-a.Jphi[p] / a.eps0 +
-                    a.ephi_factor * ((a.Br[ix(iz + 1, ir)] - a.Br[p]) / a.dz - (a.Bz[ix(iz, ir + 1)] - a.Bz[p]) / a.dr));
-    return v;
+    double inc = a.dti * (-a.Jphi[p] / a.eps0 +
+                          a.ephi_factor * ((a.Br[ix(iz + 1, ir)] - a.Br[p]) / a.dz - (a.Bz[ix(iz, ir + 1)] - a.Bz[p]) / a.dr));
+    return act0(iz, ir, EPHI_ZERO) ? v + inc : v;
   }
   // Ephi after :460-471
   __device__ double ephi_final(int iz, int ir) const {
@@ -133,16 +135,19 @@ struct F {
     if (iz == a.nz || ir == a.nr) return a.Ephi[ix(iz, ir)];
     return ephi_upd(iz, ir);
   }
+  // the new E of a point, as the B updates read it
+  __device__ double ez_at(int iz, int ir) const { return NEWE ? a.Ezn[ix(iz, ir)] : ez_final(iz, ir); }
+  __device__ double er_at(int iz, int ir) const { return NEWE ? a.Ern[ix(iz, ir)] : er_final(iz, ir); }
+  __device__ double ephi_at(int iz, int ir) const { return NEWE ? a.Ephin[ix(iz, ir)] : ephi_final(iz, ir); }
+
   // Br loop body, :474-485 (1 <= iz <= nz-1, 0 <= ir <= nr-1)
   __device__ void br_upd(int iz, int ir, double &b, double &b2) const {
     int64_t p = ix(iz, ir);
-    b = a.Br[p];
-    b2 = a.Br2[p];
-    if (act1(iz, ir, BR_ZERO)) {
-      double dB = a.dti * (ephi_final(iz, ir) - ephi_final(iz - 1, ir)) / a.dz;
-      b2 = b + dB / 2;
-      b = b + dB;
-    }
+    const double b0 = a.Br[p], b20 = a.Br2[p];
+    double dB = a.dti * (ephi_at(iz, ir) - ephi_at(iz - 1, ir)) / a.dz;
+    const bool act = act1(iz, ir, BR_ZERO);
+    b2 = act ? b0 + dB / 2 : b20;
+    b = act ? b0 + dB : b0;
   }
   __device__ void br_inner(int iz, int ir, double &b, double &b2) const {
     if (ir == a.nr) {
@@ -168,13 +173,11 @@ struct F {
   // Bz loop body, :512-527 (0 <= iz <= nz-1, 1 <= ir <= nr-1)
   __device__ void bz_upd(int iz, int ir, double &b, double &b2) const {
     int64_t p = ix(iz, ir);
-    b = a.Bz[p];
-    b2 = a.Bz2[p];
-    if (act0(iz, ir, BZ_ZERO)) {
-      double dB = a.bzf[ir] * ((double)(ir + 1) * ephi_final(iz, ir) - (double)ir * ephi_final(iz, ir - 1));
-      b2 = b + dB / 2;
-      b = b + dB;
-    }
+    const double b0 = a.Bz[p], b20 = a.Bz2[p];
+    double dB = a.bzf[ir] * ((double)(ir + 1) * ephi_at(iz, ir) - (double)ir * ephi_at(iz, ir - 1));
+    const bool act = act0(iz, ir, BZ_ZERO);
+    b2 = act ? b0 + dB / 2 : b20;
+    b = act ? b0 + dB : b0;
   }
   // Bz after :528-539
   __device__ void bz_final(int iz, int ir, double &b, double &b2) const {
@@ -214,119 +217,199 @@ __device__ inline int xcd_band_block(int bid, int nblocks) {
   return logical;
 }
 
-// One point per thread.  A block is 9 wavefronts x 64 lanes: wavefront w is
-// row r0-1+w and lane l column z0-1+l, so row r0-1 and column z0-1 are halo
-// and the block outputs a 63 x 8 tile.  Every thread computes its point's
-// final Ez, Er, Ephi into LDS; then each output thread computes its B updates.
-// Points in the box z in [1, nz-2], r in [1, nr-2] use the plain loop bodies
-// (no boundary patch reaches them); all other points go through F, which
-// applies the patches.  The B updates of points in z in [2, nz-2],
-// r in [2, nr-3] read their neighbours' new E from LDS; the others use F too.
-constexpr int QZ = 63, QR = 8, QW = QR + 1;
+// Tiles, two points per thread.  A block is 9 wavefronts x 64 lanes; lane l
+// holds columns zs+2l and zs+2l+1 (zs even: each array is read as one 16-byte
+// load per lane) and wavefront w row r0-1+w.  Row r0-1 and column zs are halo,
+// so the block owns columns zs+1 .. zs+126 (and column 0 in the first tile)
+// of rows r0 .. r0+7.  Every thread computes its points' new Ez, Er, Ephi
+// into LDS and writes those it owns: points in the E box z in [1, nz-2],
+// r in [1, nr-2] with the plain loop bodies (no boundary patch reaches them),
+// the rest through F.  Owned points in the B box z in [2, nz-2],
+// r in [2, nr-3] (whose E neighbours are all in the E box) then compute their
+// B updates from LDS; fdtd_edge_point does the B updates outside it.
+constexpr int QZ = 126, QR = 8, QW = QR + 1;
+
+__device__ inline double2 ld2(const double *p) { return *reinterpret_cast<const double2 *>(p); }
+__device__ inline double el(const double2 v, int c) { return c ? v.y : v.x; }
+
+__device__ inline bool e_box(const FdtdArgs &a, int iz, int ir) {
+  return iz >= 1 && iz <= a.nz - 2 && ir >= 1 && ir <= a.nr - 2;
+}
+__device__ inline bool b_box(const FdtdArgs &a, int iz, int ir) {
+  return iz >= 2 && iz <= a.nz - 2 && ir >= 2 && ir <= a.nr - 3;
+}
+
+// The B updates the interior tiles leave: rows 0, 1, nr-2, nr-1, nr and
+// columns 0, 1, nz-1, nz, with the boundary patches applied.  They run after
+// the tiles, so they read the complete new E.
+struct EdgeList {
+  int rows[5], nrows, cols[4], ncols;
+};
+
+__device__ void fdtd_edge_point(const FdtdArgs &a, const EdgeList &e, int64_t i) {
+  const int64_t nrowp = (int64_t)e.nrows * (a.nz + 1);
+  const int64_t ncolp = (int64_t)e.ncols * (a.nr + 1);
+  if (i >= nrowp + ncolp) return;
+  int iz, ir;
+  if (i < nrowp) {
+    ir = e.rows[i / (a.nz + 1)];
+    iz = (int)(i % (a.nz + 1));
+  } else {
+    const int64_t k = i - nrowp;
+    iz = e.cols[k / (a.nr + 1)];
+    ir = (int)(k % (a.nr + 1));
+    for (int j = 0; j < e.nrows; j++)
+      if (e.rows[j] == ir) return;   // already covered as a row point
+  }
+  F<true> f{a};
+  const int64_t q = fix(a.ldz, iz, ir);
+  if (a.tm) {
+    double b, b2;
+    f.bphi_final(iz, ir, b, b2);
+    a.Bphin[q] = b;
+    a.Bphi2[q] = b2;
+  }
+  if (a.te) {
+    double b, b2, d, d2;
+    f.br_final(iz, ir, b, b2);
+    f.bz_final(iz, ir, d, d2);
+    a.Brn[q] = b;
+    a.Br2[q] = b2;
+    a.Bzn[q] = d;
+    a.Bz2[q] = d2;
+  }
+}
+
+__global__ __launch_bounds__(256) void fdtd_edge_kernel(FdtdArgs a, EdgeList e) {
+  fdtd_edge_point(a, e, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 __global__ __launch_bounds__(64 * QW) void fdtd_kernel(FdtdArgs a, int tiles_z, int nblocks) {
-  __shared__ double sez[QW][64], ser[QW][64], sephi[QW][64];
+  __shared__ double sez[QW][128], ser[QW][128], sephi[QW][128];
   int logical = xcd_band_block(blockIdx.x, nblocks);
   if (logical >= nblocks) return;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int z0 = (logical % tiles_z) * QZ, r0 = (logical / tiles_z) * QR;
-  const int iz = z0 - 1 + lane, ir = r0 - 1 + w;
-  const bool inside = iz >= 0 && iz <= a.nz && ir >= 0 && ir <= a.nr;
+  const int zs = (logical % tiles_z) * QZ, r0 = (logical / tiles_z) * QR;
+  const int ir = r0 - 1 + w, iz0 = zs + 2 * lane;
   const int ldz = a.ldz;
-  const int izc = min(max(iz, 0), a.nz), irc = min(max(ir, 0), a.nr);
-  const int64_t p = fix(ldz, izc, irc);
-  F f{a};
-  const bool efast = iz >= 1 && iz <= a.nz - 2 && ir >= 1 && ir <= a.nr - 2;
-  const int16_t fl = a.flags[p];
-  // neighbours z+1 through the lane above (lane 63 loads its own)
-  double bphi_c = 0, br_c = 0;
-  if (a.tm) bphi_c = a.Bphi[p];
-  if (a.te) br_c = a.Br[p];
-  double bphi_zp = __shfl_down(bphi_c, 1, 64), br_zp = __shfl_down(br_c, 1, 64);
-  if (lane == 63 && efast) {
-    if (a.tm) bphi_zp = a.Bphi[p + 1];
-    if (a.te) br_zp = a.Br[p + 1];
-  }
-  double ez_n = 0, er_n = 0, ephi_n = 0;
-  if (inside) {
-    if (efast) {
-      const int c0 = a.cut0[ir], c1 = a.cut1[ir];
-      if (a.tm) {
-        const double bphi_hi = a.Bphi[p + ldz];
-        ez_n = a.Ez[p];
-        if (!(fl & EZ_ZERO) && iz <= c1)
-          ez_n += a.dti * (-a.Jz[p] / a.eps0 + (a.cp[ir] * bphi_hi - a.cm[ir] * bphi_c));
-        er_n = a.Er[p];
-        if (!(fl & ER_ZERO) && iz <= c0)
-          er_n += a.dti * (-a.Jr[p] / a.eps0 - a.er_factor * (bphi_zp - bphi_c));
-      }
-      if (a.te) {
-        const double bz_c = a.Bz[p], bz_hi = a.Bz[p + ldz];
-        ephi_n = a.Ephi[p];
-        if (!(fl & EPHI_ZERO) && iz <= c0)
-          ephi_n += a.dti * (-a.Jphi[p] / a.eps0 + a.ephi_factor * ((br_zp - br_c) / a.dz - (bz_hi - bz_c) / a.dr));
-      }
-    } else {
-      if (a.tm) {
-        ez_n = f.ez_final(iz, ir);
-        er_n = f.er_final(iz, ir);
-      }
-      if (a.te) ephi_n = f.ephi_final(iz, ir);
-    }
-  }
-  sez[w][lane] = ez_n;
-  ser[w][lane] = er_n;
-  sephi[w][lane] = ephi_n;
-  __syncthreads();
-  if (!inside || w == 0 || lane == 0 || iz >= z0 + QZ) return;
-  const bool bfast = iz >= 2 && iz <= a.nz - 2 && ir >= 2 && ir <= a.nr - 3;
-  if (a.tm) {
-    double b, b2;
-    if (bfast) {
-      b = bphi_c;
-      if (!(fl & BPHI_ZERO) && iz <= a.cut1[ir]) {
-        double dB = a.dti * ((ez_n - sez[w - 1][lane]) / a.dr - (er_n - ser[w][lane - 1]) / a.dz);
-        b2 = b + dB / 2;
-        b = b + dB;
-      } else {
-        b2 = a.Bphi2[p];
-      }
-    } else {
-      f.bphi_final(iz, ir, b, b2);
-    }
-    a.Ezn[p] = ez_n;
-    a.Ern[p] = er_n;
-    a.Bphin[p] = b;
-    a.Bphi2[p] = b2;
-  }
+  const bool rowe = ir >= 1 && ir <= a.nr - 2;   // the row has E-box points
+  // clamped so the 16-byte loads stay inside the arrays; values of points
+  // outside the E box are never used
+  const int izc = min(iz0, ldz - 2), irc = min(max(ir, 0), a.nr - 1);
+  const int64_t p = fix(ldz, izc, irc), ph = p + ldz;
+  const int fl2 = *reinterpret_cast<const int *>(a.flags + p);   // both columns' flags
+  double2 bphi = make_double2(0, 0), br = bphi, bz = bphi;
+  if (a.tm) bphi = ld2(a.Bphi + p);
   if (a.te) {
-    double b, b2, c, c2;
-    if (bfast) {
-      b = br_c;
-      if (!(fl & BR_ZERO) && iz <= a.cut1[ir]) {
-        double dB = a.dti * (ephi_n - sephi[w][lane - 1]) / a.dz;
-        b2 = b + dB / 2;
-        b = b + dB;
-      } else {
-        b2 = a.Br2[p];
+    br = ld2(a.Br + p);
+    bz = ld2(a.Bz + p);
+  }
+  // z+1 neighbours of the odd column through the next lane (lane 63 loads its own)
+  double bphi_zp1 = __shfl_down(bphi.x, 1, 64), br_zp1 = __shfl_down(br.x, 1, 64);
+  if (lane == 63 && iz0 + 2 <= a.nz) {
+    if (a.tm) bphi_zp1 = a.Bphi[fix(ldz, iz0 + 2, irc)];
+    if (a.te) br_zp1 = a.Br[fix(ldz, iz0 + 2, irc)];
+  }
+  double ez_n[2] = {0, 0}, er_n[2] = {0, 0}, ephi_n[2] = {0, 0};
+  if (rowe) {
+    const int c0 = a.cut0[ir], c1 = a.cut1[ir];
+    if (a.tm) {
+      const double2 ez = ld2(a.Ez + p), jz = ld2(a.Jz + p), er = ld2(a.Er + p), jr = ld2(a.Jr + p);
+      const double2 bphi_hi = ld2(a.Bphi + ph);
+      const double cp = a.cp[ir], cm = a.cm[ir];
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        const int iz = iz0 + c;
+        const int16_t fl = (int16_t)(c ? (fl2 >> 16) : (fl2 & 0xffff));
+        const double bphi_c = el(bphi, c), bphi_zp = c ? bphi_zp1 : bphi.y;
+        ez_n[c] = el(ez, c);
+        if (!(fl & EZ_ZERO) && iz <= c1)
+          ez_n[c] += a.dti * (-el(jz, c) / a.eps0 + (cp * el(bphi_hi, c) - cm * bphi_c));
+        er_n[c] = el(er, c);
+        if (!(fl & ER_ZERO) && iz <= c0)
+          er_n[c] += a.dti * (-el(jr, c) / a.eps0 - a.er_factor * (bphi_zp - bphi_c));
       }
-      c = a.Bz[p];
-      if (!(fl & BZ_ZERO) && iz <= a.cut0[ir]) {
-        double dB = a.bzf[ir] * ((double)(ir + 1) * ephi_n - (double)ir * sephi[w - 1][lane]);
-        c2 = c + dB / 2;
-        c = c + dB;
-      } else {
-        c2 = a.Bz2[p];
-      }
-    } else {
-      f.br_final(iz, ir, b, b2);
-      f.bz_final(iz, ir, c, c2);
     }
-    a.Ephin[p] = ephi_n;
-    a.Brn[p] = b;
-    a.Br2[p] = b2;
-    a.Bzn[p] = c;
-    a.Bz2[p] = c2;
+    if (a.te) {
+      const double2 ephi = ld2(a.Ephi + p), jphi = ld2(a.Jphi + p), bz_hi = ld2(a.Bz + ph);
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        const int iz = iz0 + c;
+        const int16_t fl = (int16_t)(c ? (fl2 >> 16) : (fl2 & 0xffff));
+        const double br_c = el(br, c), br_zp = c ? br_zp1 : br.y;
+        ephi_n[c] = el(ephi, c);
+        if (!(fl & EPHI_ZERO) && iz <= c0)
+          ephi_n[c] += a.dti * (-el(jphi, c) / a.eps0 +
+                                a.ephi_factor * ((br_zp - br_c) / a.dz - (el(bz_hi, c) - el(bz, c)) / a.dr));
+      }
+    }
+  }
+  // points outside the E box (array edges): the loop bodies with the boundary patches
+  F<false> f{a};
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const int iz = iz0 + c;
+    if (ir < 0 || ir > a.nr || iz > a.nz || e_box(a, iz, ir)) continue;
+    if (a.tm) {
+      ez_n[c] = f.ez_final(iz, ir);
+      er_n[c] = f.er_final(iz, ir);
+    }
+    if (a.te) ephi_n[c] = f.ephi_final(iz, ir);
+  }
+  reinterpret_cast<double2 *>(sez[w])[lane] = make_double2(ez_n[0], ez_n[1]);
+  reinterpret_cast<double2 *>(ser[w])[lane] = make_double2(er_n[0], er_n[1]);
+  reinterpret_cast<double2 *>(sephi[w])[lane] = make_double2(ephi_n[0], ephi_n[1]);
+  __syncthreads();
+  if (w == 0 || ir > a.nr) return;
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const int iz = iz0 + c, k = 2 * lane + c;
+    // owned: columns zs+1 .. zs+126, and column 0 in the first tile
+    if (iz > a.nz || k > QZ || (k == 0 && zs != 0)) continue;
+    const int64_t q = fix(ldz, iz, ir);
+    const int16_t fl = (int16_t)(c ? (fl2 >> 16) : (fl2 & 0xffff));
+    const bool bb = b_box(a, iz, ir);   // implies k >= 1 and an E-box row: the LDS neighbours exist
+    if (a.tm) {
+      a.Ezn[q] = ez_n[c];
+      a.Ern[q] = er_n[c];
+      if (bb) {
+        double b = el(bphi, c), b2;
+        if (!(fl & BPHI_ZERO) && iz <= a.cut1[ir]) {
+          double dB = a.dti * ((ez_n[c] - sez[w - 1][k]) / a.dr - (er_n[c] - ser[w][k - 1]) / a.dz);
+          b2 = b + dB / 2;
+          b = b + dB;
+        } else {
+          b2 = a.Bphi2[q];
+        }
+        a.Bphin[q] = b;
+        a.Bphi2[q] = b2;
+      }
+    }
+    if (a.te) {
+      a.Ephin[q] = ephi_n[c];
+      if (bb) {
+        double b = el(br, c), b2;
+        if (!(fl & BR_ZERO) && iz <= a.cut1[ir]) {
+          double dB = a.dti * (ephi_n[c] - sephi[w][k - 1]) / a.dz;
+          b2 = b + dB / 2;
+          b = b + dB;
+        } else {
+          b2 = a.Br2[q];
+        }
+        double d = el(bz, c), d2;
+        if (!(fl & BZ_ZERO) && iz <= a.cut0[ir]) {
+          double dB = a.bzf[ir] * ((double)(ir + 1) * ephi_n[c] - (double)ir * sephi[w - 1][k]);
+          d2 = d + dB / 2;
+          d = d + dB;
+        } else {
+          d2 = a.Bz2[q];
+        }
+        a.Brn[q] = b;
+        a.Br2[q] = b2;
+        a.Bzn[q] = d;
+        a.Bz2[q] = d2;
+      }
+    }
   }
 }
 
@@ -384,8 +467,26 @@ __global__ void add_line_kernel(double *J, int ldz, int direction, int64_t ipos,
 hipError_t launch_fdtd(const FdtdArgs &a, hipStream_t s) {
   int tiles_z = (a.nz + 1 + QZ - 1) / QZ, tiles_r = (a.nr + 1 + QR - 1) / QR;
   int nblocks = tiles_z * tiles_r;
-  int grid = ((nblocks + 7) / 8) * 8;
-  hipLaunchKernelGGL(fdtd_kernel, dim3(grid), dim3(64 * QW), 0, s, a, tiles_z, nblocks);
+  // every row and column outside the interior boxes, each listed once
+  EdgeList el;
+  el.nrows = el.ncols = 0;
+  const int rows[5] = {0, 1, a.nr - 2, a.nr - 1, a.nr}, cols[4] = {0, 1, a.nz - 1, a.nz};
+  for (int r : rows) {
+    bool dup = r < 0;
+    for (int j = 0; j < el.nrows; j++) dup = dup || el.rows[j] == r;
+    if (!dup) el.rows[el.nrows++] = r;
+  }
+  for (int c : cols) {
+    bool dup = c < 0;
+    for (int j = 0; j < el.ncols; j++) dup = dup || el.cols[j] == c;
+    if (!dup) el.cols[el.ncols++] = c;
+  }
+  const int64_t ne = (int64_t)el.nrows * (a.nz + 1) + (int64_t)el.ncols * (a.nr + 1);
+  hipLaunchKernelGGL(fdtd_kernel, dim3((nblocks + 7) / 8 * 8), dim3(64 * QW), 0, s, a, tiles_z, nblocks);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // the edge B updates read the complete new E
+  hipLaunchKernelGGL(fdtd_edge_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, a, el);
   return hipGetLastError();
 }
 
