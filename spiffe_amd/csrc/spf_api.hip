@@ -40,7 +40,7 @@ void free_run(Run *R) {
   F(R->pid); F(R->gen); F(R->status); F(R->counters); F(R->evt); F(R->evflag);
   for (int k = 0; k < 7; k++) F(R->ev[k]);
   F(R->last_old); F(R->evlist); F(R->move_src); F(R->move_dst); F(R->srows); F(R->lrows);
-  F(R->keys); F(R->skeys); F(R->vals); F(R->cz); F(R->scz); F(R->gstart); F(R->long_list); F(R->gflag); F(R->sort_tmp);
+  F(R->keys); F(R->skeys); F(R->vals); F(R->cz); F(R->scz); F(R->gstart); F(R->long_list); F(R->pcount); F(R->gflag); F(R->sort_tmp);
   F(R->bcount); F(R->boff); F(R->stage); F(R->append_stage); F(R->Pdev); F(R->omega); F(R->red);
   F(R->red_counter); F(R->sor_state); F(R->sor_sums);
   R->live = false;
@@ -253,7 +253,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   for (int k = 0; k < 16; k++) ALLOC(R->P[k], cap);
   ALLOC(R->pid, cap); ALLOC(R->gen, cap); ALLOC(R->status, cap);
   ALLOC(R->counters, 16);
-  ALLOC(R->evt, cap); ALLOC(R->evflag, cap);
+  ALLOC(R->evt, cap); ALLOC(R->evflag, cap); ALLOC(R->pcount, cap / 256 + 2);
   for (int k = 0; k < 7; k++) ALLOC(R->ev[k], cap);
   ALLOC(R->last_old, 4); ALLOC(R->evlist, cap); ALLOC(R->move_src, cap); ALLOC(R->move_dst, cap);
   R->srow_cap = default_srow_cap(d);
@@ -427,6 +427,7 @@ int spf_field_upload(spf_ctx *c, int32_t run, int32_t field, const double *host)
   HIPCALL(c, hipMemcpy(R->stage, host, n * sizeof(double), hipMemcpyHostToDevice));
   HIPCALL(c, launch_transpose_in(R->stage, dst, R->d.nz, R->d.nr, R->ldz, c->stream));
   HIPCALL(c, hipStreamSynchronize(c->stream));
+  if (field == SPF_EZ_IMPOSED || field == SPF_ER_IMPOSED) R->imposed_e = true;
   return SPF_OK;
 }
 
@@ -670,6 +671,7 @@ int spf_particles_push(spf_ctx *c, int32_t run, double zForcesStart) {
   a.Bz2 = R->Bz2;
   a.EzImp = R->EzImp;
   a.ErImp = R->ErImp;
+  a.imp_e = R->imposed_e;
   a.BzImp = R->BzImp;
   a.BrImp = R->BrImp;
   a.flags = R->flags;
@@ -684,6 +686,7 @@ int spf_particles_push(spf_ctx *c, int32_t run, double zForcesStart) {
   a.np = R->counters;
   a.evt = R->evt;
   a.evflag = R->evflag;
+  a.pcount = R->pcount;
   a.ev_zold = R->ev[0]; a.ev_rold = R->ev[1]; a.ev_vz = R->ev[2]; a.ev_vr = R->ev[3];
   a.ev_pz0 = R->ev[4]; a.ev_pr0 = R->ev[5]; a.ev_rp0 = R->ev[6];
   a.last_old = R->last_old;
@@ -699,9 +702,7 @@ int spf_particles_push(spf_ctx *c, int32_t run, double zForcesStart) {
   }
   RowsArgs w;
   PhaseTimer pt(c, SPF_T_PUSH);
-  HIPCALL(c, launch_set_i64(R->counters + 7, 0, c->stream));
   HIPCALL(c, launch_push(a, R->np_ub, c->stream));
-  HIPCALL(c, ordered_compact(R->evflag, R->np_ub, R->evlist, R->counters + 2, R->bcount, R->boff, c->stream));
   w.nz = d.nz;
   w.nr = d.nr;
   w.ldz = R->ldz;
@@ -715,11 +716,15 @@ int spf_particles_push(spf_ctx *c, int32_t run, double zForcesStart) {
   w.time = time;
   w.step = R->time_step;
   w.flags = R->flags;
+  w.evflag = R->evflag;
+  w.pcount = R->pcount;
+  w.n_ub = R->np_ub;
+  w.npb = (R->np_ub + 255) / 256;
   w.evlist = R->evlist;
   w.n_ev = R->counters + 2;
   w.np = R->counters;
   w.np_active = R->counters + 1;
-  w.new_active = (const unsigned long long *)(R->counters + 7);
+  w.new_active = (unsigned long long *)(R->counters + 7);
   w.evt = R->evt;
   w.ev_zold = R->ev[0]; w.ev_rold = R->ev[1]; w.ev_vz = R->ev[2]; w.ev_vr = R->ev[3];
   w.ev_pz0 = R->ev[4]; w.ev_pr0 = R->ev[5]; w.ev_rp0 = R->ev[6];

@@ -232,6 +232,7 @@ int spf_imposed_efield(spf_ctx *c, int32_t run, int32_t initialize, double ramp_
   }
   if (ff == R->last_field_factor) return SPF_OK;
   R->last_field_factor = ff;
+  R->imposed_e = true;
   HIPCALL(c, launch_field_from_psi(*R, 1, ff, c->stream));
   return SPF_OK;
 }

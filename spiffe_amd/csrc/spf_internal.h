@@ -35,7 +35,7 @@ struct FdtdArgs {
 
 struct PushArgs {
   int nz, nr, ldz;
-  int tm, te, longit, radial, has_sol;
+  int tm, te, longit, radial, has_sol, imp_e;
   double zmin, zmax, rmax, dz, dr, dt, time, c, QoMC, stiffness, zForcesStart;
   const double *Ez, *Er, *Ephi, *Bphi2, *Br2, *Bz2, *EzImp, *ErImp, *BzImp, *BrImp;
   const int16_t *flags;
@@ -47,6 +47,7 @@ struct PushArgs {
   // per-step event staging (indexed by particle)
   uint32_t *evt;
   uint8_t *evflag;
+  int32_t *pcount;                  // events per push workgroup
   double *ev_zold, *ev_rold, *ev_vz, *ev_vr, *ev_pz0, *ev_pr0, *ev_rp0;
   double *last_old;                 // old pz, pr, r_pphi of particle np-1
   unsigned long long *new_active;   // activations this step
@@ -61,10 +62,13 @@ struct RowsArgs {
   double zmin, zmax, dz, dr, dt, time;
   int64_t step;
   const int16_t *flags;
-  const int32_t *evlist;
-  const int64_t *n_ev;
+  const uint8_t *evflag;            // per particle: has an event this step
+  const int32_t *pcount;            // events per push workgroup
+  int64_t npb, n_ub;                // push workgroups, particle slots they covered
+  int32_t *evlist;                  // built here: particles with events, in index order
+  int64_t *n_ev;
   int64_t *np, *np_active;
-  const unsigned long long *new_active;
+  unsigned long long *new_active;   // read, then cleared for the next step
   const uint32_t *evt;
   const double *ev_zold, *ev_rold, *ev_vz, *ev_vr, *ev_pz0, *ev_pr0, *ev_rp0, *last_old;
   double *Q, *z, *r, *pz, *pr, *rpphi, *vz, *vr, *vphi, *r0, *z0, *t0, *rmax_p, *rmin_p, *zmax_p, *zmin_p;
@@ -129,6 +133,8 @@ struct Run {
   double *cz = nullptr, *scz = nullptr;   // contributions, unsorted and sorted
   int32_t *vals = nullptr;                // list of long segments
   int32_t *gstart = nullptr;              // segment starts
+  int32_t *pcount = nullptr;              // events per push workgroup
+  bool imposed_e = false;                 // EzImposed/ErImposed may hold nonzero values
   int32_t *long_list = nullptr;           // long segments, folded one per wavefront
   uint8_t *gflag = nullptr;
   void *sort_tmp = nullptr;

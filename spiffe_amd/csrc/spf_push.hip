@@ -31,6 +31,9 @@ struct G {
   const PushArgs &a;
   __device__ int16_t fl(long long iz, long long ir) const { return a.flags[fix(a.ldz, (int)iz, (int)ir)]; }
   __device__ double at(const double *p, long long iz, long long ir) const { return p[fix(a.ldz, (int)iz, (int)ir)]; }
+  // imposed E: while the run has never set one its arrays hold +0.0, so the
+  // literal +0.0 gives the same sums without the loads
+  __device__ double imp(const double *p, long long iz, long long ir) const { return a.imp_e ? at(p, iz, ir) : 0.0; }
 
   // src/advance_particles.c:31-65
   __device__ short loss_code(long long iz, long long ir) const {
@@ -95,8 +98,8 @@ struct G {
         iz1 = iz;
       }
       if (a.tm)
-        Ez = fb * (fl_ * (at(a.Ez, iz, ir) + at(a.EzImp, iz, ir)) + fr * (at(a.Ez, iz1, ir) + at(a.EzImp, iz1, ir))) +
-             fa * (fl_ * (at(a.Ez, iz, ir1) + at(a.EzImp, iz, ir1)) + fr * (at(a.Ez, iz1, ir1) + at(a.EzImp, iz1, ir1)));
+        Ez = fb * (fl_ * (at(a.Ez, iz, ir) + imp(a.EzImp, iz, ir)) + fr * (at(a.Ez, iz1, ir) + imp(a.EzImp, iz1, ir))) +
+             fa * (fl_ * (at(a.Ez, iz, ir1) + imp(a.EzImp, iz, ir1)) + fr * (at(a.Ez, iz1, ir1) + imp(a.EzImp, iz1, ir1)));
       if (r != 0 && a.te)
         Br = fb * (fl_ * at(a.Br2, iz, ir) + fr * at(a.Br2, iz1, ir)) + fa * (fl_ * at(a.Br2, iz, ir1) + fr * at(a.Br2, iz1, ir1));
     }
@@ -130,8 +133,8 @@ struct G {
       else if (fl(iz + 1, ir + 1) & SPF_FL_EZ_ZERO && fl(iz, ir + 1) & SPF_FL_EZ_ZERO)
         ir1 = ir;
       if (r != 0 && a.tm)
-        Er = fl_ * (fb * (at(a.Er, iz, ir) + at(a.ErImp, iz, ir)) + fa * (at(a.Er, iz, ir1) + at(a.ErImp, iz, ir1))) +
-             fr * (fb * (at(a.Er, iz1, ir) + at(a.ErImp, iz1, ir)) + fa * (at(a.Er, iz1, ir1) + at(a.ErImp, iz1, ir1)));
+        Er = fl_ * (fb * (at(a.Er, iz, ir) + imp(a.ErImp, iz, ir)) + fa * (at(a.Er, iz, ir1) + imp(a.ErImp, iz, ir1))) +
+             fr * (fb * (at(a.Er, iz1, ir) + imp(a.ErImp, iz1, ir)) + fa * (at(a.Er, iz1, ir1) + imp(a.ErImp, iz1, ir1)));
       if (a.te)
         Bz = fl_ * (fb * at(a.Bz2, iz, ir) + fa * at(a.Bz2, iz, ir1)) + fr * (fb * at(a.Bz2, iz1, ir) + fa * at(a.Bz2, iz1, ir1));
     }
@@ -235,15 +238,9 @@ struct G {
   }
 };
 
-// src/advance_particles.c:131-339 for one particle
-__global__ __launch_bounds__(256) void push_kernel(PushArgs a, int64_t n_ub) {
-  int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip >= n_ub) return;
-  int64_t np = *a.np;
-  if (ip >= np) {
-    a.evflag[ip] = 0;
-    return;
-  }
+// src/advance_particles.c:131-339 for one particle; returns whether the
+// particle has an event (loss or screen crossing) this step
+__device__ inline bool push_one(const PushArgs &a, int64_t ip, int64_t np) {
   G g{a};
   double dt = a.dt, dz = a.dz, dr = a.dr, zmin = a.zmin, zmax = a.zmax, rmax = a.rmax;
   double z = a.z[ip], r = a.r[ip];
@@ -358,7 +355,6 @@ __global__ __launch_bounds__(256) void push_kernel(PushArgs a, int64_t n_ub) {
     if (fwd || bwd) ev |= 1u << (2 + s);
   }
   bool any = (ev & ~EV_RNEG) != 0;
-  a.evflag[ip] = any ? 1 : 0;
   if (any) {
     a.evt[ip] = ev;
     a.ev_zold[ip] = zold;
@@ -369,6 +365,19 @@ __global__ __launch_bounds__(256) void push_kernel(PushArgs a, int64_t n_ub) {
     a.ev_pr0[ip] = pr0;
     a.ev_rp0[ip] = rp0;
   }
+  return any;
+}
+
+// one thread per particle; each workgroup also counts its events for the
+// ordered event list that rows_kernel builds
+__global__ __launch_bounds__(256) void push_kernel(PushArgs a, int64_t n_ub) {
+  const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t np = *a.np;
+  bool any = false;
+  if (ip < np && ip < n_ub) any = push_one(a, ip, np);
+  if (ip < n_ub) a.evflag[ip] = any ? 1 : 0;
+  const int cnt = __syncthreads_count(any);
+  if (threadIdx.x == 0) a.pcount[blockIdx.x] = cnt;
 }
 
 // src/secondaryEmission.c:239-372
@@ -472,7 +481,53 @@ __device__ int block_flag_scan(bool f, int *wsum, int &excl) {
 __global__ __launch_bounds__(ROWS_THREADS) void rows_kernel(RowsArgs a) {
   __shared__ int wsum[ROWS_THREADS / 64];
   __shared__ long long sh_first_lost;
-  const int64_t n_ev = *a.n_ev;
+  __shared__ int qblk[ROWS_THREADS];
+  __shared__ long long qoff[ROWS_THREADS];
+  // ---- ordered event list: push workgroups in order (a scan of their
+  // counts), then each workgroup's flagged particles in order (ballots)
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = ROWS_THREADS / 64;
+  int64_t n_ev = 0;
+  for (int64_t c = 0; c < a.npb; c += blockDim.x) {
+    const int64_t b = c + threadIdx.x;
+    const int cnt = b < a.npb ? a.pcount[b] : 0;
+    // exclusive prefix of the counts, and the list of workgroups with events
+    int excl_n, excl_b;
+    const int tot_b = block_flag_scan(cnt > 0, wsum, excl_b);
+    int v = cnt;
+    for (int off = 1; off < 64; off <<= 1) {
+      int u = __shfl_up(v, off, 64);
+      if (lane >= off) v += u;
+    }
+    if (lane == 63) wsum[wv] = v;
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int k = 0; k < nwv; k++) {
+      before += k < wv ? wsum[k] : 0;
+      total += wsum[k];
+    }
+    __syncthreads();
+    excl_n = before + v - cnt;
+    if (cnt > 0) {
+      qblk[excl_b] = (int)b;
+      qoff[excl_b] = n_ev + excl_n;
+    }
+    __syncthreads();
+    // one wavefront per listed workgroup: 256 flags as four ballots
+    for (int q = wv; q < tot_b; q += nwv) {
+      const int64_t base = (int64_t)qblk[q] * 256;
+      int64_t o = qoff[q];
+      for (int j = 0; j < 4; j++) {
+        const int64_t i = base + 64 * j + lane;
+        const bool f = i < a.n_ub && a.evflag[i];
+        const unsigned long long m = __ballot(f);
+        if (f) a.evlist[o + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)i;
+        o += __popcll(m);
+      }
+    }
+    n_ev += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *a.n_ev = n_ev;
   const int64_t np = a.np[0];
   const double dt = a.dt;
   // ---- screens
@@ -581,7 +636,11 @@ __global__ __launch_bounds__(ROWS_THREADS) void rows_kernel(RowsArgs a) {
   __syncthreads();
   if (threadIdx.x == 0) *a.n_lrows = lbase + nlost;
   // ---- compaction, src/advance_particles.c:490-518
-  int64_t new_active = (int64_t)*a.new_active;
+  int64_t new_active = 0;
+  if (threadIdx.x == 0) {
+    new_active = (int64_t)*a.new_active;
+    *a.new_active = 0;   // for the next step's activations
+  }
   if (a.compat) {
     // the reference re-tests the same slot without moving its lost flag, so
     // np ends at the first lost index
