@@ -216,10 +216,10 @@ __device__ inline void sor_setup(const SorCommon &a, const SorTile<G> &t, int w,
 // tile row w*RS + i, whose parity is the parity of i (rlo is even); its
 // colour-C point is column (i + C) & 1 of the lane's pair.  Returns the sum of
 // |residual| over the lane's updated owned points.
-template <class G, int C, bool AX>
+template <class G, int C>
 __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsigned okb, const double2 up_row,
                                         const double2 dn_row, const double *cr, const double *dr, int w,
-                                        double omega, bool redge) {
+                                        double omega, bool redge, bool AX) {
   const double A = a.a, E = a.e;
   double q[G::RS], x[G::RS], res[G::RS];
   bool slow = false;
@@ -241,11 +241,15 @@ __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsi
     const double up = i < G::RS - 1 ? ln.P[i + 1 < G::RS ? i + 1 : i][col] : (col ? up_row.y : up_row.x);
     const double c = cr[lr];
     double residual, diag, y;
-    if (AX && i == 0) {
-      // axis row, src/poisson.c:84-90: no "below" term, a 4*c "above" weight, diagonal e - 2*c
-      diag = E - 2 * c;
-      y = a.y_axis;
-      residual = A * zs + 4 * c * up + diag * ln.P[i][col] - ln.D[i][col];
+    if (i == 0) {
+      // row 0 of the tile may be the axis row, src/poisson.c:84-90: no "below"
+      // term (added as -0.0, an exact no-op), a 4*c "above" weight, diagonal e - 2*c
+      const double down = col ? dn_row.y : dn_row.x;
+      const double cu = AX ? 4 * c : c;
+      const double dn = AX ? -0.0 : dr[lr] * down;
+      diag = AX ? E - 2 * c : E;
+      y = AX ? a.y_axis : a.y_e;
+      residual = A * zs + cu * up + dn + diag * ln.P[i][col] - ln.D[i][col];
     } else {
       const double down = i > 0 ? ln.P[i > 0 ? i - 1 : 0][col] : (col ? dn_row.y : dn_row.x);
       diag = E;
@@ -276,40 +280,53 @@ __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsi
   return acc;
 }
 
-// k <= SK half-sweeps n0 .. n0+k-1; ares[j] gets the lane's owned residual sum of step j
+// One local step j of a batch, colour C; returns the lane's owned residual sum.
+template <class G, int C>
+__device__ inline double sor_step(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane,
+                                  int64_t n, int j, double2 (*xrow)[G::SW][2][64], const double *cr,
+                                  const double *dr, bool axis_wave) {
+  const int par = j & 1;
+  // publish the first and last rows, then read the neighbouring wavefronts' rows
+  xrow[par][w][0][lane] = make_double2(ln.P[0][0], ln.P[0][1]);
+  xrow[par][w][1][lane] = make_double2(ln.P[G::RS - 1][0], ln.P[G::RS - 1][1]);
+  __syncthreads();
+  const double2 up_row = w + 1 < G::SW ? xrow[par][w + 1][0][lane] : make_double2(0.0, 0.0);
+  const double2 dn_row = w > 0 ? xrow[par][w - 1][1][lane] : make_double2(0.0, 0.0);
+  const double omega = a.omega[n];
+  // points valid at this step: column depth and the wavefront's share of the valid rows
+  const int vr0 = t.sr_lo ? j + 1 : 0, vr1 = t.LR - (t.sr_hi ? j + 1 : 0);
+  const int lo = min(max(vr0 - w * G::RS, 0), G::RS), hi = min(max(vr1 - w * G::RS, 0), G::RS);
+  const unsigned rmask = hi > lo ? (((1u << (2 * hi)) - 1u) & ~((1u << (2 * lo)) - 1u)) : 0u;
+  const unsigned okb = ln.upd & rmask & ((ln.zdepth[0] > j ? G::COL0 : 0u) | (ln.zdepth[1] > j ? G::COL1 : 0u));
+  return sor_half_sweep<G, C>(a, ln, okb, up_row, dn_row, cr, dr, w, omega, t.redge, axis_wave);
+}
+
+// k <= SK half-sweeps n0 .. n0+k-1; ares[j] gets the lane's owned residual sum
+// of step j.  For an even n0 (every batch of a solve) the steps are unrolled
+// with their colours 0, 1, 0, ... known at compile time.
 template <class G>
 __device__ inline void sor_sweeps(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane,
                                   int64_t n0, int k, double2 (*xrow)[G::SW][2][64], const double *cr,
                                   const double *dr, double (&ares)[G::SK]) {
-  const bool axis_wave = t.rlo == 0 && w == 0;
 #pragma unroll
   for (int j = 0; j < G::SK; j++) ares[j] = 0;
-  for (int j = 0; j < k; j++) {
-    const int par = j & 1;
-    // publish the first and last rows, then read the neighbouring wavefronts' rows
-    xrow[par][w][0][lane] = make_double2(ln.P[0][0], ln.P[0][1]);
-    xrow[par][w][1][lane] = make_double2(ln.P[G::RS - 1][0], ln.P[G::RS - 1][1]);
-    __syncthreads();
-    const double2 up_row = w + 1 < G::SW ? xrow[par][w + 1][0][lane] : make_double2(0.0, 0.0);
-    const double2 dn_row = w > 0 ? xrow[par][w - 1][1][lane] : make_double2(0.0, 0.0);
-    const double omega = a.omega[n0 + j];
-    // points valid at this step: column depth and the wavefront's share of the valid rows
-    const int vr0 = t.sr_lo ? j + 1 : 0, vr1 = t.LR - (t.sr_hi ? j + 1 : 0);
-    const int lo = min(max(vr0 - w * G::RS, 0), G::RS), hi = min(max(vr1 - w * G::RS, 0), G::RS);
-    const unsigned rmask = hi > lo ? (((1u << (2 * hi)) - 1u) & ~((1u << (2 * lo)) - 1u)) : 0u;
-    const unsigned okb =
-        ln.upd & rmask & ((ln.zdepth[0] > j ? G::COL0 : 0u) | (ln.zdepth[1] > j ? G::COL1 : 0u));
-    double acc;
-    const bool odd = ((n0 + j) & 1) != 0;
-    if (axis_wave)
-      acc = odd ? sor_half_sweep<G, 1, true>(a, ln, okb, up_row, dn_row, cr, dr, w, omega, t.redge)
-                : sor_half_sweep<G, 0, true>(a, ln, okb, up_row, dn_row, cr, dr, w, omega, t.redge);
-    else
-      acc = odd ? sor_half_sweep<G, 1, false>(a, ln, okb, up_row, dn_row, cr, dr, w, omega, t.redge)
-                : sor_half_sweep<G, 0, false>(a, ln, okb, up_row, dn_row, cr, dr, w, omega, t.redge);
+  const bool axis_wave = t.rlo == 0 && w == 0;
+  if ((n0 & 1) == 0) {
 #pragma unroll
-    for (int jj = 0; jj < G::SK; jj++)
-      if (jj == j) ares[jj] = acc;
+    for (int j = 0; j < G::SK; j++) {
+      if (j >= k) break;
+      ares[j] = (j & 1) ? sor_step<G, 1>(a, t, ln, w, lane, n0 + j, j, xrow, cr, dr, axis_wave)
+                        : sor_step<G, 0>(a, t, ln, w, lane, n0 + j, j, xrow, cr, dr, axis_wave);
+    }
+  } else {
+#pragma unroll 1
+    for (int j = 0; j < k; j++) {
+      const double acc = ((n0 + j) & 1) ? sor_step<G, 1>(a, t, ln, w, lane, n0 + j, j, xrow, cr, dr, axis_wave)
+                                        : sor_step<G, 0>(a, t, ln, w, lane, n0 + j, j, xrow, cr, dr, axis_wave);
+#pragma unroll
+      for (int jj = 0; jj < G::SK; jj++)
+        if (jj == j) ares[jj] = acc;
+    }
   }
 }
 
