@@ -361,55 +361,76 @@ __global__ __launch_bounds__(64 * QW) void fdtd_kernel(FdtdArgs a, int tiles_z, 
   reinterpret_cast<double2 *>(sephi[w])[lane] = make_double2(ephi_n[0], ephi_n[1]);
   __syncthreads();
   if (w == 0 || ir > a.nr) return;
+  // new B of the owned points in the B box; then every owned value is stored,
+  // a lane's two columns as one 16-byte store when both are written
+  bool own[2], bb[2];
+  double bphi_n[2], bphi2_n[2], br_n[2], br2_n[2], bz_n[2], bz2_n[2];
 #pragma unroll
   for (int c = 0; c < 2; c++) {
     const int iz = iz0 + c, k = 2 * lane + c;
     // owned: columns zs+1 .. zs+126, and column 0 in the first tile
-    if (iz > a.nz || k > QZ || (k == 0 && zs != 0)) continue;
+    own[c] = !(iz > a.nz || k > QZ || (k == 0 && zs != 0));
+    bb[c] = own[c] && b_box(a, iz, ir);   // implies k >= 1 and an E-box row: the LDS neighbours exist
+    bphi_n[c] = bphi2_n[c] = br_n[c] = br2_n[c] = bz_n[c] = bz2_n[c] = 0;
+    if (!bb[c]) continue;
     const int64_t q = fix(ldz, iz, ir);
     const int16_t fl = (int16_t)(c ? (fl2 >> 16) : (fl2 & 0xffff));
-    const bool bb = b_box(a, iz, ir);   // implies k >= 1 and an E-box row: the LDS neighbours exist
     if (a.tm) {
-      a.Ezn[q] = ez_n[c];
-      a.Ern[q] = er_n[c];
-      if (bb) {
-        double b = el(bphi, c), b2;
-        if (!(fl & BPHI_ZERO) && iz <= a.cut1[ir]) {
-          double dB = a.dti * ((ez_n[c] - sez[w - 1][k]) / a.dr - (er_n[c] - ser[w][k - 1]) / a.dz);
-          b2 = b + dB / 2;
-          b = b + dB;
-        } else {
-          b2 = a.Bphi2[q];
-        }
-        a.Bphin[q] = b;
-        a.Bphi2[q] = b2;
+      double b = el(bphi, c), b2;
+      if (!(fl & BPHI_ZERO) && iz <= a.cut1[ir]) {
+        double dB = a.dti * ((ez_n[c] - sez[w - 1][k]) / a.dr - (er_n[c] - ser[w][k - 1]) / a.dz);
+        b2 = b + dB / 2;
+        b = b + dB;
+      } else {
+        b2 = a.Bphi2[q];
       }
+      bphi_n[c] = b;
+      bphi2_n[c] = b2;
     }
     if (a.te) {
-      a.Ephin[q] = ephi_n[c];
-      if (bb) {
-        double b = el(br, c), b2;
-        if (!(fl & BR_ZERO) && iz <= a.cut1[ir]) {
-          double dB = a.dti * (ephi_n[c] - sephi[w][k - 1]) / a.dz;
-          b2 = b + dB / 2;
-          b = b + dB;
-        } else {
-          b2 = a.Br2[q];
-        }
-        double d = el(bz, c), d2;
-        if (!(fl & BZ_ZERO) && iz <= a.cut0[ir]) {
-          double dB = a.bzf[ir] * ((double)(ir + 1) * ephi_n[c] - (double)ir * sephi[w - 1][k]);
-          d2 = d + dB / 2;
-          d = d + dB;
-        } else {
-          d2 = a.Bz2[q];
-        }
-        a.Brn[q] = b;
-        a.Br2[q] = b2;
-        a.Bzn[q] = d;
-        a.Bz2[q] = d2;
+      double b = el(br, c), b2;
+      if (!(fl & BR_ZERO) && iz <= a.cut1[ir]) {
+        double dB = a.dti * (ephi_n[c] - sephi[w][k - 1]) / a.dz;
+        b2 = b + dB / 2;
+        b = b + dB;
+      } else {
+        b2 = a.Br2[q];
       }
+      double d = el(bz, c), d2;
+      if (!(fl & BZ_ZERO) && iz <= a.cut0[ir]) {
+        double dB = a.bzf[ir] * ((double)(ir + 1) * ephi_n[c] - (double)ir * sephi[w - 1][k]);
+        d2 = d + dB / 2;
+        d = d + dB;
+      } else {
+        d2 = a.Bz2[q];
+      }
+      br_n[c] = b;
+      br2_n[c] = b2;
+      bz_n[c] = d;
+      bz2_n[c] = d2;
     }
+  }
+  const int64_t q0 = fix(ldz, min(iz0, ldz - 2), ir);
+  auto put = [&](double *dst, const double (&v)[2], const bool (&m)[2]) {
+    if (m[0] && m[1])
+      *reinterpret_cast<double2 *>(dst + q0) = make_double2(v[0], v[1]);
+    else {
+      if (m[0]) dst[q0] = v[0];
+      if (m[1]) dst[q0 + 1] = v[1];
+    }
+  };
+  if (a.tm) {
+    put(a.Ezn, ez_n, own);
+    put(a.Ern, er_n, own);
+    put(a.Bphin, bphi_n, bb);
+    put(a.Bphi2, bphi2_n, bb);
+  }
+  if (a.te) {
+    put(a.Ephin, ephi_n, own);
+    put(a.Brn, br_n, bb);
+    put(a.Br2, br2_n, bb);
+    put(a.Bzn, bz_n, bb);
+    put(a.Bz2, bz2_n, bb);
   }
 }
 
