@@ -4,7 +4,7 @@
  * The reference calls its per-step physics as plain C functions on
  * caller-owned FIELDS/BEAM structs (src/spiffe.h:474-561).  This library owns
  * the field and particle state in HBM instead, behind a context handle; the
- * host keeps the deck, the emission models and all file I/O.  Every entry
+ * host keeps the deck and all file I/O.  Every entry
  * point returns 0 on success or a nonzero SPF_E* code; spf_last_error() gives
  * the message.  Calls are stream-ordered on the context's HIP stream and do
  * not block unless documented.  A context is used from one host thread and
@@ -205,6 +205,57 @@ int spf_poisson_sweeps(spf_ctx *ctx, int32_t run, int32_t which, int64_t n0, int
 /* replaces compute_imposed_Efield, src/poisson_correction.c:456-562 */
 int spf_imposed_efield(spf_ctx *ctx, int32_t run, int32_t initialize, double ramp_time, double flat_top_time);
 
+/* ---- cathode emission, src/cathode.c ----
+ * The define_cathode namelist (src/cathode.nl).  spf_cathode_defaults() fills
+ * the defaults of src/cathode.c:32-40.  The emission profile arrives as two
+ * arrays (the host reads the SDDS file named by `profile`, src/cathode.c:149-166);
+ * emission logs are not written.  Random draws come from one stream per run,
+ * re-seeded by every define (as random_1(-seed) re-seeds the reference's one
+ * stream): Philox4x64-10 keyed by |random_number_seed|, so the particles drawn
+ * are the library's own, not the SDDS toolkit's. */
+typedef struct {
+  double z_position, inner_radius, outer_radius;
+  double current_density, temperature, work_function;   /* work_function in eV */
+  int32_t field_emission, add_thermal_velocities;
+  double field_emission_beta;
+  int32_t determine_temperature, autophase;
+  double electrons_per_macroparticle, number_per_step;
+  double start_time, stop_time, time_offset;
+  double initial_pz, initial_omega, stiffness;
+  int32_t discretize_radii, spread_over_dt, zoned_emission;
+  int32_t distribution_correction_interval;
+  int32_t halton_radix_dt, halton_radix_r;
+  int64_t random_number_seed;
+  int64_t n_profile;                          /* 0: no profile */
+  const double *profile_time, *profile_factor;
+  double e_mks, me_mks, k_boltzmann_mks;      /* physical constants used by the host */
+} spf_cathode_desc;
+
+/* What a cathode holds after its first emission (CATHODE, src/spiffe.h:236-262). */
+typedef struct {
+  double current_density, ref_current_density, number_per_step, electrons_per_macroparticle;
+  double Q, temperature, sigma_v, inner_radius, outer_radius, initial_vz;
+  int64_t start_step, stop_step, n_bins, n_emitted, last_count;
+} spf_cathode_state;
+
+void spf_cathode_defaults(spf_cathode_desc *desc);
+/* process_cathode_definition, src/cathode.c:21-237: checks the namelist and adds
+ * one cathode, or one per radial cell with field_emission; returns the index of
+ * the first and the number added.  The cathode's q/(m c) and stiffness must equal
+ * the run's (src/cathode.c:661-663 copies them into the beam). */
+int spf_cathode_define(spf_ctx *ctx, int32_t run, const spf_cathode_desc *desc, int32_t *first, int32_t *count);
+/* set_current_density, src/cathode.c:673-715: Richardson-Schottky or field
+ * emission from Ez at the cathode centre; synchronizing when a model reads Ez. */
+int spf_cathode_set_current_density(spf_ctx *ctx, int32_t run, int32_t cathode);
+/* emit_electrons_from_cathode, src/cathode.c:239-671, at the run's current step:
+ * the count and the random draws on the host, the new particles written at the
+ * end of the device arrays; n_emitted (may be NULL) gets the count. */
+int spf_cathode_emit(spf_ctx *ctx, int32_t run, int32_t cathode, int64_t *n_emitted);
+int spf_cathode_state_get(spf_ctx *ctx, int32_t run, int32_t cathode, spf_cathode_state *out);
+/* the run's nextParticleID (src/spiffe.c:191 starts it at 1): set_next > 0 sets
+ * it; next (may be NULL) gets the value in force */
+int spf_particle_id_next(spf_ctx *ctx, int32_t run, int64_t set_next, int64_t *next);
+
 /* ---- phase timers ----
  * No counterpart in the reference: telemetry for benchmarks and serving loops.
  * When enabled, a pair of HIP events on the context's stream brackets one call
@@ -228,8 +279,10 @@ int spf_timing_enable(spf_ctx *ctx, int32_t every);
 int spf_timing_read(spf_ctx *ctx, double *ms, int64_t *timed, int64_t *launched);
 
 /* ---- one whole step of the loop body, src/integrate.c:134-194 ----
- * time <- step*dt+start; J reset; deposit (if SPF_SPACE_CHARGE); n_lines line
- * currents; Poisson correction (if set up); FDTD; push; step++. */
+ * ramped imposed E (if set); time <- step*dt+start; emission from every
+ * cathode of the run; J reset; deposit (if SPF_SPACE_CHARGE); n_lines line
+ * currents; Poisson correction (if set up); FDTD; push; current densities of
+ * the cathodes; step++. */
 int spf_step(spf_ctx *ctx, int32_t run, int32_t n_lines, const int32_t *line_dir, const int64_t *line_pos,
              const int64_t *line_start, const int64_t *line_end, const double *line_J);
 

@@ -69,6 +69,30 @@ class OPoisson(ctypes.Structure):
                 ("last_iterations", ctypes.c_int64), ("Qreal", _d), ("Qerror", _d), ("Qsurface", _d)]
 
 
+class OCathodeDesc(ctypes.Structure):
+    """ocathode_desc: the define_cathode namelist (src/cathode.nl)."""
+    _fields_ = [
+        ("z_position", _d), ("inner_radius", _d), ("outer_radius", _d),
+        ("current_density", _d), ("temperature", _d), ("work_function", _d),
+        ("field_emission", ctypes.c_int32), ("add_thermal_velocities", ctypes.c_int32),
+        ("field_emission_beta", _d),
+        ("determine_temperature", ctypes.c_int32), ("autophase", ctypes.c_int32),
+        ("electrons_per_macroparticle", _d), ("number_per_step", _d),
+        ("start_time", _d), ("stop_time", _d), ("time_offset", _d),
+        ("initial_pz", _d), ("initial_omega", _d), ("stiffness", _d),
+        ("discretize_radii", ctypes.c_int32), ("spread_over_dt", ctypes.c_int32),
+        ("zoned_emission", ctypes.c_int32), ("distribution_correction_interval", ctypes.c_int32),
+        ("halton_radix_dt", ctypes.c_int32), ("halton_radix_r", ctypes.c_int32),
+        ("random_number_seed", ctypes.c_int64),
+        ("n_profile", ctypes.c_int64), ("profile_time", _dp), ("profile_factor", _dp),
+        ("e_mks", _d), ("me_mks", _d), ("k_boltzmann_mks", _d)]
+
+
+class ORng(ctypes.Structure):
+    _fields_ = [("key0", ctypes.c_uint64), ("block", ctypes.c_uint64), ("buf", ctypes.c_uint64 * 4),
+                ("pos", ctypes.c_int32), ("has_saved", ctypes.c_int32), ("saved", _d)]
+
+
 SCREEN_ROW_DTYPE = np.dtype(OScreenRow)
 LOST_ROW_DTYPE = np.dtype(OLostRow)
 FIELD_NAMES = ["Ez", "Jz", "Er", "Jr", "Bphi", "Bphi2", "Ephi", "Jphi", "Br", "Br2", "Bz", "Bz2",
@@ -116,6 +140,31 @@ def load():
         lib.oracle_antenna_current.argtypes = [_dp, _dp, ctypes.c_int64, P(ctypes.c_int64), _d, _d, _d, _d,
                                                P(ctypes.c_int32)]
         lib.oracle_antenna_current.restype = _d
+        u64p = ctypes.POINTER(ctypes.c_uint64)
+        lib.oracle_philox4x64_10.argtypes = [u64p, u64p, u64p]
+        lib.oracle_rng_seed.argtypes = [P(ORng), ctypes.c_uint64]
+        lib.oracle_rng_next64.argtypes = [P(ORng)]
+        lib.oracle_rng_next64.restype = ctypes.c_uint64
+        lib.oracle_rng_uniform.argtypes = [P(ORng)]
+        lib.oracle_rng_uniform.restype = _d
+        lib.oracle_rng_gauss.argtypes = [P(ORng)]
+        lib.oracle_rng_gauss.restype = _d
+        lib.oracle_radical_inverse.argtypes = [ctypes.c_uint64, ctypes.c_int32]
+        lib.oracle_radical_inverse.restype = _d
+        lib.oracle_cathodes_create.restype = ctypes.c_void_p
+        lib.oracle_cathodes_destroy.argtypes = [ctypes.c_void_p]
+        lib.oracle_cathodes_rng.argtypes = [ctypes.c_void_p]
+        lib.oracle_cathodes_rng.restype = P(ORng)
+        lib.oracle_cathodes_next_id.argtypes = [ctypes.c_void_p]
+        lib.oracle_cathodes_next_id.restype = ctypes.c_int64
+        lib.oracle_cathode_define.argtypes = [ctypes.c_void_p, P(OFields), P(OCathodeDesc), P(ctypes.c_int32),
+                                              P(ctypes.c_int32), ctypes.c_char_p, ctypes.c_int64]
+        lib.oracle_field_emission_current_density.argtypes = [_d, _d, _d]
+        lib.oracle_field_emission_current_density.restype = _d
+        lib.oracle_set_current_density.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(OFields)]
+        lib.oracle_emit_electrons_from_cathode.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(OBeam), P(OFields),
+                                                           ctypes.c_char_p, ctypes.c_int64]
+        lib.oracle_cathode_state.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, P(ctypes.c_int64)]
         _lib = lib
     return _lib
 
@@ -172,6 +221,63 @@ class OracleSim:
         self.b = b
         self.pc = None
         self.last_field_factor = ctypes.c_double(-1.0)
+        self.cathodes = None   # oracle cathode set, created by define_cathode
+        self.n_cathodes = 0
+
+    def __del__(self):
+        if getattr(self, "cathodes", None):
+            self.lib.oracle_cathodes_destroy(self.cathodes)
+            self.cathodes = None
+
+    # ---- cathodes (src/cathode.c) ----
+    def define_cathode(self, profile=None, **namelist):
+        from spiffe_amd import constants as K
+        d = OCathodeDesc()
+        # defaults of src/cathode.c:32-40
+        d.stiffness, d.zoned_emission, d.random_number_seed, d.field_emission_beta = 1.0, 1, 987654321, 1.0
+        d.e_mks, d.me_mks, d.k_boltzmann_mks = K.e_mks, K.me_mks, K.k_boltzmann_mks
+        for key, val in namelist.items():
+            setattr(d, key, val)
+        keep = []
+        if profile is not None:
+            t = np.ascontiguousarray(profile[0], dtype=np.float64)
+            a = np.ascontiguousarray(profile[1], dtype=np.float64)
+            keep = [t, a]
+            d.n_profile = len(t)
+            d.profile_time, d.profile_factor = t.ctypes.data_as(_dp), a.ctypes.data_as(_dp)
+        if self.cathodes is None:
+            self.cathodes = self.lib.oracle_cathodes_create()
+        first, count = ctypes.c_int32(), ctypes.c_int32()
+        err = ctypes.create_string_buffer(256)
+        rc = self.lib.oracle_cathode_define(self.cathodes, ctypes.byref(self.f), ctypes.byref(d), ctypes.byref(first),
+                                            ctypes.byref(count), err, 256)
+        del keep
+        if rc:
+            raise ValueError(err.value.decode())
+        self.n_cathodes = first.value + count.value
+        return list(range(first.value, first.value + count.value))
+
+    def emit_electrons_from_cathode(self, k):
+        err = ctypes.create_string_buffer(256)
+        n0 = self.b.np
+        rc = self.lib.oracle_emit_electrons_from_cathode(self.cathodes, k, ctypes.byref(self.b), ctypes.byref(self.f),
+                                                         err, 256)
+        if rc:
+            raise ValueError(err.value.decode())
+        return self.b.np - n0
+
+    def set_current_density(self, k):
+        self.lib.oracle_set_current_density(self.cathodes, k, ctypes.byref(self.f))
+
+    def cathode_state(self, k):
+        d8 = (ctypes.c_double * 8)()
+        i5 = (ctypes.c_int64 * 5)()
+        self.lib.oracle_cathode_state(self.cathodes, k, d8, i5)
+        names = ["current_density", "ref_current_density", "number_per_step", "electrons_per_macroparticle", "Q",
+                 "temperature", "sigma_v", "initial_vz"]
+        out = {n: d8[i] for i, n in enumerate(names)}
+        out.update(start_step=i5[0], stop_step=i5[1], n_bins=i5[2], step_emission_started=i5[3])
+        return out
 
     # ---- state ----
     def set_field(self, name, a):
@@ -264,9 +370,11 @@ class OracleSim:
                                                ctypes.byref(self.last_field_factor))
 
     def step(self, lines=(), screens=(), compat=True):
-        """Loop body of src/integrate.c:134-194 (after emission), same order as the device's spf_step."""
+        """Loop body of src/integrate.c:134-194, emission included, in the order of the device's spf_step."""
         f = self.f
         f.time = f.time_step * f.dti + f.start_time
+        for k in range(self.n_cathodes):
+            self.emit_electrons_from_cathode(k)
         self.zero_current_arrays()
         if f.modes & 4:
             self.add_particle_currents()
@@ -276,5 +384,7 @@ class OracleSim:
             self.add_poisson_correction((f.modes & 4) != 0)
         self.advance_field_solutions()
         rows, lost = self.advance_particles(screens, compat)
+        for k in range(self.n_cathodes):
+            self.set_current_density(k)
         f.time_step += 1
         return rows, lost

@@ -126,6 +126,55 @@ typedef struct {
   double Qreal, Qerror, Qsurface;
 } opoisson;
 
+/* define_cathode namelist, src/cathode.nl (the profile file as two arrays) */
+typedef struct {
+  double z_position, inner_radius, outer_radius;
+  double current_density, temperature, work_function;
+  int32_t field_emission, add_thermal_velocities;
+  double field_emission_beta;
+  int32_t determine_temperature, autophase;
+  double electrons_per_macroparticle, number_per_step;
+  double start_time, stop_time, time_offset;
+  double initial_pz, initial_omega, stiffness;
+  int32_t discretize_radii, spread_over_dt, zoned_emission;
+  int32_t distribution_correction_interval;
+  int32_t halton_radix_dt, halton_radix_r;
+  int64_t random_number_seed;
+  int64_t n_profile;
+  const double *profile_time, *profile_factor;
+  double e_mks, me_mks, k_boltzmann_mks;
+} ocathode_desc;
+
+/* the uniform stream standing in for random_1 (Philox4x64-10) */
+typedef struct {
+  uint64_t key0, block;
+  uint64_t buf[4];
+  int32_t pos, has_saved;
+  double saved;
+} orng;
+
+typedef struct ocathode_set ocathode_set;
+
+void oracle_philox4x64_10(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
+void oracle_rng_seed(orng *g, uint64_t seed);
+uint64_t oracle_rng_next64(orng *g);
+double oracle_rng_uniform(orng *g);
+double oracle_rng_gauss(orng *g);
+double oracle_radical_inverse(uint64_t k, int32_t base);
+ocathode_set *oracle_cathodes_create(void);
+void oracle_cathodes_destroy(ocathode_set *s);
+orng *oracle_cathodes_rng(ocathode_set *s);
+int64_t oracle_cathodes_next_id(ocathode_set *s);
+int oracle_cathode_define(ocathode_set *s, const ofields *f, const ocathode_desc *d, int32_t *first,
+                          int32_t *count, char *err, int64_t errlen);
+double oracle_field_emission_current_density(double E, double beta, double wf);
+void oracle_set_current_density(ocathode_set *s, int32_t ic, const ofields *f);
+int oracle_emit_electrons_from_cathode(ocathode_set *s, int32_t ic, obeam *b, const ofields *f, char *err,
+                                       int64_t errlen);
+/* d8: J, J_ref, number_per_step, electrons_per_macroparticle, Q, T, sigmaV, initial_vz;
+ * i5: start_step, stop_step, n_bins, step_emission_started, number of cathodes */
+int oracle_cathode_state(ocathode_set *s, int32_t ic, double *d8, int64_t *i5);
+
 void oracle_calculate_flags(int16_t *metal_flags, int32_t nz, int32_t nr);
 void oracle_zero_current_arrays(ofields *f);
 void oracle_multiply_current_arrays(ofields *f, double factor);

@@ -89,6 +89,36 @@ class PoissonParams(ctypes.Structure):
                 ("maximum_iterations", ctypes.c_int64), ("guess_code", ctypes.c_int64)]
 
 
+# define_cathode namelist fields in the order of spf_cathode_desc
+CATHODE_FIELDS = [
+    ("z_position", ctypes.c_double), ("inner_radius", ctypes.c_double), ("outer_radius", ctypes.c_double),
+    ("current_density", ctypes.c_double), ("temperature", ctypes.c_double), ("work_function", ctypes.c_double),
+    ("field_emission", ctypes.c_int32), ("add_thermal_velocities", ctypes.c_int32),
+    ("field_emission_beta", ctypes.c_double),
+    ("determine_temperature", ctypes.c_int32), ("autophase", ctypes.c_int32),
+    ("electrons_per_macroparticle", ctypes.c_double), ("number_per_step", ctypes.c_double),
+    ("start_time", ctypes.c_double), ("stop_time", ctypes.c_double), ("time_offset", ctypes.c_double),
+    ("initial_pz", ctypes.c_double), ("initial_omega", ctypes.c_double), ("stiffness", ctypes.c_double),
+    ("discretize_radii", ctypes.c_int32), ("spread_over_dt", ctypes.c_int32), ("zoned_emission", ctypes.c_int32),
+    ("distribution_correction_interval", ctypes.c_int32),
+    ("halton_radix_dt", ctypes.c_int32), ("halton_radix_r", ctypes.c_int32),
+    ("random_number_seed", ctypes.c_int64),
+    ("n_profile", ctypes.c_int64), ("profile_time", _dp), ("profile_factor", _dp),
+    ("e_mks", ctypes.c_double), ("me_mks", ctypes.c_double), ("k_boltzmann_mks", ctypes.c_double),
+]
+
+
+class CathodeDesc(ctypes.Structure):
+    _fields_ = CATHODE_FIELDS
+
+
+class CathodeState(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ["current_density", "ref_current_density", "number_per_step",
+                                                "electrons_per_macroparticle", "Q", "temperature", "sigma_v",
+                                                "inner_radius", "outer_radius", "initial_vz"]] + \
+               [(n, ctypes.c_int64) for n in ["start_step", "stop_step", "n_bins", "n_emitted", "last_count"]]
+
+
 # every entry point declared in include/spiffe_amd.h: name -> (restype, argtypes)
 _vp = ctypes.c_void_p
 _i32, _i64, _d = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
@@ -125,6 +155,12 @@ SIGNATURES = {
     "spf_timing_enable": (ctypes.c_int, [_vp, _i32]),
     "spf_timing_read": (ctypes.c_int, [_vp, _dp, _pi64, _pi64]),
     "spf_step": (ctypes.c_int, [_vp, _i32, _i32, _pi32, _pi64, _pi64, _pi64, _dp]),
+    "spf_cathode_defaults": (None, [ctypes.POINTER(CathodeDesc)]),
+    "spf_cathode_define": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(CathodeDesc), _pi32, _pi32]),
+    "spf_cathode_set_current_density": (ctypes.c_int, [_vp, _i32, _i32]),
+    "spf_cathode_emit": (ctypes.c_int, [_vp, _i32, _i32, _pi64]),
+    "spf_cathode_state_get": (ctypes.c_int, [_vp, _i32, _i32, ctypes.POINTER(CathodeState)]),
+    "spf_particle_id_next": (ctypes.c_int, [_vp, _i32, _i64, _pi64]),
 }
 
 _lib = None

@@ -12,13 +12,21 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libspiffe_amd.so")
 SOURCES = ["spf_api.hip", "spf_api_poisson.hip", "spf_fdtd.hip", "spf_push.hip", "spf_deposit.hip",
-           "spf_poisson.hip", "spf_util.hip"]
+           "spf_poisson.hip", "spf_util.hip", "spf_cathode.hip"]
+# host-only C++ (no HIP), built with the system C++ compiler
+HOST_SOURCES = ["spf_emit.cpp"]
 
 # -ffp-contract=off: no FMA contraction, each FP64 operation rounds as written;
 # -packed-fp32-ops: no packed FP32 instructions are generated.
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-result",
             "-Wno-unused-function"]
+
+
+# host C++: no FMA contraction, and sin/cos stay separate libm calls (no
+# sincos fusion) so the emission draws are the libm results of each call
+CXXFLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fno-builtin-sin",
+            "-fno-builtin-cos", "-Wall"]
 
 
 def _hipcc():
@@ -30,17 +38,21 @@ def _hipcc():
 
 def _compile(src):
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [_hipcc()] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    if src.endswith(".cpp"):
+        cmd = [os.environ.get("CXX", "g++")] + CXXFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    else:
+        cmd = [_hipcc()] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, r.stdout, r.stderr))
+        raise RuntimeError("compile failed for %s:\n%s\n%s" % (src, r.stdout, r.stderr))
     return obj
 
 
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
-    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+    srcs = SOURCES + HOST_SOURCES
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
     cmd = [_hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -41,7 +41,7 @@ void free_run(Run *R) {
   for (int k = 0; k < 7; k++) F(R->ev[k]);
   F(R->last_old); F(R->evlist); F(R->move_src); F(R->move_dst); F(R->srows); F(R->lrows);
   F(R->keys); F(R->skeys); F(R->vals); F(R->cz); F(R->scz); F(R->gstart); F(R->long_list); F(R->pcount); F(R->gflag); F(R->sort_tmp);
-  F(R->bcount); F(R->boff); F(R->stage); F(R->append_stage); F(R->Pdev); F(R->omega); F(R->red);
+  F(R->bcount); F(R->boff); F(R->stage); F(R->append_stage); F(R->emit_stage); F(R->Pdev); F(R->omega); F(R->red);
   F(R->red_counter); F(R->sor_state); F(R->sor_sums);
   R->live = false;
 }

@@ -248,6 +248,8 @@ int spf_step(spf_ctx *c, int32_t run, int32_t n_lines, const int32_t *line_dir, 
   // src/integrate.c:136-138: the ramped imposed field uses the previous step's time
   if (R->ramp_time > 0 && (rc = spf_imposed_efield(c, run, 0, R->ramp_time, R->flat_top_time))) return rc;
   R->time = R->time_step * R->d.dt + R->d.start_time;
+  for (int32_t k = 0; k < (int32_t)R->cathodes.size(); k++)   // src/integrate.c:162-163
+    if ((rc = cathode_emit_step(c, run, k, nullptr))) return rc;
   if ((rc = spf_currents_reset(c, run))) return rc;
   if ((R->d.modes & SPF_SPACE_CHARGE) && (rc = spf_currents_deposit(c, run))) return rc;
   for (int32_t k = 0; k < n_lines; k++)
@@ -256,6 +258,8 @@ int spf_step(spf_ctx *c, int32_t run, int32_t n_lines, const int32_t *line_dir, 
   if (R->pc_set && (rc = spf_poisson_correct(c, run, (R->d.modes & SPF_SPACE_CHARGE) != 0, nullptr))) return rc;
   if ((rc = spf_fields_advance(c, run))) return rc;
   if ((rc = spf_particles_push(c, run, -1))) return rc;
+  for (int32_t k = 0; k < (int32_t)R->cathodes.size(); k++)   // src/integrate.c:190-191
+    if ((rc = cathode_current_density(c, run, k))) return rc;
   R->time_step++;
   return SPF_OK;
 }

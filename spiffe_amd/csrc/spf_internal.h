@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/spiffe_amd.h"
+#include "spf_emit.h"
 
 namespace spf {
 
@@ -162,6 +163,12 @@ struct Run {
   std::vector<double> screen_z;
   std::vector<int> screen_dir;
   double ramp_time = 0, flat_top_time = 0;
+  // cathode emission: host-side state, one uniform stream per run
+  EmitRng emit_rng;
+  std::vector<Cathode> cathodes;
+  int64_t next_pid = 1;         // nextParticleID, src/spiffe.c:191
+  double *emit_stage = nullptr; // per-particle draws of one emission call
+  size_t emit_cap = 0;
 };
 
 // ---- launchers implemented in the .hip files ----
@@ -179,6 +186,12 @@ hipError_t launch_rows(const RowsArgs &a, hipStream_t s);
 size_t append_stage_bytes(int64_t n);
 hipError_t launch_append_kernels(Run &R, const void *stage, int64_t n, int64_t n_active, double *const *Pdev,
                                  hipStream_t s);
+// np += n, npActive += n_active after n particles were written past np
+hipError_t launch_append_commit(Run &R, int64_t n, int64_t n_active, hipStream_t s);
+// cathode emission (spf_cathode.hip): one cathode's emission at the current step,
+// and its current-density update
+int cathode_emit_step(spf_ctx *c, int32_t run, int32_t cathode, int64_t *n_emitted);
+int cathode_current_density(spf_ctx *c, int32_t run, int32_t cathode);
 // ordered compaction: out[k] = k-th index i < n with flags[i] != 0; *count = total.
 hipError_t ordered_compact(const uint8_t *flags, int64_t n, int32_t *out, int64_t *count, int32_t *bcount,
                            int32_t *boff, hipStream_t s);

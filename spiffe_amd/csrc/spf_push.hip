@@ -770,4 +770,9 @@ hipError_t launch_append_kernels(Run &R, const void *stage, int64_t n, int64_t n
   return hipGetLastError();
 }
 
+hipError_t launch_append_commit(Run &R, int64_t n, int64_t n_active, hipStream_t s) {
+  hipLaunchKernelGGL(append_commit_kernel, dim3(1), dim3(1), 0, s, R.counters, n, n_active, R.cap, R.overflow);
+  return hipGetLastError();
+}
+
 }  // namespace spf

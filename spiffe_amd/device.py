@@ -224,6 +224,52 @@ class DeviceRun:
                                                  ctypes.byref(nl_)))
         return rows[:nr_.value], lost[:nl_.value]
 
+    # ---- cathode emission (src/cathode.c) ----
+    def define_cathode(self, profile=None, **namelist):
+        """process_cathode_definition: `namelist` holds define_cathode values
+        (src/cathode.nl); `profile` is (time, factor) arrays.  Returns the
+        indices of the cathodes added."""
+        d = abi.CathodeDesc()
+        self.lib.spf_cathode_defaults(ctypes.byref(d))
+        d.e_mks, d.me_mks, d.k_boltzmann_mks = K.e_mks, K.me_mks, K.k_boltzmann_mks
+        names = {n for n, _ in abi.CATHODE_FIELDS}
+        for key, val in namelist.items():
+            if key not in names or key in ("n_profile", "profile_time", "profile_factor"):
+                raise ValueError("unknown define_cathode parameter %r" % key)
+            setattr(d, key, val)
+        keep = []
+        if profile is not None:
+            t = np.ascontiguousarray(profile[0], dtype=np.float64)
+            a = np.ascontiguousarray(profile[1], dtype=np.float64)
+            if t.shape != a.shape or t.ndim != 1:
+                raise ValueError("profile time and factor must be 1-D arrays of one length")
+            keep = [t, a]
+            d.n_profile = len(t)
+            d.profile_time, d.profile_factor = _ptr(t, ctypes.c_double), _ptr(a, ctypes.c_double)
+        first, count = ctypes.c_int32(), ctypes.c_int32()
+        self.ctx.check(self.lib.spf_cathode_define(self.ctx.h, self.id, ctypes.byref(d), ctypes.byref(first),
+                                                   ctypes.byref(count)))
+        del keep
+        return list(range(first.value, first.value + count.value))
+
+    def emit_electrons_from_cathode(self, cathode):
+        n = ctypes.c_int64()
+        self.ctx.check(self.lib.spf_cathode_emit(self.ctx.h, self.id, int(cathode), ctypes.byref(n)))
+        return n.value
+
+    def set_current_density(self, cathode):
+        self.ctx.check(self.lib.spf_cathode_set_current_density(self.ctx.h, self.id, int(cathode)))
+
+    def cathode_state(self, cathode):
+        st = abi.CathodeState()
+        self.ctx.check(self.lib.spf_cathode_state_get(self.ctx.h, self.id, int(cathode), ctypes.byref(st)))
+        return {n: getattr(st, n) for n, _ in abi.CathodeState._fields_}
+
+    def next_particle_id(self, set_next=0):
+        v = ctypes.c_int64()
+        self.ctx.check(self.lib.spf_particle_id_next(self.ctx.h, self.id, int(set_next), ctypes.byref(v)))
+        return v.value
+
     # ---- the reference's per-step operations ----
     def advance_field_solutions(self):
         self.ctx.check(self.lib.spf_fields_advance(self.ctx.h, self.id))
@@ -270,7 +316,7 @@ class DeviceRun:
                                                    float(flat_top_time)))
 
     def step(self, lines=()):
-        """One loop body of src/integrate.c:134-194 (after emission)."""
+        """One loop body of src/integrate.c:134-194, emission from the run's cathodes included."""
         n = len(lines)
         d = np.ascontiguousarray([l[0] for l in lines], dtype=np.int32)
         pos = np.ascontiguousarray([l[1] for l in lines], dtype=np.int64)
