@@ -205,6 +205,37 @@ int spf_poisson_sweeps(spf_ctx *ctx, int32_t run, int32_t which, int64_t n0, int
 /* replaces compute_imposed_Efield, src/poisson_correction.c:456-562 */
 int spf_imposed_efield(spf_ctx *ctx, int32_t run, int32_t initialize, double ramp_time, double flat_top_time);
 
+/* ---- geometry: boundary file, drawing and metal flags ----
+ * What process_geometry_definition (src/geometry.c:21-369) and the drawing
+ * routines of src/grid_drawing.c leave in metal_flags, PsiImposed and
+ * materialID.  Host-only: no context and no GPU; a nonzero return comes with
+ * its message in err (err_len bytes, may be 0). */
+typedef struct {
+  int32_t nt;                     /* 1 line, 2 arc, 3 start of a new shape, 4 vacuum point */
+  double x, y, x0, y0, r;         /* scaled by zr_factor; x, y are DBL_MAX*zr_factor when not given */
+  double theta;                   /* end angle of an arc in radians (DBL_MAX when not given) */
+  double a, b;                    /* ellipse semi-axes (nt=2, r=0) */
+  double potential;
+  int16_t material_id, ramp_potential;
+} spf_boundary_point;
+
+/* The &po / &point namelists of a boundary file (the text of the file named by
+ * define_geometry's `boundary`), src/geometry.c:133-209: per-point defaults,
+ * degrees to radians, zr_factor, arc checks and end angles.  With points NULL
+ * only the count is returned. */
+int spf_geometry_parse(const char *text, double zr_factor, spf_boundary_point *points, int64_t max_points,
+                       int64_t *n_points, char *err, int64_t err_len);
+/* Draws the boundary, fills vacuum from the axis (or the nt=4 point), trims the
+ * grid edges and sets the field flags and FL_LAST_ACTIVE_PT
+ * (src/geometry.c:211-362).  flags: (nz+1)*(nr+1) int16 in [iz][ir] order;
+ * psi_imposed (boundary potentials) and material may be NULL. */
+int spf_geometry_draw(int32_t nz, int32_t nr, double zmin, double zmax, double rmax,
+                      const spf_boundary_point *points, int64_t n_points, int16_t *flags, double *psi_imposed,
+                      int16_t *material, char *err, int64_t err_len);
+/* Adds the field flags and FL_LAST_ACTIVE_PT (src/geometry.c:317-362) to an
+ * array holding only FL_IS_SURFACE / FL_IS_INSIDE marks, in place. */
+int spf_geometry_flags(int32_t nz, int32_t nr, int16_t *flags);
+
 /* ---- cathode emission, src/cathode.c ----
  * The define_cathode namelist (src/cathode.nl).  spf_cathode_defaults() fills
  * the defaults of src/cathode.c:32-40.  The emission profile arrives as two

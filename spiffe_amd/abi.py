@@ -89,6 +89,12 @@ class PoissonParams(ctypes.Structure):
                 ("maximum_iterations", ctypes.c_int64), ("guess_code", ctypes.c_int64)]
 
 
+class BoundaryPoint(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int32)] + [(n, ctypes.c_double) for n in
+                                           ["x", "y", "x0", "y0", "r", "theta", "a", "b", "potential"]] + \
+               [("material_id", ctypes.c_int16), ("ramp_potential", ctypes.c_int16)]
+
+
 # define_cathode namelist fields in the order of spf_cathode_desc
 CATHODE_FIELDS = [
     ("z_position", ctypes.c_double), ("inner_radius", ctypes.c_double), ("outer_radius", ctypes.c_double),
@@ -155,6 +161,12 @@ SIGNATURES = {
     "spf_timing_enable": (ctypes.c_int, [_vp, _i32]),
     "spf_timing_read": (ctypes.c_int, [_vp, _dp, _pi64, _pi64]),
     "spf_step": (ctypes.c_int, [_vp, _i32, _i32, _pi32, _pi64, _pi64, _pi64, _dp]),
+    "spf_geometry_parse": (ctypes.c_int, [ctypes.c_char_p, _d, ctypes.POINTER(BoundaryPoint), _i64, _pi64,
+                                          ctypes.c_char_p, _i64]),
+    "spf_geometry_draw": (ctypes.c_int, [_i32, _i32, _d, _d, _d, ctypes.POINTER(BoundaryPoint), _i64,
+                                         ctypes.POINTER(ctypes.c_int16), _dp, ctypes.POINTER(ctypes.c_int16),
+                                         ctypes.c_char_p, _i64]),
+    "spf_geometry_flags": (ctypes.c_int, [_i32, _i32, ctypes.POINTER(ctypes.c_int16)]),
     "spf_cathode_defaults": (None, [ctypes.POINTER(CathodeDesc)]),
     "spf_cathode_define": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(CathodeDesc), _pi32, _pi32]),
     "spf_cathode_set_current_density": (ctypes.c_int, [_vp, _i32, _i32]),

@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libspiffe_amd.so")
 SOURCES = ["spf_api.hip", "spf_api_poisson.hip", "spf_fdtd.hip", "spf_push.hip", "spf_deposit.hip",
            "spf_poisson.hip", "spf_util.hip", "spf_cathode.hip"]
 # host-only C++ (no HIP), built with the system C++ compiler
-HOST_SOURCES = ["spf_emit.cpp"]
+HOST_SOURCES = ["spf_emit.cpp", "spf_namelist.cpp", "spf_geometry.cpp"]
 
 # -ffp-contract=off: no FMA contraction, each FP64 operation rounds as written;
 # -packed-fp32-ops: no packed FP32 instructions are generated.

@@ -10,15 +10,20 @@ uniform 1 mm-radius slug just off the cathode: the shapes and sizes of the
 configuration with synthetic contents (no eigenmode solve, no emission
 history).
 """
+import os
+
 import numpy as np
 
 from . import abi
 from . import constants as K
 from .device import Grid, courant_dt
-from .geometry import derive_flags, polygon_marks
+from .geometry import boundary_file
 
+DECKS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "decks")
+GUN_GEO = os.path.join(DECKS, "gun_1p6cell.geo")
+DC_GUN_GEO = os.path.join(DECKS, "dc_gun.geo")
 GUN_POLY_16 = [(0.0, 0.0), (0.0, 0.042), (0.0325, 0.042), (0.0325, 0.0125), (0.0375, 0.0125), (0.0375, 0.042),
-               (0.105, 0.042), (0.105, 0.0), (0.0, 0.0)]
+               (0.105, 0.042), (0.105, 0.0), (0.0, 0.0)]   # the outline of gun_1p6cell.geo
 
 
 def gun_grid(dz=1e-4, modes=abi.TM_FIELDS | abi.TE_FIELDS | abi.RADIAL_INTERP | abi.LONGIT_INTERP |
@@ -32,7 +37,24 @@ def gun_grid(dz=1e-4, modes=abi.TM_FIELDS | abi.TE_FIELDS | abi.RADIAL_INTERP | 
 
 
 def gun_flags(grid):
-    return derive_flags(polygon_marks(grid.nz, grid.nr, grid.zmin, grid.zmax, grid.rmax, GUN_POLY_16))
+    """Flags of the 1.6-cell gun boundary file drawn on `grid`."""
+    return boundary_file(GUN_GEO, grid.nz, grid.nr, grid.zmin, grid.zmax, grid.rmax)[0]
+
+
+def dc_gun_grid(dz=5e-5, modes=abi.TM_FIELDS | abi.RADIAL_INTERP | abi.LONGIT_INTERP | abi.SPACE_CHARGE):
+    """The DC-gun request of the serving stream: 0.05 mm mesh over 3 cm x 1.2 cm."""
+    zmax, rmax = 0.030, 0.012
+    nz = int(round(zmax / dz)) + 1
+    nr = int(round(rmax / dz)) + 1
+    g = Grid(nz=nz, nr=nr, zmin=0.0, zmax=zmax, rmax=rmax, dt=0.0, modes=modes)
+    g.dt = courant_dt(g.dz, g.dr, 0.9)
+    return g
+
+
+def dc_gun_geometry(grid):
+    """(flags, PsiImposed boundary potentials) of the DC-gun boundary file on `grid`."""
+    flags, psi, _ = boundary_file(DC_GUN_GEO, grid.nz, grid.nr, grid.zmin, grid.zmax, grid.rmax)
+    return flags, psi
 
 
 def gun_fields(grid, E0=120e6):
