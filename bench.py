@@ -77,6 +77,7 @@ def build_run(ctx, n_particles):
     run = DeviceRun(ctx, g, f, max_particles=n_particles, screens=W.GUN_SCREENS, compat=False)
     for k, v in W.gun_fields(g).items():
         run.upload(k, v)
+    run.define_solenoid(**W.GUN_SOLENOID)
     run.append(W.gun_bunch(g, n_particles))
     run.setup_poisson_correction(**W.GUN_POISSON)
     return g, f, run
@@ -89,6 +90,7 @@ def cpu_baseline(g, f, n_particles, budget_s):
     o = OracleSim(g, f, max_particles=n_particles)
     for k, v in W.gun_fields(g).items():
         o.set_field(k, v)
+    o.define_solenoid(**W.GUN_SOLENOID)
     o.append(W.gun_bunch(g, n_particles))
     o.setup_poisson_correction(**W.GUN_POISSON)
     steps, pushes = 0, 0

@@ -236,6 +236,30 @@ int spf_geometry_draw(int32_t nz, int32_t nr, double zmin, double zmax, double r
  * array holding only FL_IS_SURFACE / FL_IS_INSIDE marks, in place. */
 int spf_geometry_flags(int32_t nz, int32_t nr, int16_t *flags);
 
+/* ---- imposed solenoid fields, src/solenoid.c:18-127, src/Bsolenoid.c:17-86 ----
+ * The define_solenoid namelist (src/solenoid.nl).  `turns` equispaced current
+ * rings from z_start to z_end (mirrored about z = 0 for symmetry +-1) add
+ * current x their field to BzImposed/BrImposed at every grid point with
+ * ir <= evaluation_radius_limit/dr; Bz_peak rescales the accumulated field to
+ * that on-axis maximum; bucking sets the current that cancels the on-axis
+ * field already present at z_buck.  Extension (no counterpart in the
+ * reference): gaussian != 0 adds instead an on-axis Gaussian of peak Bz_peak
+ * centred at gaussian_center with rms gaussian_sigma, and its first-order
+ * off-axis expansion Br = -(r/2) dBz/dz.  The run's gather uses the imposed
+ * magnetic field from then on.  Synchronizing. */
+typedef struct {
+  double radius, evaluation_radius_limit, z_start, z_end, current, Bz_peak;
+  int64_t turns, symmetry;
+  int32_t bucking;
+  double z_buck;
+  int32_t gaussian;
+  double gaussian_center, gaussian_sigma;
+} spf_solenoid_desc;
+
+void spf_solenoid_defaults(spf_solenoid_desc *desc);
+/* current_used (may be NULL) gets the ring current applied (the bucking one if set) */
+int spf_solenoid_add(spf_ctx *ctx, int32_t run, const spf_solenoid_desc *desc, double *current_used);
+
 /* ---- cathode emission, src/cathode.c ----
  * The define_cathode namelist (src/cathode.nl).  spf_cathode_defaults() fills
  * the defaults of src/cathode.c:32-40.  The emission profile arrives as two

@@ -88,6 +88,12 @@ class OCathodeDesc(ctypes.Structure):
         ("e_mks", _d), ("me_mks", _d), ("k_boltzmann_mks", _d)]
 
 
+class OSolenoidDesc(ctypes.Structure):
+    _fields_ = [(n, _d) for n in ["radius", "evaluation_radius_limit", "z_start", "z_end", "current", "Bz_peak"]] + \
+               [("turns", ctypes.c_int64), ("symmetry", ctypes.c_int64), ("bucking", ctypes.c_int32),
+                ("z_buck", _d), ("gaussian", ctypes.c_int32), ("gaussian_center", _d), ("gaussian_sigma", _d)]
+
+
 class ORng(ctypes.Structure):
     _fields_ = [("key0", ctypes.c_uint64), ("block", ctypes.c_uint64), ("buf", ctypes.c_uint64 * 4),
                 ("pos", ctypes.c_int32), ("has_saved", ctypes.c_int32), ("saved", _d)]
@@ -165,6 +171,10 @@ def load():
         lib.oracle_emit_electrons_from_cathode.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(OBeam), P(OFields),
                                                            ctypes.c_char_p, ctypes.c_int64]
         lib.oracle_cathode_state.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, P(ctypes.c_int64)]
+        lib.oracle_ellip_ke.argtypes = [_d, _dp, _dp]
+        lib.oracle_b_ring.argtypes = [_dp, _dp, _d, _d, _d]
+        lib.oracle_b_solenoid.argtypes = [_dp, _dp, _d, _d, _d, _d, _d, ctypes.c_int64, ctypes.c_int64]
+        lib.oracle_define_solenoid.argtypes = [P(OFields), P(OSolenoidDesc), _dp, ctypes.c_char_p, ctypes.c_int64]
         _lib = lib
     return _lib
 
@@ -228,6 +238,21 @@ class OracleSim:
         if getattr(self, "cathodes", None):
             self.lib.oracle_cathodes_destroy(self.cathodes)
             self.cathodes = None
+
+    # ---- imposed solenoid field (src/solenoid.c) ----
+    def define_solenoid(self, **namelist):
+        d = OSolenoidDesc()
+        d.turns = 1
+        for key, val in namelist.items():
+            setattr(d, key, val)
+        for n in ("BzImposed", "BrImposed"):      # allocated by the first solenoid, src/solenoid.c:51-56
+            if not getattr(self.f, n):
+                setattr(self.f, n, self.arr[n].ctypes.data_as(_dp))
+        cur = ctypes.c_double()
+        err = ctypes.create_string_buffer(256)
+        if self.lib.oracle_define_solenoid(ctypes.byref(self.f), ctypes.byref(d), ctypes.byref(cur), err, 256):
+            raise ValueError(err.value.decode())
+        return cur.value
 
     # ---- cathodes (src/cathode.c) ----
     def define_cathode(self, profile=None, **namelist):

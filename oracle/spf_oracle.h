@@ -126,6 +126,22 @@ typedef struct {
   double Qreal, Qerror, Qsurface;
 } opoisson;
 
+/* define_solenoid namelist, src/solenoid.nl (+ the Gaussian-profile extension) */
+typedef struct {
+  double radius, evaluation_radius_limit, z_start, z_end, current, Bz_peak;
+  int64_t turns, symmetry;
+  int32_t bucking;
+  double z_buck;
+  int32_t gaussian;
+  double gaussian_center, gaussian_sigma;
+} osolenoid_desc;
+
+void oracle_ellip_ke(double k, double *K, double *E);
+void oracle_b_ring(double *B_rho, double *B_z, double rho, double z, double radius);
+void oracle_b_solenoid(double *B_rho, double *B_z, double rho, double z, double radius, double zStart, double zEnd,
+                       int64_t coils, int64_t symmetry);
+int oracle_define_solenoid(ofields *f, const osolenoid_desc *d, double *current_used, char *err, int64_t errlen);
+
 /* define_cathode namelist, src/cathode.nl (the profile file as two arrays) */
 typedef struct {
   double z_position, inner_radius, outer_radius;

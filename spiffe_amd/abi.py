@@ -95,6 +95,14 @@ class BoundaryPoint(ctypes.Structure):
                [("material_id", ctypes.c_int16), ("ramp_potential", ctypes.c_int16)]
 
 
+class SolenoidDesc(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ["radius", "evaluation_radius_limit", "z_start", "z_end", "current",
+                                                "Bz_peak"]] + \
+               [("turns", ctypes.c_int64), ("symmetry", ctypes.c_int64), ("bucking", ctypes.c_int32),
+                ("z_buck", ctypes.c_double), ("gaussian", ctypes.c_int32), ("gaussian_center", ctypes.c_double),
+                ("gaussian_sigma", ctypes.c_double)]
+
+
 # define_cathode namelist fields in the order of spf_cathode_desc
 CATHODE_FIELDS = [
     ("z_position", ctypes.c_double), ("inner_radius", ctypes.c_double), ("outer_radius", ctypes.c_double),
@@ -167,6 +175,8 @@ SIGNATURES = {
                                          ctypes.POINTER(ctypes.c_int16), _dp, ctypes.POINTER(ctypes.c_int16),
                                          ctypes.c_char_p, _i64]),
     "spf_geometry_flags": (ctypes.c_int, [_i32, _i32, ctypes.POINTER(ctypes.c_int16)]),
+    "spf_solenoid_defaults": (None, [ctypes.POINTER(SolenoidDesc)]),
+    "spf_solenoid_add": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(SolenoidDesc), _dp]),
     "spf_cathode_defaults": (None, [ctypes.POINTER(CathodeDesc)]),
     "spf_cathode_define": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(CathodeDesc), _pi32, _pi32]),
     "spf_cathode_set_current_density": (ctypes.c_int, [_vp, _i32, _i32]),

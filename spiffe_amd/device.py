@@ -224,6 +224,22 @@ class DeviceRun:
                                                  ctypes.byref(nl_)))
         return rows[:nr_.value], lost[:nl_.value]
 
+    # ---- imposed solenoid field (src/solenoid.c) ----
+    def define_solenoid(self, **namelist):
+        """process_solenoid_definition with define_solenoid values (src/solenoid.nl),
+        plus the gaussian / gaussian_center / gaussian_sigma extension.  Returns the
+        ring current applied."""
+        d = abi.SolenoidDesc()
+        self.lib.spf_solenoid_defaults(ctypes.byref(d))
+        names = {n for n, _ in abi.SolenoidDesc._fields_}
+        for key, val in namelist.items():
+            if key not in names:
+                raise ValueError("unknown define_solenoid parameter %r" % key)
+            setattr(d, key, val)
+        cur = ctypes.c_double()
+        self.ctx.check(self.lib.spf_solenoid_add(self.ctx.h, self.id, ctypes.byref(d), ctypes.byref(cur)))
+        return cur.value
+
     # ---- cathode emission (src/cathode.c) ----
     def define_cathode(self, profile=None, **namelist):
         """process_cathode_definition: `namelist` holds define_cathode values

@@ -3,8 +3,9 @@
 gun_16cell(): the production-size single run -- a 1.6-cell S-band gun
 (half cell 0-3.5 cm, full cell 3.5-10.5 cm, cell radius 4.2 cm, iris radius
 1.25 cm, iris 0.5 cm thick), dr = dz = 0.1 mm (1051 x 421 grid lines),
-TM+TE fields, 1M macroparticles carrying 1 nC, Poisson correction every 10
-steps to 1e-8, a screen at z = 10 cm.  Fields start from a synthetic
+TM+TE fields, an external Gaussian solenoid (0.25 T at z = 20 cm, sigma
+5 cm), 1M macroparticles carrying 1 nC, Poisson correction every 10 steps to
+1e-8, a screen at z = 10 cm.  Fields start from a synthetic
 standing-wave pattern scaled to 120 MV/m on axis and the bunch starts as a
 uniform 1 mm-radius slug just off the cathode: the shapes and sizes of the
 configuration with synthetic contents (no eigenmode solve, no emission
@@ -85,4 +86,6 @@ def gun_bunch(grid, n, charge=1e-9, radius=1e-3, seed=7, z0=0.3e-3, length=1.5e-
 
 
 GUN_SCREENS = [(0.10, 1)]
+# external solenoid: Gaussian on-axis Bz, 0.25 T peak at z = 20 cm, sigma 5 cm
+GUN_SOLENOID = dict(gaussian=1, Bz_peak=0.25, gaussian_center=0.20, gaussian_sigma=0.05)
 GUN_POISSON = dict(step_interval=10, accuracy=1e-8, maximum_iterations=3000)
