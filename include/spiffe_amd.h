@@ -311,6 +311,46 @@ int spf_cathode_state_get(spf_ctx *ctx, int32_t run, int32_t cathode, spf_cathod
  * it; next (may be NULL) gets the value in force */
 int spf_particle_id_next(spf_ctx *ctx, int32_t run, int64_t set_next, int64_t *next);
 
+/* ---- deck runner ----
+ * Runs a spiffe input deck (the command loop of src/spiffe.c:213-296) on the
+ * library: define_geometry (boundary file drawn by spf_geometry_*),
+ * load_fields, define_antenna, define_cathode, define_solenoid, define_screen,
+ * poisson_correction, define_field_saving, set_constant_fields and one
+ * integrate.  Screen, lost-particle and field-saving files are written as
+ * binary SDDS with the reference's columns; relative file names are taken
+ * from the deck's directory, "%s" stands for the rootname.  Diagnostics the
+ * runner does not write (field sampling, snapshots, field output, status
+ * output) are skipped; commands that would change the physics and are not
+ * built (secondary emission, emitters, particle loading, resistors, test
+ * beams, on-axis field maps, translation, smoothing) return SPF_EUNSUPPORTED.
+ * No counterpart as one function in the reference. */
+typedef struct {
+  int32_t device;
+  int32_t compat;          /* as spf_run_desc.compat (spf_deck_defaults: 1) */
+  int32_t verbose;
+  int64_t max_particles;   /* particle slots; 0: sized from the cathodes */
+  int64_t max_steps;       /* > 0: stop integrate after this many steps */
+} spf_deck_options;
+
+typedef struct {
+  int64_t steps, particles, emitted, screen_rows, lost_rows, field_saves, max_particles;
+} spf_deck_stats;
+
+void spf_deck_defaults(spf_deck_options *opt);
+int spf_deck_run(const char *deck_path, const spf_deck_options *opt, spf_deck_stats *stats, char *err,
+                 int64_t err_len);
+
+/* ---- SDDS files ----
+ * Readers for the SDDS1 files spiffe reads and writes (ASCII or binary pages;
+ * double, float, long, short, character and string data).  Host-only. */
+int spf_sdds_info(const char *path, int64_t *n_pages, int64_t *n_rows_first_page, char *err, int64_t err_len);
+/* numeric column `column` of page `page` (0-based) into out[max]; *n = rows */
+int spf_sdds_column(const char *path, int64_t page, const char *column, double *out, int64_t max, int64_t *n,
+                    char *err, int64_t err_len);
+/* numeric parameter of a page (fixed values included) */
+int spf_sdds_parameter(const char *path, int64_t page, const char *name, double *value, char *err,
+                       int64_t err_len);
+
 /* ---- phase timers ----
  * No counterpart in the reference: telemetry for benchmarks and serving loops.
  * When enabled, a pair of HIP events on the context's stream brackets one call

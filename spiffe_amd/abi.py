@@ -103,6 +103,16 @@ class SolenoidDesc(ctypes.Structure):
                 ("gaussian_sigma", ctypes.c_double)]
 
 
+class DeckOptions(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("compat", ctypes.c_int32), ("verbose", ctypes.c_int32),
+                ("max_particles", ctypes.c_int64), ("max_steps", ctypes.c_int64)]
+
+
+class DeckStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ["steps", "particles", "emitted", "screen_rows", "lost_rows",
+                                               "field_saves", "max_particles"]]
+
+
 # define_cathode namelist fields in the order of spf_cathode_desc
 CATHODE_FIELDS = [
     ("z_position", ctypes.c_double), ("inner_radius", ctypes.c_double), ("outer_radius", ctypes.c_double),
@@ -177,6 +187,13 @@ SIGNATURES = {
     "spf_geometry_flags": (ctypes.c_int, [_i32, _i32, ctypes.POINTER(ctypes.c_int16)]),
     "spf_solenoid_defaults": (None, [ctypes.POINTER(SolenoidDesc)]),
     "spf_solenoid_add": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(SolenoidDesc), _dp]),
+    "spf_deck_defaults": (None, [ctypes.POINTER(DeckOptions)]),
+    "spf_deck_run": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(DeckOptions), ctypes.POINTER(DeckStats),
+                                    ctypes.c_char_p, _i64]),
+    "spf_sdds_info": (ctypes.c_int, [ctypes.c_char_p, _pi64, _pi64, ctypes.c_char_p, _i64]),
+    "spf_sdds_column": (ctypes.c_int, [ctypes.c_char_p, _i64, ctypes.c_char_p, _dp, _i64, _pi64, ctypes.c_char_p,
+                                       _i64]),
+    "spf_sdds_parameter": (ctypes.c_int, [ctypes.c_char_p, _i64, ctypes.c_char_p, _dp, ctypes.c_char_p, _i64]),
     "spf_cathode_defaults": (None, [ctypes.POINTER(CathodeDesc)]),
     "spf_cathode_define": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(CathodeDesc), _pi32, _pi32]),
     "spf_cathode_set_current_density": (ctypes.c_int, [_vp, _i32, _i32]),

@@ -14,7 +14,9 @@ LIB = os.path.join(HERE, "libspiffe_amd.so")
 SOURCES = ["spf_api.hip", "spf_api_poisson.hip", "spf_fdtd.hip", "spf_push.hip", "spf_deposit.hip",
            "spf_poisson.hip", "spf_util.hip", "spf_cathode.hip", "spf_solenoid.hip"]
 # host-only C++ (no HIP), built with the system C++ compiler
-HOST_SOURCES = ["spf_emit.cpp", "spf_namelist.cpp", "spf_geometry.cpp"]
+HOST_SOURCES = ["spf_emit.cpp", "spf_namelist.cpp", "spf_geometry.cpp", "spf_sdds.cpp", "spf_deck.cpp"]
+# the command-line deck runner, linked against the library
+EXE = os.path.join(HERE, "spiffe-amd")
 
 # -ffp-contract=off: no FMA contraction, each FP64 operation rounds as written;
 # -packed-fp32-ops: no packed FP32 instructions are generated.
@@ -59,6 +61,12 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    cmd = [os.environ.get("CXX", "g++")] + CXXFLAGS + [os.path.join(CSRC, "spf_main.cpp"), "-o", EXE + ".tmp",
+                                                        "-L" + HERE, "-lspiffe_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("runner link failed:\n%s\n%s" % (r.stdout, r.stderr))
+    os.replace(EXE + ".tmp", EXE)
     if verbose:
         print("built", LIB)
     return LIB
