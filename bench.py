@@ -83,6 +83,45 @@ def build_run(ctx, n_particles):
     return g, f, run
 
 
+def kernels_at_scale(ctx, n_particles, dz, reps):
+    """FDTD and push of the high-resolution single run's size (a 4201 x 1681-line
+    gun at dz = dr = 25 um, 16M particles): device ms per launch from the
+    library's HIP-event timers (every launch timed) and algorithmic bytes."""
+    from spiffe_amd import workloads as W
+    from spiffe_amd.device import DeviceRun
+    g = W.gun_grid(dz=dz)
+    f = W.gun_flags(g)
+    run = DeviceRun(ctx, g, f, max_particles=n_particles, screens=W.GUN_SCREENS, compat=False)
+    for k, v in W.gun_fields(g).items():
+        run.upload(k, v)
+    run.define_solenoid(**W.GUN_SOLENOID)
+    run.append(W.gun_bunch(g, n_particles))
+    for _ in range(2):
+        run.advance_field_solutions()
+        run.advance_particles()
+    ctx.synchronize()
+    np0, _ = run.count()
+    ctx.timing(1)
+    for _ in range(reps):
+        run.advance_field_solutions()
+    for _ in range(reps):
+        run.advance_particles()
+    tm = ctx.timings()
+    ctx.timing(0)
+    np1, _ = run.count()
+    cells = (g.nz - 1) * (g.nr - 1)
+    t_f = tm["fdtd"][0] / tm["fdtd"][1]
+    t_p = tm["push"][0] / tm["push"][1]
+    npk = 0.5 * (np0 + np1)
+    out = {"grid_lines": [g.nz, g.nr], "cells": cells, "particles": npk, "launches": reps,
+           "fdtd_ms": t_f, "fdtd_GBs": cells * B_CELL_TMTE / (t_f * 1e-3) / 1e9,
+           "push_ms": t_p, "push_GBs": npk * B_PUSH / (t_p * 1e-3) / 1e9}
+    out["fdtd_frac"] = out["fdtd_GBs"] / HBM_PEAK_GBS
+    out["push_frac"] = out["push_GBs"] / HBM_PEAK_GBS
+    run.close()
+    return out
+
+
 def cpu_baseline(g, f, n_particles, budget_s):
     """The restatement on one host core, same workload, bounded sample of steps."""
     from oracle.oracle import OracleSim
@@ -116,6 +155,9 @@ def main():
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--cpu-budget", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--at-scale", action="store_true",
+                    help="also time FDTD and push at the high-resolution run's size (4201x1681 lines, 16M particles)")
+    ap.add_argument("--scale-particles", type=int, default=16_000_000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device copy (5 times), the reference for HBM byte counters")
     args = ap.parse_args()
@@ -195,6 +237,12 @@ def main():
     run.set_step(0)
     it = run.add_poisson_correction(True)
 
+    scale = None
+    if args.at_scale:
+        run.close()
+        run = None
+        scale = kernels_at_scale(ctx, args.scale_particles, 25e-6, 20)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(g, f, args.particles, args.cpu_budget)
@@ -204,15 +252,19 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "1.6-cell S-band gun, dr=dz=0.1 mm (1051x421 lines), 1M macroparticles 1 nC, "
-                               "TM+TE, Poisson every 10 steps to 1e-8, screen z=10 cm",
+                               "TM+TE, Gaussian solenoid 0.25 T at z=20 cm, Poisson every 10 steps to 1e-8, "
+                               "screen z=10 cm",
                    "grid_lines": [g.nz, g.nr], "cells": cells, "particles": args.particles, "dt": g.dt},
         "cell_updates_per_s": cells * args.steps / wall,
         "phase_ms": phase, "poisson_iterations": it, "kernel_ms_per_step": share,
         "hbm_frac": {k: v / HBM_PEAK_GBS for k, v in gbs.items()},
         "roofline": rl, "cpu_baseline": cpu,
     }
+    if scale:
+        out["kernels_at_scale"] = scale
     print(json.dumps(out))
-    run.close()
+    if run:
+        run.close()
     ctx.close()
 
 
