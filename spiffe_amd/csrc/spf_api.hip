@@ -149,7 +149,9 @@ int spf_ctx_create(int32_t device, spf_ctx **out) {
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc((void **)&c->pinned, 64 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
-      hipEventCreateWithFlags(&c->stage_done, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->stage_done, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->sor_ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->sor_ev[1], hipEventDisableTiming) != hipSuccess) {
     delete c;
     return SPF_EHIP;
   }
@@ -169,6 +171,8 @@ int spf_ctx_destroy(spf_ctx *c) {
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->hstage) (void)hipHostFree(c->hstage);
   if (c->stage_done) (void)hipEventDestroy(c->stage_done);
+  for (hipEvent_t e : c->sor_ev)
+    if (e) (void)hipEventDestroy(e);
   for (auto &p : c->ev_pending) {
     (void)hipEventDestroy(p.e0);
     (void)hipEventDestroy(p.e1);

@@ -13,8 +13,7 @@ namespace {
 
 constexpr double PI_D = 3.141592653589793238462643;
 constexpr int64_t kSorCheckBatches = 16;   // batches queued between host checks of the done flag
-constexpr int kCheck = 4;                  // batches per stopping-rule check
-constexpr int kRing = kCheck + 1;          // Psi buffers the batches rotate through
+constexpr int kRing = 5;                   // Psi buffers the batches rotate through (> batches per check)
 
 // omega for half-sweep k: src/poisson.c:47-49 and :121-124
 int ensure_omega(spf_ctx *c, Run *R, int64_t n) {
@@ -54,11 +53,15 @@ void adopt_result(Run *R, double **Psi, double *const *ring, int nring, int out)
 // solve_poisson_cyl, src/poisson.c:17-130, on *Psi (a run-owned buffer that
 // may be swapped with one of the run's spare buffers); *iters = n on
 // convergence, -1 otherwise.  Batches rotate through a ring of kRing
-// buffers: batch b reads ring[b % kRing] and writes ring[(b+1) % kRing].  The stopping rule is applied every kCheck batches by
-// one check launch over their half-sweeps; queued batches return at once
-// after a stop, and the batch that contains the stopping half-sweep still has
-// its input in the ring (kRing > kCheck), so it is re-run for exactly the
+// buffers: batch b reads ring[b % kRing] and writes ring[(b+1) % kRing].  The
+// stopping rule is applied every sor_check_group() batches by one check
+// launch over their half-sweeps; queued batches return at once after a stop,
+// and the batch that contains the stopping half-sweep still has its input in
+// the ring (kRing > batches per check), so it is re-run for exactly the
 // half-sweeps needed.
+// Batches are queued in windows of kSorCheckBatches; the host reads a
+// window's done flag while the next window is already queued, so the device
+// never waits for the host between windows.
 int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accuracy, int64_t limit, int64_t *iters) {
   if (accuracy <= 0) return fail(c, SPF_EINVAL, "accuracy is <= 0 in solve_poisson_cyl");
   if (limit <= 1) return fail(c, SPF_EINVAL, "iteration_limit is <= 1 in solve_poisson_cyl");
@@ -68,13 +71,20 @@ int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accura
   HIPCALL(c, launch_sor_reset(*R, s));
   if (rhs)
     HIPCALL(c, launch_sum_abs(rhs, R->ldz, R->d.nz, R->d.nr - 1, R->red, R->red_n, R->sor_sums, s));
-  const int K = sor_batch();
+  const int K = sor_batch(), KC = sor_check_group();
   double *ring[kRing] = {*Psi, R->PsiSpare[0], R->PsiSpare[1], R->PsiSpare[2], R->PsiSpare[3]};
   int64_t nb = (limit + K - 1) / K, b = 0, converged = -1;
-  while (b < nb) {
+  int64_t *flag[2] = {c->pinned + 16, c->pinned + 18};   // [done, iterations] of each window in flight
+  int pending = -1, win = 0;
+  auto collect = [&](int slot) -> int {
+    HIPCALL(c, hipEventSynchronize(c->sor_ev[slot]));
+    if (flag[slot][0]) converged = flag[slot][1];
+    return SPF_OK;
+  };
+  while (b < nb && converged < 0) {
     int64_t win_end = std::min<int64_t>(nb, b + kSorCheckBatches);
     while (b < win_end) {
-      int64_t g0 = b, g1 = std::min<int64_t>(win_end, b + kCheck);
+      int64_t g0 = b, g1 = std::min<int64_t>(win_end, b + KC);
       {
         PhaseTimer bt(c, SPF_T_SOR_BATCH, g1 - b);   // the group's batch launches, back to back
         for (; b < g1; b++) {
@@ -86,13 +96,13 @@ int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accura
       }
       HIPCALL(c, launch_sor_check(*R, g0 * K, (int)(std::min<int64_t>(b * K, limit) - g0 * K), accuracy, s));
     }
-    HIPCALL(c, hipMemcpyAsync(c->pinned, R->sor_state, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    HIPCALL(c, hipStreamSynchronize(s));
-    if (c->pinned[0]) {
-      converged = c->pinned[1];
-      break;
-    }
+    const int slot = win++ & 1;
+    HIPCALL(c, hipMemcpyAsync(flag[slot], R->sor_state, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCALL(c, hipEventRecord(c->sor_ev[slot], s));
+    if (pending >= 0 && (rc = collect(pending))) return rc;
+    pending = slot;
   }
+  if (converged < 0 && pending >= 0 && (rc = collect(pending))) return rc;
   int out;
   if (converged >= 0) {
     int64_t bc = converged / K, n0 = bc * K;

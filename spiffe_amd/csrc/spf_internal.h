@@ -210,6 +210,7 @@ hipError_t launch_sum_plain(const double *a, int ldz, int iz_n, int ir_n, double
                             hipStream_t s);
 hipError_t launch_sor_reset(Run &R, hipStream_t s);
 int64_t sor_tiles(const Run &R);
+int sor_check_group();   // batches per stopping-rule check (<= 4)
 int sor_batch();
 
 hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const double *rhs, int64_t n0, int k,
@@ -230,6 +231,7 @@ struct spf_ctx {
   void *hstage = nullptr;      // pinned staging for appends
   size_t hstage_bytes = 0;
   hipEvent_t stage_done = nullptr;
+  hipEvent_t sor_ev[2] = {nullptr, nullptr};   // ends of the two SOR check windows in flight
   // phase timers (SPF_T_*): event pairs recorded on the stream, resolved lazily
   int64_t timing_every = 0;    // 0: off; N: one call in N of each phase is timed
   int64_t t_calls[SPF_T_N] = {};

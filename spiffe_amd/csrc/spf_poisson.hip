@@ -54,8 +54,16 @@ struct SorCfg {
   static constexpr unsigned COL0 = 0x55555555u & ((1u << (2 * RS)) - 1u);
   static constexpr unsigned COL1 = 0xAAAAAAAAu & ((1u << (2 * RS)) - 1u);
 };
-using BatchCfg = SorCfg<8, 6, 6>;     // 112 x 20 owned points, 6 wavefronts
+// tile shape (half-sweeps per batch, rows per lane, wavefronts), fixed at
+// build time; the default owns 112 x 20 points with 6 wavefronts
+#ifndef SPF_SOR_SHAPE
+#define SPF_SOR_SHAPE 8, 6, 6
+#endif
+using BatchCfg = SorCfg<SPF_SOR_SHAPE>;
 constexpr int CB = 1024;              // check kernel: 32 lanes per half-sweep, 32 half-sweeps
+// batches per stopping-rule check: as many as one check launch covers, at most 4
+constexpr int kCheckGroup = (CB / 32) / BatchCfg::SK < 4 ? (CB / 32) / BatchCfg::SK : 4;
+static_assert(kCheckGroup >= 1, "a check covers at least one batch of half-sweeps");
 
 // fixed-order sum over a workgroup: butterfly within each wavefront, then the
 // wavefront sums in order (the same operations every launch)
@@ -84,11 +92,14 @@ __device__ inline double from_right(double v, double edge) {
   return __hiloint2double(hi, lo);
 }
 
-// x / d, correctly rounded, for |x| in (1e-280, 1e280) and a host-checked
-// y = RN(1/d): q0 = RN(x*y) and two remainder corrections
-// q <- RN(q + RN(x - d*q)*y) give RN(x/d) (Markstein's correction theorem:
-// the first step makes q faithful, the second rounds it correctly; nothing
-// underflows in that range).  Callers divide directly outside it.
+// x / d, correctly rounded, for a host-checked d < 0 with y = RN(1/d), and
+// x = +-0 or |x| in (1e-280, 1e280): q0 = RN(x*y) and two remainder
+// corrections q <- RN(q + RN(x - d*q)*y) give RN(x/d) (Markstein's correction
+// theorem: the first step makes q faithful, the second rounds it correctly;
+// nothing underflows in that range).  For x = +-0, q0 is the correctly signed
+// zero, each remainder is +0 and +0*y = -0 (y < 0) leaves q0 unchanged.
+// Callers divide directly outside that set.  Zero residuals are common (metal,
+// and vacuum the solve has not reached), so they stay on the fast path.
 __device__ inline double div_fast(double x, double d, double y) {
   double q = x * y;
   double r = __builtin_fma(-q, d, x);
@@ -99,14 +110,25 @@ __device__ inline double div_fast(double x, double d, double y) {
 
 __device__ inline bool div_fast_ok(double x) {
   double ax = fabs(x);
-  return ax > 1e-280 && ax < 1e280;
+  return ax == 0 || (ax > 1e-280 && ax < 1e280);
+}
+
+// Workgroups are dealt to the 8 XCDs in turn (workgroup b runs on XCD b % 8).
+// Tile order is row-major, so giving XCD x the contiguous run of tiles
+// [start_x, start_x + count_x) puts a band of whole tile rows on each XCD:
+// a tile's halo was mostly written by tiles of the same XCD in the previous
+// batch and is still in that XCD's L2, as are the band's constant rhs and
+// flags.  The map is a bijection of [0, n).
+__device__ inline int xcd_band_tile(int b, int n) {
+  const int x = b & 7, slot = b >> 3, q = n >> 3, rem = n & 7;
+  return x * q + min(x, rem) + slot;
 }
 
 struct SorCommon {
   int nz, nr, ldz, left_dir, right_dir;
   double a, e;
   double y_e, y_axis;    // RN(1/e), RN(1/(e - 2*c[0]))
-  int use_y;             // 1: both reciprocals are safe for div_fast
+  int use_y;             // 1: both diagonals are negative and their reciprocals safe for div_fast
   const double *c, *d;   // per-row stencil coefficients
   const double *omega;   // per-half-sweep omega
   const double *rhs;     // may be null
@@ -149,6 +171,14 @@ struct SorLane {
   bool r_mirror[2];  // column is iz == nz-1: its right neighbour mirrors the left one
 };
 
+// Column pair of a lane: the lane's two columns iz_e, iz_e+1 (iz_e even) are
+// adjacent in a row, and ldz is even, so both come in one 16-byte access
+// (flags: one 4-byte access).  iz_e is clamped to ldz-2 so the pair stays in
+// the row's storage; columns past nz are never updated or owned.
+__device__ inline int64_t sor_pair(const SorCommon &a, int iz_e, int ir) {
+  return fix(a.ldz, min(iz_e, a.ldz - 2), min(ir, a.nr));
+}
+
 // this lane's points from src; every address is clamped into the array so
 // all loads issue together (points outside the tile are never updated)
 template <class G>
@@ -157,28 +187,29 @@ __device__ inline void sor_load_P(const SorCommon &a, const SorTile<G> &t, int w
   const int iz_e = t.zlo + 2 * lane;
 #pragma unroll
   for (int i = 0; i < G::RS; i++) {
-    const int irc = min(t.rlo + w * G::RS + i, a.nr);
-#pragma unroll
-    for (int col = 0; col < 2; col++) P[i][col] = src[fix(a.ldz, min(iz_e + col, a.nz), irc)];
+    const double2 v = *reinterpret_cast<const double2 *>(src + sor_pair(a, iz_e, t.rlo + w * G::RS + i));
+    P[i][0] = v.x;
+    P[i][1] = v.y;
   }
 }
 
-// right-hand side, masks and row coefficients (cr/dr in LDS, TRW each)
+// right-hand side, masks and row coefficients (cr/dr in LDS, TRW each);
+// without a right-hand side the loads read src (same layout) and are discarded
 template <class G>
 __device__ inline void sor_setup(const SorCommon &a, const SorTile<G> &t, int w, int lane, SorLane<G> &ln,
-                                 double *cr, double *dr) {
+                                 const double *src, double *cr, double *dr) {
   const int iz_e = t.zlo + 2 * lane;
-  const double *rhs = a.rhs ? a.rhs : a.omega;   // any valid address when there is no rhs
+  const double *rhs = a.rhs ? a.rhs : src;
   int16_t fl[G::RS][2];
 #pragma unroll
   for (int i = 0; i < G::RS; i++) {
-    const int irc = min(t.rlo + w * G::RS + i, a.nr);
-#pragma unroll
-    for (int col = 0; col < 2; col++) {
-      const int64_t p = fix(a.ldz, min(iz_e + col, a.nz), irc);
-      ln.D[i][col] = a.rhs ? rhs[p] : 0.0;
-      fl[i][col] = a.flags[p];
-    }
+    const int64_t p = sor_pair(a, iz_e, t.rlo + w * G::RS + i);
+    const double2 v = *reinterpret_cast<const double2 *>(rhs + p);
+    const uint32_t f2 = *reinterpret_cast<const uint32_t *>(a.flags + p);
+    ln.D[i][0] = a.rhs ? v.x : 0.0;
+    ln.D[i][1] = a.rhs ? v.y : 0.0;
+    fl[i][0] = (int16_t)(f2 & 0xffffu);
+    fl[i][1] = (int16_t)(f2 >> 16);
   }
   for (int k = threadIdx.x; k < G::TRW; k += G::NT) {
     const int irc = min(t.rlo + k, a.nr);
@@ -259,7 +290,7 @@ __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsi
     res[i] = residual;
     x[i] = omega * residual;
     q[i] = div_fast(x[i], diag, y);
-    slow |= !div_fast_ok(x[i]);
+    slow |= ((okb >> (2 * i + col)) & 1) && !div_fast_ok(x[i]);
   }
   if (slow || !a.use_y) {
 #pragma unroll
@@ -280,10 +311,11 @@ __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsi
   return acc;
 }
 
-// One local step j of a batch, colour C; returns the lane's owned residual sum.
+// One local step j of a batch, colour C, relaxation factor omega; returns the
+// lane's owned residual sum.
 template <class G, int C>
 __device__ inline double sor_step(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane,
-                                  int64_t n, int j, double2 (*xrow)[G::SW][2][64], const double *cr,
+                                  double omega, int j, double2 (*xrow)[G::SW][2][64], const double *cr,
                                   const double *dr, bool axis_wave) {
   const int par = j & 1;
   // publish the first and last rows, then read the neighbouring wavefronts' rows
@@ -292,7 +324,6 @@ __device__ inline double sor_step(const SorCommon &a, const SorTile<G> &t, SorLa
   __syncthreads();
   const double2 up_row = w + 1 < G::SW ? xrow[par][w + 1][0][lane] : make_double2(0.0, 0.0);
   const double2 dn_row = w > 0 ? xrow[par][w - 1][1][lane] : make_double2(0.0, 0.0);
-  const double omega = a.omega[n];
   // points valid at this step: column depth and the wavefront's share of the valid rows
   const int vr0 = t.sr_lo ? j + 1 : 0, vr1 = t.LR - (t.sr_hi ? j + 1 : 0);
   const int lo = min(max(vr0 - w * G::RS, 0), G::RS), hi = min(max(vr1 - w * G::RS, 0), G::RS);
@@ -301,13 +332,15 @@ __device__ inline double sor_step(const SorCommon &a, const SorTile<G> &t, SorLa
   return sor_half_sweep<G, C>(a, ln, okb, up_row, dn_row, cr, dr, w, omega, t.redge, axis_wave);
 }
 
-// k <= SK half-sweeps n0 .. n0+k-1; ares[j] gets the lane's owned residual sum
-// of step j.  For an even n0 (every batch of a solve) the steps are unrolled
-// with their colours 0, 1, 0, ... known at compile time.
+// k <= SK half-sweeps n0 .. n0+k-1 with relaxation factors om[0 .. k-1] (LDS,
+// read after each step's barrier);
+// ares[j] gets the lane's owned residual sum of step j.  For an even n0 (every
+// batch of a solve) the steps are unrolled with their colours 0, 1, 0, ...
+// known at compile time.
 template <class G>
 __device__ inline void sor_sweeps(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane,
-                                  int64_t n0, int k, double2 (*xrow)[G::SW][2][64], const double *cr,
-                                  const double *dr, double (&ares)[G::SK]) {
+                                  int64_t n0, int k, const double *om, double2 (*xrow)[G::SW][2][64],
+                                  const double *cr, const double *dr, double (&ares)[G::SK]) {
 #pragma unroll
   for (int j = 0; j < G::SK; j++) ares[j] = 0;
   const bool axis_wave = t.rlo == 0 && w == 0;
@@ -315,14 +348,15 @@ __device__ inline void sor_sweeps(const SorCommon &a, const SorTile<G> &t, SorLa
 #pragma unroll
     for (int j = 0; j < G::SK; j++) {
       if (j >= k) break;
-      ares[j] = (j & 1) ? sor_step<G, 1>(a, t, ln, w, lane, n0 + j, j, xrow, cr, dr, axis_wave)
-                        : sor_step<G, 0>(a, t, ln, w, lane, n0 + j, j, xrow, cr, dr, axis_wave);
+      ares[j] = (j & 1) ? sor_step<G, 1>(a, t, ln, w, lane, om[j], j, xrow, cr, dr, axis_wave)
+                        : sor_step<G, 0>(a, t, ln, w, lane, om[j], j, xrow, cr, dr, axis_wave);
     }
   } else {
 #pragma unroll 1
     for (int j = 0; j < k; j++) {
-      const double acc = ((n0 + j) & 1) ? sor_step<G, 1>(a, t, ln, w, lane, n0 + j, j, xrow, cr, dr, axis_wave)
-                                        : sor_step<G, 0>(a, t, ln, w, lane, n0 + j, j, xrow, cr, dr, axis_wave);
+      const double omega = om[j];
+      const double acc = ((n0 + j) & 1) ? sor_step<G, 1>(a, t, ln, w, lane, omega, j, xrow, cr, dr, axis_wave)
+                                        : sor_step<G, 0>(a, t, ln, w, lane, omega, j, xrow, cr, dr, axis_wave);
 #pragma unroll
       for (int jj = 0; jj < G::SK; jj++)
         if (jj == j) ares[jj] = acc;
@@ -349,15 +383,21 @@ __device__ inline void sor_block_sums(const double (&ares)[G::SK], int k, int w,
   }
 }
 
+// owned points to dst: a 16-byte store where both columns are owned
 template <class G>
 __device__ inline void sor_write_owned(const SorCommon &a, const SorTile<G> &t, const SorLane<G> &ln, int w,
                                        int lane, double *dst) {
   const int iz_e = t.zlo + 2 * lane;
 #pragma unroll
   for (int i = 0; i < G::RS; i++) {
-#pragma unroll
-    for (int col = 0; col < 2; col++)
-      if ((ln.own >> (2 * i + col)) & 1) dst[fix(a.ldz, iz_e + col, t.rlo + w * G::RS + i)] = ln.P[i][col];
+    const unsigned o = (ln.own >> (2 * i)) & 3u;
+    double *p = dst + fix(a.ldz, iz_e, t.rlo + w * G::RS + i);
+    if (o == 3u) {
+      *reinterpret_cast<double2 *>(p) = make_double2(ln.P[i][0], ln.P[i][1]);
+    } else {
+      if (o & 1u) p[0] = ln.P[i][0];
+      if (o & 2u) p[1] = ln.P[i][1];
+    }
   }
 }
 
@@ -379,14 +419,18 @@ __global__ __launch_bounds__(BatchCfg::NT) void sor_batch_kernel(SorArgs a) {
   __shared__ double cr[G::TRW], dr[G::TRW];
   const int64_t done = a.force ? 0 : a.state[0];
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const SorTile<G> t(a.m, blockIdx.x);
+  const int tile = xcd_band_tile((int)blockIdx.x, a.m.ntiles);
+  const SorTile<G> t(a.m, tile);
+  // the batch's relaxation factors, fetched with the tile
+  __shared__ double om[G::SK];
+  if (threadIdx.x < G::SK) om[threadIdx.x] = a.m.omega[a.n0 + min((int)threadIdx.x, a.k - 1)];
   SorLane<G> ln;
   sor_load_P<G>(a.m, t, w, lane, a.Pin, ln.P);
-  sor_setup<G>(a.m, t, w, lane, ln, cr, dr);
+  sor_setup<G>(a.m, t, w, lane, ln, a.Pin, cr, dr);
   if (done) return;
   double ares[G::SK];
-  sor_sweeps<G>(a.m, t, ln, w, lane, a.n0, a.k, xrow, cr, dr, ares);
-  sor_block_sums<G>(ares, a.k, w, lane, wsum, a.partials + blockIdx.x, a.m.ntiles);
+  sor_sweeps<G>(a.m, t, ln, w, lane, a.n0, a.k, om, xrow, cr, dr, ares);
+  sor_block_sums<G>(ares, a.k, w, lane, wsum, a.partials + tile, a.m.ntiles);
   sor_write_owned<G>(a.m, t, ln, w, lane, a.Pout);
 }
 
@@ -599,7 +643,7 @@ SorCommon sor_common(Run &R, const double *rhs, int oz, int orr) {
     double m = std::fabs(v);
     return m > 1e-200 && m < 1e200;
   };
-  a.use_y = safe(a.e) && safe(diag0) && safe(a.y_e) && safe(a.y_axis);
+  a.use_y = safe(a.e) && safe(diag0) && safe(a.y_e) && safe(a.y_axis) && a.e < 0 && diag0 < 0;
   a.c = R.sor_c;
   a.d = R.sor_d;
   a.omega = R.omega;
@@ -618,6 +662,8 @@ int64_t sor_tiles(const Run &R) {
 }
 
 int sor_batch() { return BatchCfg::SK; }
+
+int sor_check_group() { return kCheckGroup; }
 
 hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const double *rhs, int64_t n0, int k,
                             int partial_row, int force, hipStream_t s) {
