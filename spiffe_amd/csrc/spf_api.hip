@@ -519,11 +519,9 @@ int spf_fields_advance(spf_ctx *c, int32_t run) {
 int spf_currents_reset(spf_ctx *c, int32_t run) {
   Run *R = get_run(c, run);
   if (!R) return fail(c, SPF_EINVAL, "bad run id");
-  if (R->d.modes & SPF_TM_FIELDS) {
-    HIPCALL(c, launch_fill(R->Jz, R->cells, 0.0, c->stream));
-    HIPCALL(c, launch_fill(R->Jr, R->cells, 0.0, c->stream));
-  }
-  if (R->d.modes & SPF_TE_FIELDS) HIPCALL(c, launch_fill(R->Jphi, R->cells, 0.0, c->stream));
+  const bool tm = R->d.modes & SPF_TM_FIELDS, te = R->d.modes & SPF_TE_FIELDS;
+  HIPCALL(c, launch_fill3(tm ? R->Jz : nullptr, tm ? R->Jr : nullptr, te ? R->Jphi : nullptr, R->cells, 0.0,
+                          c->stream));
   return SPF_OK;
 }
 

@@ -44,6 +44,8 @@ struct KeyArgs {
   double *vals;
   uint32_t discard;
   int cell_bits;
+  int64_t *counters;            // [5], [6], [9] preset by thread 0 for the segment pass
+  int64_t n_pairs;
 };
 
 // NGP value of the Jz/Jphi loops for cell row ir (0 <= ir <= nr-2), :92-100, :254-262
@@ -54,6 +56,11 @@ __device__ inline double jm_centered(double QVG, long long ir) {
 
 __global__ __launch_bounds__(256) void key_kernel(KeyArgs a) {
   int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip == 0) {
+    a.counters[5] = 0;            // number of segments
+    a.counters[6] = a.n_pairs;    // first discarded position
+    a.counters[9] = 0;            // number of long segments
+  }
   if (ip >= a.n_ub) return;
   int64_t np = *a.np;
   bool live = ip < np && (a.status[ip] & SPF_PART_ACTIVE);
@@ -121,22 +128,6 @@ __global__ __launch_bounds__(256) void key_kernel(KeyArgs a) {
       a.vals[slot] = 0.0;
     }
   }
-}
-
-__global__ void init_segments_kernel(int64_t *counters, int64_t n) {
-  counters[5] = 0;   // number of segments
-  counters[6] = n;   // first discarded position
-  counters[9] = 0;   // number of long segments
-}
-
-__global__ __launch_bounds__(256) void mark_kernel(const uint32_t *sk, int64_t n, uint8_t *flag, int64_t *counters,
-                                                   uint32_t discard) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t k = sk[i];
-  bool first = i == 0 || sk[i - 1] != k;
-  flag[i] = (k != discard && first) ? 1 : 0;
-  if (k == discard && first) counters[6] = i;
 }
 
 struct FoldArgs {
@@ -328,19 +319,16 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   k.vals = R.cz;
   k.discard = discard;
   k.cell_bits = cell_bits;
-  hipLaunchKernelGGL(init_segments_kernel, dim3(1), dim3(1), 0, s, R.counters, n);
+  k.counters = R.counters;
+  k.n_pairs = n;
+  hipLaunchKernelGGL(key_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, s, k);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(key_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, s, k);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
   size_t bytes = R.sort_tmp_bytes;
   e = rocprim::radix_sort_pairs(R.sort_tmp, bytes, R.keys, R.skeys, R.cz, R.scz, (size_t)n, 0u, (unsigned)key_bits,
                                 s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(mark_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, R.skeys, n, R.gflag,
-                     R.counters, discard);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  e = ordered_compact(R.gflag, n, R.gstart, R.counters + 5, R.bcount, R.boff, s);
+  e = segment_starts(R.skeys, n, discard, R.gstart, R.counters, R.bcount, R.boff, s);
   if (e != hipSuccess) return e;
   FoldArgs f;
   f.nz = R.d.nz;

@@ -459,9 +459,12 @@ __global__ void transpose_in_i16_kernel(const int16_t *h, int16_t *d, int nz, in
   d[(int64_t)ir * ldz + iz] = h[(int64_t)iz * (nr + 1) + ir];
 }
 
-__global__ void fill_kernel(double *p, int64_t n, double v) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = v;
+// up to three arrays of n values set to v in one launch (null pointers skipped)
+__global__ void fill3_kernel(double *p0, double *p1, double *p2, int64_t n, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 3 * n; i += (int64_t)gridDim.x * blockDim.x) {
+    double *p = i < n ? p0 : (i < 2 * n ? p1 : p2);
+    if (p) p[i % n] = v;
+  }
 }
 
 __global__ void scale_kernel(double *p, int64_t n, double f) {
@@ -529,11 +532,11 @@ hipError_t launch_transpose_in_i16(const int16_t *h, int16_t *d, int nz, int nr,
   return hipGetLastError();
 }
 
-hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  int64_t blocks = (n + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+hipError_t launch_fill3(double *p0, double *p1, double *p2, int64_t n, double v, hipStream_t s) {
+  if (n <= 0 || (!p0 && !p1 && !p2)) return hipSuccess;
+  int64_t blocks = (3 * n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(fill3_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p0, p1, p2, n, v);
   return hipGetLastError();
 }
 

@@ -176,7 +176,7 @@ hipError_t launch_fdtd(const FdtdArgs &a, hipStream_t s);
 hipError_t launch_transpose_in(const double *host_layout, double *dev, int nz, int nr, int ldz, hipStream_t s);
 hipError_t launch_transpose_out(const double *dev, double *host_layout, int nz, int nr, int ldz, hipStream_t s);
 hipError_t launch_transpose_in_i16(const int16_t *host_layout, int16_t *dev, int nz, int nr, int ldz, hipStream_t s);
-hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t s);
+hipError_t launch_fill3(double *p0, double *p1, double *p2, int64_t n, double v, hipStream_t s);
 hipError_t launch_set_i64(int64_t *p, int64_t v, hipStream_t s);
 hipError_t launch_scale(double *p, int64_t n, double f, hipStream_t s);
 hipError_t launch_add_line(double *J, int ldz, int direction, int64_t ipos, int64_t istart, int64_t iend, double v,
@@ -193,8 +193,11 @@ hipError_t launch_append_commit(Run &R, int64_t n, int64_t n_active, hipStream_t
 int cathode_emit_step(spf_ctx *c, int32_t run, int32_t cathode, int64_t *n_emitted);
 int cathode_current_density(spf_ctx *c, int32_t run, int32_t cathode);
 // ordered compaction: out[k] = k-th index i < n with flags[i] != 0; *count = total.
-hipError_t ordered_compact(const uint8_t *flags, int64_t n, int32_t *out, int64_t *count, int32_t *bcount,
-                           int32_t *boff, hipStream_t s);
+// positions of sorted keys sk[0..n) that start a segment (key differs from
+// its predecessor, not `discard`), in order, into out; counters[5] gets their
+// number, counters[6] the first discard position (preset to n)
+hipError_t segment_starts(const uint32_t *sk, int64_t n, uint32_t discard, int32_t *out, int64_t *counters,
+                          int32_t *bcount, int32_t *boff, hipStream_t s);
 int64_t compact_blocks(int64_t n);
 // deposition
 hipError_t deposit_currents(Run &R, hipStream_t s);   // Jz, Jr, Jphi += particle currents

@@ -1,5 +1,5 @@
-// spf_util.hip -- deterministic building blocks: ordered stream compaction
-// (index order preserved, no atomics) and fixed-order FP64 reductions.
+// spf_util.hip -- deterministic building blocks: ordered segment-start
+// compaction (position order preserved) and fixed-order FP64 reductions.
 //
 // Every reduction here sums in an order fixed by the problem size alone, so a
 // run gives the same bits whatever else shares the GPU.
@@ -30,13 +30,29 @@ __device__ int block_scan_flag(bool f, int *wsum, int &excl) {
   return total;
 }
 
-__global__ __launch_bounds__(CB) void compact_count_kernel(const uint8_t *flags, int64_t n, int32_t *bcount) {
+// Position i of sorted keys starts a segment when its key differs from the
+// previous one and is not the discard key.
+struct SegStart {
+  const uint32_t *sk;
+  uint32_t discard;
+  __device__ bool operator()(int64_t i) const {
+    const uint32_t k = sk[i];
+    return k != discard && (i == 0 || sk[i - 1] != k);
+  }
+};
+
+// Per-block segment counts; also records in counters[6] the first position
+// holding the discard key (counters[6] is preset to n).
+__global__ __launch_bounds__(CB) void seg_count_kernel(SegStart f, int64_t n, int32_t *bcount, int64_t *counters) {
   __shared__ int wsum[CB / 64];
-  int64_t base = (int64_t)blockIdx.x * CSPAN;
+  const int64_t base = (int64_t)blockIdx.x * CSPAN;
   int c = 0;
   for (int j = 0; j < CITEMS; j++) {
-    int64_t i = base + (int64_t)j * CB + threadIdx.x;
-    if (i < n && flags[i]) c++;
+    const int64_t i = base + (int64_t)j * CB + threadIdx.x;
+    if (i < n) {
+      if (f(i)) c++;
+      if (f.sk[i] == f.discard && (i == 0 || f.sk[i - 1] != f.discard)) counters[6] = i;
+    }
   }
   // wave sums then block sum (integer: order irrelevant)
   for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
@@ -49,25 +65,25 @@ __global__ __launch_bounds__(CB) void compact_count_kernel(const uint8_t *flags,
   }
 }
 
-// single workgroup: exclusive scan of nb block counts
-__global__ __launch_bounds__(1024) void compact_scan_kernel(const int32_t *bcount, int64_t nb, int32_t *boff,
-                                                            int64_t *count) {
+// single workgroup: exclusive scan of nb block counts, total into *count
+__global__ __launch_bounds__(1024) void block_scan_kernel(const int32_t *bcount, int64_t nb, int32_t *boff,
+                                                          int64_t *count) {
   __shared__ int buf[1024];
   __shared__ long long carry;
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
   for (int64_t c = 0; c < nb; c += 1024) {
-    int64_t k = c + threadIdx.x;
-    int v = k < nb ? bcount[k] : 0;
+    const int64_t k = c + threadIdx.x;
+    const int v = k < nb ? bcount[k] : 0;
     buf[threadIdx.x] = v;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-      int t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+      const int t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
       __syncthreads();
       buf[threadIdx.x] += t;
       __syncthreads();
     }
-    int incl = buf[threadIdx.x];
+    const int incl = buf[threadIdx.x];
     if (k < nb) boff[k] = (int32_t)(carry + incl - v);
     __syncthreads();
     if (threadIdx.x == 1023) carry += incl;
@@ -76,17 +92,17 @@ __global__ __launch_bounds__(1024) void compact_scan_kernel(const int32_t *bcoun
   if (threadIdx.x == 0) *count = carry;
 }
 
-__global__ __launch_bounds__(CB) void compact_scatter_kernel(const uint8_t *flags, int64_t n, const int32_t *boff,
-                                                             int32_t *out) {
+// segment starts in position order at their block's offset
+__global__ __launch_bounds__(CB) void seg_scatter_kernel(SegStart f, int64_t n, const int32_t *boff, int32_t *out) {
   __shared__ int wsum[CB / 64];
-  int64_t base = (int64_t)blockIdx.x * CSPAN;
+  const int64_t base = (int64_t)blockIdx.x * CSPAN;
   int running = boff[blockIdx.x];
   for (int j = 0; j < CITEMS; j++) {
-    int64_t i = base + (int64_t)j * CB + threadIdx.x;
-    bool f = i < n && flags[i];
+    const int64_t i = base + (int64_t)j * CB + threadIdx.x;
+    const bool fl = i < n && f(i);
     int excl;
-    int tot = block_scan_flag(f, wsum, excl);
-    if (f) out[running + excl] = (int32_t)i;
+    const int tot = block_scan_flag(fl, wsum, excl);
+    if (fl) out[running + excl] = (int32_t)i;
     running += tot;
   }
 }
@@ -140,17 +156,17 @@ hipError_t launch_set_i64(int64_t *p, int64_t v, hipStream_t s) {
 
 int64_t compact_blocks(int64_t n) { return (n + CSPAN - 1) / CSPAN; }
 
-hipError_t ordered_compact(const uint8_t *flags, int64_t n, int32_t *out, int64_t *count, int32_t *bcount,
-                           int32_t *boff, hipStream_t s) {
-  int64_t nb = compact_blocks(n);
-  if (nb == 0) return launch_set_i64(count, 0, s);
-  hipLaunchKernelGGL(compact_count_kernel, dim3((unsigned)nb), dim3(CB), 0, s, flags, n, bcount);
+hipError_t segment_starts(const uint32_t *sk, int64_t n, uint32_t discard, int32_t *out, int64_t *counters,
+                          int32_t *bcount, int32_t *boff, hipStream_t s) {
+  const int64_t nb = compact_blocks(n);
+  if (nb == 0) return launch_set_i64(counters + 5, 0, s);
+  SegStart f{sk, discard};
+  hipLaunchKernelGGL(seg_count_kernel, dim3((unsigned)nb), dim3(CB), 0, s, f, n, bcount, counters);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(1024), 0, s, bcount, nb, boff, count);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(compact_scatter_kernel, dim3((unsigned)nb), dim3(CB), 0, s, flags, n, boff, out);
+  hipLaunchKernelGGL(block_scan_kernel, dim3(1), dim3(1024), 0, s, bcount, nb, boff, counters + 5);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(seg_scatter_kernel, dim3((unsigned)nb), dim3(CB), 0, s, f, n, boff, out);
   return hipGetLastError();
 }
 
