@@ -33,7 +33,7 @@ void free_run(Run *R) {
   };
   for (int k = 0; k < 6; k++)
     for (int j = 0; j < 2; j++) F(R->E2[k][j]);
-  F(R->Bphi2); F(R->Br2); F(R->Bz2); F(R->Jz); F(R->Jr); F(R->Jphi); F(R->Psi); F(R->Q); for (int k = 0; k < 4; k++) F(R->PsiSpare[k]);
+  F(R->Bphi2); F(R->Br2); F(R->Bz2); F(R->Jz); F(R->Jr); F(R->Jphi); F(R->Psi); F(R->Q); for (int k = 0; k < 5; k++) F(R->PsiSpare[k]);
   F(R->EzImp); F(R->ErImp); F(R->PsiImp); F(R->BzImp); F(R->BrImp); F(R->flags); F(R->cut0); F(R->cut1);
   F(R->cp); F(R->cm); F(R->bzf); F(R->sor_c); F(R->sor_d); F(R->volume);
   for (int k = 0; k < 16; k++) F(R->P[k]);
@@ -196,7 +196,7 @@ void *spf_stream(spf_ctx *c) { return c ? (void *)c->stream : nullptr; }
 int64_t spf_run_footprint(const spf_run_desc *d) {
   if (!d || d->nz < 3 || d->nr < 3) return -1;
   int64_t cells = ldz_for(d->nz) * (int64_t)(d->nr + 1);
-  int64_t field_bytes = cells * 8 * 32 + cells * 2;                      // 32 double grids + flags
+  int64_t field_bytes = cells * 8 * 33 + cells * 2;                      // 33 double grids + flags
   int64_t n = d->max_particles > 0 ? d->max_particles : 1;
   int64_t part_bytes = n * (16 * 8 + 4 + 2 + 4);                         // particle SoA
   int64_t event_bytes = n * (4 + 1 + 7 * 8 + 4 + 8);                     // staging, lists
@@ -245,7 +245,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
     for (int j = 0; j < 2; j++) ALLOC(R->E2[k][j], cells);
   ALLOC(R->Bphi2, cells); ALLOC(R->Br2, cells); ALLOC(R->Bz2, cells);
   ALLOC(R->Jz, cells); ALLOC(R->Jr, cells); ALLOC(R->Jphi, cells);
-  ALLOC(R->Psi, cells); ALLOC(R->Q, cells); for (int k = 0; k < 4; k++) ALLOC(R->PsiSpare[k], cells);
+  ALLOC(R->Psi, cells); ALLOC(R->Q, cells); for (int k = 0; k < 5; k++) ALLOC(R->PsiSpare[k], cells);
   ALLOC(R->EzImp, cells); ALLOC(R->ErImp, cells); ALLOC(R->PsiImp, cells);
   ALLOC(R->BzImp, cells); ALLOC(R->BrImp, cells);
   ALLOC(R->flags, cells);
@@ -275,7 +275,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   }
   R->nblk_cap = compact_blocks(cap3) + 1;
   ALLOC(R->bcount, R->nblk_cap); ALLOC(R->boff, R->nblk_cap);
-  R->red_n = std::max<int64_t>(sor_tiles(*R) * sor_batch() * 4, 512) + 16;
+  R->red_n = std::max<int64_t>(sor_tiles(*R) * sor_batch() * 4 * 2, 512) + 16;   // two check groups of partials
   ALLOC(R->red, R->red_n);
   ALLOC(R->red_counter, 4); ALLOC(R->sor_state, 4); ALLOC(R->sor_sums, 8);
   ALLOC(R->Pdev, 16);

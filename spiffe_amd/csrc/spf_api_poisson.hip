@@ -13,7 +13,7 @@ namespace {
 
 constexpr double PI_D = 3.141592653589793238462643;
 constexpr int64_t kSorCheckBatches = 16;   // batches queued between host checks of the done flag
-constexpr int kRing = 5;                   // Psi buffers the batches rotate through (> batches per check)
+constexpr int kRing = 6;                   // Psi buffers the batches rotate through (> batches per check + 1)
 
 // omega for half-sweep k: src/poisson.c:47-49 and :121-124
 int ensure_omega(spf_ctx *c, Run *R, int64_t n) {
@@ -42,11 +42,11 @@ int ensure_omega(spf_ctx *c, Run *R, int64_t n) {
 // the run's spares.
 void adopt_result(Run *R, double **Psi, double *const *ring, int nring, int out) {
   double *res = ring[out];
-  double *spare[4];
+  double *spare[5];
   int si = 0;
   for (int i = 0; i < nring; i++)
     if (i != out) spare[si++] = ring[i];
-  for (int i = 0; i < 4; i++) R->PsiSpare[i] = spare[i];
+  for (int i = 0; i < 5; i++) R->PsiSpare[i] = spare[i];
   *Psi = res;
 }
 
@@ -54,14 +54,16 @@ void adopt_result(Run *R, double **Psi, double *const *ring, int nring, int out)
 // may be swapped with one of the run's spare buffers); *iters = n on
 // convergence, -1 otherwise.  Batches rotate through a ring of kRing
 // buffers: batch b reads ring[b % kRing] and writes ring[(b+1) % kRing].  The
-// stopping rule is applied every sor_check_group() batches by one check
-// launch over their half-sweeps; queued batches return at once after a stop,
-// and the batch that contains the stopping half-sweep still has its input in
-// the ring (kRing > batches per check), so it is re-run for exactly the
-// half-sweeps needed.
-// Batches are queued in windows of kSorCheckBatches; the host reads a
-// window's done flag while the next window is already queued, so the device
-// never waits for the host between windows.
+// stopping rule is applied per check group of sor_check_group() batches by
+// an extra workgroup of the next group's first batch launch (a check launch
+// after the last group), on partial rows that alternate between two halves
+// of the partials buffer.  After a stop the queued batches return at once;
+// at most the rest of the stopping group and the next group's first batch
+// have run past the stopping half-sweep, so the batch that contains it still
+// has its input in the ring (kRing > batches per check + 1) and is re-run for
+// exactly the half-sweeps needed.  Batches are queued in windows of
+// kSorCheckBatches; the host reads a window's done flag while the next window
+// is already queued, so the device never waits for the host between windows.
 int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accuracy, int64_t limit, int64_t *iters) {
   if (accuracy <= 0) return fail(c, SPF_EINVAL, "accuracy is <= 0 in solve_poisson_cyl");
   if (limit <= 1) return fail(c, SPF_EINVAL, "iteration_limit is <= 1 in solve_poisson_cyl");
@@ -72,10 +74,14 @@ int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accura
   if (rhs)
     HIPCALL(c, launch_sum_abs(rhs, R->ldz, R->d.nz, R->d.nr - 1, R->red, R->red_n, R->sor_sums, s));
   const int K = sor_batch(), KC = sor_check_group();
-  double *ring[kRing] = {*Psi, R->PsiSpare[0], R->PsiSpare[1], R->PsiSpare[2], R->PsiSpare[3]};
+  const int64_t group_rows = (int64_t)KC * K;   // partial rows of one check group
+  double *ring[kRing] = {*Psi, R->PsiSpare[0], R->PsiSpare[1], R->PsiSpare[2], R->PsiSpare[3], R->PsiSpare[4]};
   int64_t nb = (limit + K - 1) / K, b = 0, converged = -1;
   int64_t *flag[2] = {c->pinned + 16, c->pinned + 18};   // [done, iterations] of each window in flight
   int pending = -1, win = 0;
+  int64_t gi = 0;                                          // check groups queued so far
+  int64_t prev_row = 0, prev_n0 = 0;                       // the queued group whose check is still due
+  int prev_k = 0;
   auto collect = [&](int slot) -> int {
     HIPCALL(c, hipEventSynchronize(c->sor_ev[slot]));
     if (flag[slot][0]) converged = flag[slot][1];
@@ -84,17 +90,24 @@ int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accura
   while (b < nb && converged < 0) {
     int64_t win_end = std::min<int64_t>(nb, b + kSorCheckBatches);
     while (b < win_end) {
-      int64_t g0 = b, g1 = std::min<int64_t>(win_end, b + KC);
+      const int64_t g0 = b, g1 = std::min<int64_t>(win_end, b + KC);
+      const int64_t row0 = (gi & 1) * group_rows;
       {
         PhaseTimer bt(c, SPF_T_SOR_BATCH, g1 - b);   // the group's batch launches, back to back
         for (; b < g1; b++) {
           int64_t n0 = b * K;
           int k = (int)std::min<int64_t>(K, limit - n0);
+          // the group's first batch also checks the previous group
+          const int ck = b == g0 ? prev_k : 0;
           HIPCALL(c, launch_sor_batch(*R, ring[b % kRing], ring[(b + 1) % kRing], rhs, n0, k,
-                                      (int)((b - g0) * K), 0, s));
+                                      (int)(row0 + (b - g0) * K), 0, s, prev_row, prev_n0, ck, accuracy));
+          if (b == g0) prev_k = 0;
         }
       }
-      HIPCALL(c, launch_sor_check(*R, g0 * K, (int)(std::min<int64_t>(b * K, limit) - g0 * K), accuracy, s));
+      prev_row = row0;
+      prev_n0 = g0 * K;
+      prev_k = (int)(std::min<int64_t>(g1 * K, limit) - g0 * K);
+      gi++;
     }
     const int slot = win++ & 1;
     HIPCALL(c, hipMemcpyAsync(flag[slot], R->sor_state, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -102,7 +115,13 @@ int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accura
     if (pending >= 0 && (rc = collect(pending))) return rc;
     pending = slot;
   }
-  if (converged < 0 && pending >= 0 && (rc = collect(pending))) return rc;
+  if (converged < 0) {
+    // the last group's check, then the final flag
+    if (prev_k > 0) HIPCALL(c, launch_sor_check(*R, prev_row, prev_n0, prev_k, accuracy, s));
+    HIPCALL(c, hipMemcpyAsync(c->pinned + 20, R->sor_state, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCALL(c, hipStreamSynchronize(s));
+    if (c->pinned[20]) converged = c->pinned[21];
+  }
   int out;
   if (converged >= 0) {
     int64_t bc = converged / K, n0 = bc * K;

@@ -102,7 +102,7 @@ struct Run {
   double *Bphi2 = nullptr, *Br2 = nullptr, *Bz2 = nullptr;
   double *Jz = nullptr, *Jr = nullptr, *Jphi = nullptr;
   double *Psi = nullptr, *Q = nullptr;
-  double *PsiSpare[4] = {};   // spare buffers the SOR batches rotate through
+  double *PsiSpare[5] = {};   // spare buffers the SOR batches rotate through
   double *EzImp = nullptr, *ErImp = nullptr, *PsiImp = nullptr, *BzImp = nullptr, *BrImp = nullptr;
   int16_t *flags = nullptr;
   int32_t *cut0 = nullptr, *cut1 = nullptr;
@@ -216,9 +216,13 @@ int64_t sor_tiles(const Run &R);
 int sor_check_group();   // batches per stopping-rule check (<= 4)
 int sor_batch();
 
+// check_k > 0: the launch carries one extra workgroup that applies the
+// stopping rule to the previous check group (check_k half-sweeps from
+// check_n0, partial rows from check_row) while the tiles run
 hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const double *rhs, int64_t n0, int k,
-                            int partial_row, int force, hipStream_t s);
-hipError_t launch_sor_check(Run &R, int64_t n0, int k, double accuracy, hipStream_t s);
+                            int partial_row, int force, hipStream_t s, int64_t check_row = 0, int64_t check_n0 = 0,
+                            int check_k = 0, double accuracy = 0);
+hipError_t launch_sor_check(Run &R, int64_t check_row, int64_t n0, int k, double accuracy, hipStream_t s);
 hipError_t launch_field_from_psi(Run &R, int imposed, double field_factor, hipStream_t s);
 hipError_t launch_zero_psi(Run &R, hipStream_t s);
 

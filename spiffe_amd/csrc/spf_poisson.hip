@@ -7,9 +7,10 @@
 // affect Psi; the device gets the same bits with any schedule that keeps the
 // colour order (here: several half-sweeps per launch on LDS tiles).  The
 // convergence test needs sum |residual| after every half-sweep: each tile
-// writes one partial per half-sweep and a one-workgroup check kernel adds
-// them in tile order (fixed, so deterministic) and sets a done flag that makes
-// the queued batches return at once.  The sum's rounding differs from the
+// writes one partial per half-sweep; one extra workgroup of the next batch
+// launch (or a check launch after the last batch) adds them in tile order
+// (fixed, so deterministic) and sets a done flag that makes the queued
+// batches return at once.  The sum's rounding differs from the
 // serial sum, so the iteration count can differ from the reference only when
 // the residual lands within rounding of the threshold.
 #include <cmath>
@@ -34,8 +35,8 @@ namespace {
 // one-colour-per-sweep order produces.  Each workgroup also sums |residual|
 // over its owned points for every step in a fixed order.
 // sor_batch_kernel runs one batch per launch, reading Psi from one buffer
-// and writing the owned points to another; sor_check_kernel applies the
-// stopping rule.
+// and writing the owned points to another; its extra workgroup (or
+// sor_check_kernel after the last batch) applies the stopping rule.
 template <int SK_, int RS_, int SW_>
 struct SorCfg {
   static constexpr int SK = SK_;            // half-sweeps per batch = halo width
@@ -406,14 +407,83 @@ struct SorArgs {
   const double *Pin;
   double *Pout;
   double *partials;      // [SK][ntiles] of this batch
-  const int64_t *state;  // [0] done
+  int64_t *state;        // [0] done, [1] iterations
   int64_t n0;
   int k;                 // half-sweeps in this batch
   int force;             // run even if done is set (the exact re-run after convergence)
+  // check_k > 0: workgroup ntiles applies the stopping rule to the previous
+  // check group, half-sweeps check_n0 .. check_n0+check_k-1 with partial rows
+  // check_partials[0 .. check_k-1]
+  const double *check_partials;
+  double *sums;          // [0] drive sum, [1] residual of the last checked half-sweep
+  double accuracy;
+  int64_t check_n0;
+  int check_k;
 };
+
+// Total of one half-sweep's tile partials by the 32 lanes l of a
+// half-wavefront: lane l adds tiles l, l+32, ... (8 loads in flight, added in
+// order), then a fixed butterfly; every lane returns the total.  The order
+// depends on ntiles alone, so the stopping rule is deterministic.
+__device__ inline double sor_partial_total(const double *row, int ntiles, int l) {
+  double s = 0;
+  for (int t0 = l; t0 < ntiles; t0 += 32 * 8) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      int t = t0 + 32 * u;
+      x[u] = t < ntiles ? row[t] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += x[u];
+  }
+  for (int off = 16; off > 0; off >>= 1) s += __shfl_xor(s, off, 32);
+  return s;
+}
+
+// The stopping rule (src/poisson.c:125-126) after half-sweeps n0, n0+1, ...
+// in order, on their totals tot[0 .. k-1]; one thread.
+__device__ inline void sor_stop_rule(const double *tot, int k, int64_t n0, double accuracy, int64_t *state,
+                                     double *sums) {
+  for (int jj = 0; jj < k; jj++) {
+    int64_t n = n0 + jj;
+    double total = tot[jj];
+    sums[1] = total;
+    if (n > 0 && (total < accuracy * sums[0] || total == 0)) {
+      state[0] = 1;
+      state[1] = n;
+      return;
+    }
+  }
+}
+
+// all threads of a workgroup of nt threads (a multiple of 32): totals of the
+// k half-sweeps into tot, then the stopping rule
+__device__ inline void sor_group_check(const double *partials, int ntiles, int k, int64_t n0, double accuracy,
+                                       int64_t *state, double *sums, double *tot, int nt) {
+  if (state[0]) return;
+  const int g = threadIdx.x >> 5, l = threadIdx.x & 31;
+  for (int j0 = 0; j0 < k; j0 += nt / 32) {
+    const int j = j0 + g;
+    if (j < k) {
+      const double tj = sor_partial_total(partials + (int64_t)j * ntiles, ntiles, l);
+      if (l == 0) tot[j] = tj;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) sor_stop_rule(tot, k, n0, accuracy, state, sums);
+}
 
 __global__ __launch_bounds__(BatchCfg::NT) void sor_batch_kernel(SorArgs a) {
   using G = BatchCfg;
+  if ((int)blockIdx.x == a.m.ntiles) {
+    // the extra workgroup: stopping rule of the previous check group, on a
+    // CU the tiles leave free (tiles that start after it stops the solve
+    // return at once; their output is never used)
+    __shared__ double tot[CB / 32];
+    sor_group_check(a.check_partials, a.m.ntiles, a.check_k, a.check_n0, a.accuracy, a.state, a.sums, tot, G::NT);
+    return;
+  }
   __shared__ double2 xrow[2][G::SW][2][64];   // [parity][wave][first,last row][lane]
   __shared__ double wsum[G::SK][G::SW];
   __shared__ double cr[G::TRW], dr[G::TRW];
@@ -443,41 +513,10 @@ struct CheckArgs {
   double *sums;     // [0] drive sum, [1] residual of the last checked half-sweep
 };
 
-// applies the stopping rule after each half-sweep of a batch, in order: 32
-// lanes per half-sweep add the tile partials (strided, then a fixed butterfly)
+// the stopping rule of one check group on its own (the last group of a solve)
 __global__ __launch_bounds__(CB) void sor_check_kernel(CheckArgs a) {
   __shared__ double tot[CB / 32];
-  if (a.state[0]) return;
-  const int j = threadIdx.x >> 5, l = threadIdx.x & 31;
-  double s = 0;
-  if (j < a.k) {
-    const double *row = a.partials + (int64_t)j * a.ntiles;
-    // 8 independent loads in flight per lane, then added in order
-    for (int t0 = l; t0 < a.ntiles; t0 += 32 * 8) {
-      double x[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        int t = t0 + 32 * u;
-        x[u] = t < a.ntiles ? row[t] : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; u++) s += x[u];
-    }
-  }
-  for (int off = 16; off > 0; off >>= 1) s += __shfl_xor(s, off, 32);
-  if (l == 0) tot[j] = s;
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  for (int jj = 0; jj < a.k; jj++) {
-    int64_t n = a.n0 + jj;
-    double total = tot[jj];
-    a.sums[1] = total;
-    if (n > 0 && (total < a.accuracy * a.sums[0] || total == 0)) {
-      a.state[0] = 1;
-      a.state[1] = n;
-      return;
-    }
-  }
+  sor_group_check(a.partials, a.ntiles, a.k, a.n0, a.accuracy, a.state, a.sums, tot, CB);
 }
 
 struct QArgs {
@@ -666,7 +705,8 @@ int sor_batch() { return BatchCfg::SK; }
 int sor_check_group() { return kCheckGroup; }
 
 hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const double *rhs, int64_t n0, int k,
-                            int partial_row, int force, hipStream_t s) {
+                            int partial_row, int force, hipStream_t s, int64_t check_row, int64_t check_n0,
+                            int check_k, double accuracy) {
   SorArgs a;
   a.m = sor_common(R, rhs, BatchCfg::OZ, BatchCfg::OR);
   a.Pin = Pin;
@@ -676,13 +716,19 @@ hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const doubl
   a.n0 = n0;
   a.k = k;
   a.force = force;
-  hipLaunchKernelGGL(sor_batch_kernel, dim3((unsigned)a.m.ntiles), dim3(BatchCfg::NT), 0, s, a);
+  a.check_partials = R.red + check_row * a.m.ntiles;
+  a.sums = R.sor_sums;
+  a.accuracy = accuracy;
+  a.check_n0 = check_n0;
+  a.check_k = check_k;
+  const unsigned grid = (unsigned)a.m.ntiles + (check_k > 0 ? 1u : 0u);
+  hipLaunchKernelGGL(sor_batch_kernel, dim3(grid), dim3(BatchCfg::NT), 0, s, a);
   return hipGetLastError();
 }
 
-hipError_t launch_sor_check(Run &R, int64_t n0, int k, double accuracy, hipStream_t s) {
+hipError_t launch_sor_check(Run &R, int64_t check_row, int64_t n0, int k, double accuracy, hipStream_t s) {
   CheckArgs c;
-  c.partials = R.red;
+  c.partials = R.red + check_row * sor_tiles(R);
   c.ntiles = (int)sor_tiles(R);
   c.k = k;
   c.n0 = n0;
