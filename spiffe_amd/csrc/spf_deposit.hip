@@ -193,6 +193,27 @@ __global__ __launch_bounds__(256) void fold_kernel(FoldArgs a) {
   }
 }
 
+// s + v[0] + v[1] + ... + v[15], left to right, each addition rounded (IEEE
+// v_add_f64, the same operation as `s += v`).  Written as one block of
+// dependent additions so nothing is scheduled between them: the chain then
+// runs at the latency of one addition per step while the next block's LDS
+// reads are in flight.
+__device__ inline double add16_in_order(double s, const double2 (&v)[8]) {
+  asm volatile(
+      "v_add_f64 %0, %0, %1\n\tv_add_f64 %0, %0, %2\n\t"
+      "v_add_f64 %0, %0, %3\n\tv_add_f64 %0, %0, %4\n\t"
+      "v_add_f64 %0, %0, %5\n\tv_add_f64 %0, %0, %6\n\t"
+      "v_add_f64 %0, %0, %7\n\tv_add_f64 %0, %0, %8\n\t"
+      "v_add_f64 %0, %0, %9\n\tv_add_f64 %0, %0, %10\n\t"
+      "v_add_f64 %0, %0, %11\n\tv_add_f64 %0, %0, %12\n\t"
+      "v_add_f64 %0, %0, %13\n\tv_add_f64 %0, %0, %14\n\t"
+      "v_add_f64 %0, %0, %15\n\tv_add_f64 %0, %0, %16"
+      : "+v"(s)
+      : "v"(v[0].x), "v"(v[0].y), "v"(v[1].x), "v"(v[1].y), "v"(v[2].x), "v"(v[2].y), "v"(v[3].x), "v"(v[3].y),
+        "v"(v[4].x), "v"(v[4].y), "v"(v[5].x), "v"(v[5].y), "v"(v[6].x), "v"(v[6].y), "v"(v[7].x), "v"(v[7].y));
+  return s;
+}
+
 // One wavefront per long segment.  The wavefront loads the segment in
 // coalesced chunks of LCH values into registers, one chunk ahead, and parks
 // each chunk in LDS; lane 0 adds the parked chunk in order while the next
@@ -228,31 +249,29 @@ __global__ __launch_bounds__(64) void fold_long_kernel(FoldArgs a) {
         v[u] = i < e0 ? a.sv[i] : 0.0;
       }
       if (lane == 0) {
-        // 16 parked values per step, the next 16 already being read from LDS
+        // 32 parked values per round from two register sets: while one set's
+        // 16 additions run, the other set is being refilled from LDS
         const int m = (int)(e0 - c0 < LCH ? e0 - c0 : LCH);
         const double2 *b2 = reinterpret_cast<const double2 *>(buf);
         int i = 0;
-        if (m >= 16) {
-          double2 cur[8], nxt[8];
+        if (m >= 32) {
+          double2 A[8], B[8];
 #pragma unroll
-          for (int k = 0; k < 8; k++) cur[k] = b2[k];
-          for (; i + 32 <= m; i += 16) {
+          for (int k = 0; k < 8; k++) A[k] = b2[k];
 #pragma unroll
-            for (int k = 0; k < 8; k++) nxt[k] = b2[(i + 16) / 2 + k];
+          for (int k = 0; k < 8; k++) B[k] = b2[8 + k];
+          // A holds values i..i+15, B holds i+16..i+31, i + 32 <= m
+          for (; i + 64 <= m; i += 32) {
+            s = add16_in_order(s, A);
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-              s += cur[k].x;
-              s += cur[k].y;
-            }
+            for (int k = 0; k < 8; k++) A[k] = b2[(i + 32) / 2 + k];
+            s = add16_in_order(s, B);
 #pragma unroll
-            for (int k = 0; k < 8; k++) cur[k] = nxt[k];
+            for (int k = 0; k < 8; k++) B[k] = b2[(i + 48) / 2 + k];
           }
-#pragma unroll
-          for (int k = 0; k < 8; k++) {
-            s += cur[k].x;
-            s += cur[k].y;
-          }
-          i += 16;
+          s = add16_in_order(s, A);
+          s = add16_in_order(s, B);
+          i += 32;
         }
         for (; i < m; i++) s += buf[i];
       }
