@@ -42,7 +42,7 @@ void free_run(Run *R) {
   F(R->last_old); F(R->evlist); F(R->move_src); F(R->move_dst); F(R->srows); F(R->lrows);
   F(R->keys); F(R->skeys); F(R->vals); F(R->cz); F(R->scz); F(R->gstart); F(R->long_list); F(R->pcount); F(R->gflag); F(R->sort_tmp);
   F(R->bcount); F(R->boff); F(R->stage); F(R->append_stage); F(R->emit_stage); F(R->Pdev); F(R->omega); F(R->red);
-  F(R->red_counter); F(R->sor_state); F(R->sor_sums);
+  F(R->red_counter); F(R->sor_state); F(R->sor_sums); F(R->sor_masks);
   R->live = false;
 }
 
@@ -278,6 +278,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   R->red_n = std::max<int64_t>(sor_tiles(*R) * sor_batch() * 4 * 2, 512) + 16;   // two check groups of partials
   ALLOC(R->red, R->red_n);
   ALLOC(R->red_counter, 4); ALLOC(R->sor_state, 4); ALLOC(R->sor_sums, 8);
+  ALLOC(R->sor_masks, sor_mask_words(*R));
   ALLOC(R->Pdev, 16);
 #undef ALLOC
   hipStream_t s = c->stream;

@@ -71,6 +71,7 @@ int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accura
   if (rc) return rc;
   hipStream_t s = c->stream;
   HIPCALL(c, launch_sor_reset(*R, s));
+  HIPCALL(c, launch_sor_masks(*R, s));
   if (rhs)
     HIPCALL(c, launch_sum_abs(rhs, R->ldz, R->d.nz, R->d.nr - 1, R->red, R->red_n, R->sor_sums, s));
   const int K = sor_batch(), KC = sor_check_group();
@@ -183,6 +184,7 @@ int spf_poisson_sweeps(spf_ctx *c, int32_t run, int32_t which, int64_t n0, int64
   double **Psi = which == 0 ? &R->Psi : &R->PsiImp;
   const double *rhs = which == 0 ? R->Q : nullptr;
   const int K = sor_batch();
+  HIPCALL(c, launch_sor_masks(*R, c->stream));
   double *buf[2] = {*Psi, R->PsiSpare[0]};
   int64_t nb = (count + K - 1) / K;
   {

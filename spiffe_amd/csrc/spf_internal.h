@@ -158,6 +158,7 @@ struct Run {
   int64_t red_n = 0;
   int32_t *red_counter = nullptr;
   int64_t *sor_state = nullptr; // [0] done, [1] iterations
+  uint32_t *sor_masks = nullptr; // per-tile lane masks of the SOR batches
   double *sor_sums = nullptr;   // [0] drive sum, [1] last residual sum, [2..] misc
   double last_field_factor = -1;
   std::vector<double> screen_z;
@@ -214,6 +215,8 @@ hipError_t launch_sum_plain(const double *a, int ldz, int iz_n, int ir_n, double
 hipError_t launch_sor_reset(Run &R, hipStream_t s);
 int64_t sor_tiles(const Run &R);
 int sor_check_group();   // batches per stopping-rule check (<= 4)
+int64_t sor_mask_words(const Run &R);
+hipError_t launch_sor_masks(Run &R, hipStream_t s);   // before the batches of a solve
 int sor_batch();
 
 // check_k > 0: the launch carries one extra workgroup that applies the

@@ -50,7 +50,7 @@ struct SorCfg {
   static_assert(OZ % 2 == 0 && OR % 2 == 0 && SK % 2 == 0 && RS % 2 == 0,
                 "tile origins and lane rows must be even so each row's colour column is fixed");
   static_assert(OR >= SK && OZ >= SK, "a halo must come from the adjacent tiles only");
-  static_assert(2 * RS <= 32, "one mask bit per point");
+  static_assert(2 * RS <= 16, "one mask bit per point, two masks in a word");
   // mask bits of the lane's even (column 0) and odd (column 1) points
   static constexpr unsigned COL0 = 0x55555555u & ((1u << (2 * RS)) - 1u);
   static constexpr unsigned COL1 = 0xAAAAAAAAu & ((1u << (2 * RS)) - 1u);
@@ -134,6 +134,7 @@ struct SorCommon {
   const double *omega;   // per-half-sweep omega
   const double *rhs;     // may be null
   const int16_t *flags;
+  const uint32_t *masks;  // [ntiles][NT]: update mask | ownership mask << 16, per lane
   int tiles_z, ntiles;
 };
 
@@ -194,31 +195,58 @@ __device__ inline void sor_load_P(const SorCommon &a, const SorTile<G> &t, int w
   }
 }
 
-// right-hand side, masks and row coefficients (cr/dr in LDS, TRW each);
-// without a right-hand side the loads read src (same layout) and are discarded
+// Update and ownership masks of this lane's points (bit 2*i+col), from the
+// flags and the tile geometry: a point is updated in the array interior and
+// not metal (src/poisson.c:58-120), owned in the tile's owned region.  They
+// do not change during a solve, so sor_masks_kernel stores them once per
+// solve and the batches load one word per lane.
 template <class G>
-__device__ inline void sor_setup(const SorCommon &a, const SorTile<G> &t, int w, int lane, SorLane<G> &ln,
+__device__ inline void sor_lane_masks(const SorCommon &a, const SorTile<G> &t, int w, int lane, unsigned &upd_m,
+                                      unsigned &own_m) {
+  const int iz_e = t.zlo + 2 * lane;
+  upd_m = 0;
+  own_m = 0;
+#pragma unroll
+  for (int i = 0; i < G::RS; i++) {
+    const int lr = w * G::RS + i, ir = t.rlo + lr;
+    const uint32_t f2 = *reinterpret_cast<const uint32_t *>(a.flags + sor_pair(a, iz_e, ir));
+#pragma unroll
+    for (int col = 0; col < 2; col++) {
+      const int iz = iz_e + col;
+      const int16_t fl = (int16_t)(col ? (f2 >> 16) : (f2 & 0xffffu));
+      const bool in = lr < t.LR && 2 * lane + col < t.LZ;
+      const bool in_range =
+          (iz >= 1 && iz <= a.nz - 2) || (iz == 0 && a.left_dir) || (iz == a.nz - 1 && a.right_dir);
+      const bool upd = in && ir <= a.nr - 2 && iz <= a.nz - 1 && in_range &&
+                       !(fl & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE));
+      if (upd) upd_m |= 1u << (2 * i + col);
+      if (iz >= t.z0 && iz < t.zo1 && ir >= t.r0 && ir < t.ro1) own_m |= 1u << (2 * i + col);
+    }
+  }
+}
+
+// right-hand side, masks (one stored word per lane) and row coefficients
+// (cr/dr in LDS, TRW each); without a right-hand side the loads read src
+// (same layout) and are discarded
+template <class G>
+__device__ inline void sor_setup(const SorCommon &a, const SorTile<G> &t, int tile, int w, int lane, SorLane<G> &ln,
                                  const double *src, double *cr, double *dr) {
   const int iz_e = t.zlo + 2 * lane;
   const double *rhs = a.rhs ? a.rhs : src;
-  int16_t fl[G::RS][2];
+  const uint32_t m = a.masks[(int64_t)tile * G::NT + threadIdx.x];
 #pragma unroll
   for (int i = 0; i < G::RS; i++) {
-    const int64_t p = sor_pair(a, iz_e, t.rlo + w * G::RS + i);
-    const double2 v = *reinterpret_cast<const double2 *>(rhs + p);
-    const uint32_t f2 = *reinterpret_cast<const uint32_t *>(a.flags + p);
+    const double2 v = *reinterpret_cast<const double2 *>(rhs + sor_pair(a, iz_e, t.rlo + w * G::RS + i));
     ln.D[i][0] = a.rhs ? v.x : 0.0;
     ln.D[i][1] = a.rhs ? v.y : 0.0;
-    fl[i][0] = (int16_t)(f2 & 0xffffu);
-    fl[i][1] = (int16_t)(f2 >> 16);
   }
   for (int k = threadIdx.x; k < G::TRW; k += G::NT) {
     const int irc = min(t.rlo + k, a.nr);
     cr[k] = a.c[irc];
     dr[k] = a.d[irc];
   }
-  ln.upd = 0;
-  ln.own = 0;
+  ln.upd = m & 0xffffu;
+  ln.own = m >> 16;
 #pragma unroll
   for (int col = 0; col < 2; col++) {
     const int lz = 2 * lane + col;
@@ -226,21 +254,10 @@ __device__ inline void sor_setup(const SorCommon &a, const SorTile<G> &t, int w,
     ln.r_mirror[col] = iz_e + col == a.nz - 1;
   }
 #pragma unroll
-  for (int i = 0; i < G::RS; i++) {
-    const int lr = w * G::RS + i, ir = t.rlo + lr;
+  for (int i = 0; i < G::RS; i++)
 #pragma unroll
-    for (int col = 0; col < 2; col++) {
-      const int iz = iz_e + col;
-      const bool in = lr < t.LR && 2 * lane + col < t.LZ;
-      const bool in_range =
-          (iz >= 1 && iz <= a.nz - 2) || (iz == 0 && a.left_dir) || (iz == a.nz - 1 && a.right_dir);
-      const bool upd = in && ir <= a.nr - 2 && iz <= a.nz - 1 && in_range &&
-                       !(fl[i][col] & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE));
-      if (upd) ln.upd |= 1u << (2 * i + col);
-      if (iz >= t.z0 && iz < t.zo1 && ir >= t.r0 && ir < t.ro1) ln.own |= 1u << (2 * i + col);
-      if (!upd) ln.D[i][col] = 0.0;
-    }
-  }
+    for (int col = 0; col < 2; col++)
+      if (!((ln.upd >> (2 * i + col)) & 1u)) ln.D[i][col] = 0.0;
 }
 
 // One half-sweep of colour C on the registers of this lane; AX: this
@@ -496,12 +513,22 @@ __global__ __launch_bounds__(BatchCfg::NT) void sor_batch_kernel(SorArgs a) {
   if (threadIdx.x < G::SK) om[threadIdx.x] = a.m.omega[a.n0 + min((int)threadIdx.x, a.k - 1)];
   SorLane<G> ln;
   sor_load_P<G>(a.m, t, w, lane, a.Pin, ln.P);
-  sor_setup<G>(a.m, t, w, lane, ln, a.Pin, cr, dr);
+  sor_setup<G>(a.m, t, tile, w, lane, ln, a.Pin, cr, dr);
   if (done) return;
   double ares[G::SK];
   sor_sweeps<G>(a.m, t, ln, w, lane, a.n0, a.k, om, xrow, cr, dr, ares);
   sor_block_sums<G>(ares, a.k, w, lane, wsum, a.partials + tile, a.m.ntiles);
   sor_write_owned<G>(a.m, t, ln, w, lane, a.Pout);
+}
+
+// the lanes' update and ownership masks of every tile, once per solve
+__global__ __launch_bounds__(BatchCfg::NT) void sor_masks_kernel(SorCommon a, uint32_t *out) {
+  using G = BatchCfg;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const SorTile<G> t(a, (int)blockIdx.x);
+  unsigned upd, own;
+  sor_lane_masks<G>(a, t, w, lane, upd, own);
+  out[(int64_t)blockIdx.x * G::NT + threadIdx.x] = upd | (own << 16);
 }
 
 struct CheckArgs {
@@ -688,6 +715,7 @@ SorCommon sor_common(Run &R, const double *rhs, int oz, int orr) {
   a.omega = R.omega;
   a.rhs = rhs;
   a.flags = R.flags;
+  a.masks = R.sor_masks;
   a.tiles_z = (R.d.nz + 1 + oz - 1) / oz;
   a.ntiles = a.tiles_z * ((R.d.nr + 1 + orr - 1) / orr);
   return a;
@@ -701,6 +729,14 @@ int64_t sor_tiles(const Run &R) {
 }
 
 int sor_batch() { return BatchCfg::SK; }
+
+int64_t sor_mask_words(const Run &R) { return sor_tiles(R) * BatchCfg::NT; }
+
+hipError_t launch_sor_masks(Run &R, hipStream_t s) {
+  SorCommon a = sor_common(R, nullptr, BatchCfg::OZ, BatchCfg::OR);
+  hipLaunchKernelGGL(sor_masks_kernel, dim3((unsigned)a.ntiles), dim3(BatchCfg::NT), 0, s, a, R.sor_masks);
+  return hipGetLastError();
+}
 
 int sor_check_group() { return kCheckGroup; }
 
