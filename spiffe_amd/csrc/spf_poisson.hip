@@ -79,17 +79,17 @@ __device__ double block_sum_det(double v, double *wsum) {
   return t;
 }
 
-// lane l receives lane l-1's value (wave_shr:1); lane 0 keeps `edge`
-__device__ inline double from_left(double v, double edge) {
-  int lo = __builtin_amdgcn_update_dpp(__double2loint(edge), __double2loint(v), 0x138, 0xF, 0xF, false);
-  int hi = __builtin_amdgcn_update_dpp(__double2hiint(edge), __double2hiint(v), 0x138, 0xF, 0xF, false);
+// lane l receives lane l-1's value (wave_shr:1); lane 0 receives 0
+__device__ inline double from_left(double v) {
+  int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xF, 0xF, true);
+  int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 
-// lane l receives lane l+1's value (wave_shl:1); lane 63 keeps `edge`
-__device__ inline double from_right(double v, double edge) {
-  int lo = __builtin_amdgcn_update_dpp(__double2loint(edge), __double2loint(v), 0x130, 0xF, 0xF, false);
-  int hi = __builtin_amdgcn_update_dpp(__double2hiint(edge), __double2hiint(v), 0x130, 0xF, 0xF, false);
+// lane l receives lane l+1's value (wave_shl:1); lane 63 receives 0
+__device__ inline double from_right(double v) {
+  int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xF, 0xF, true);
+  int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 
@@ -265,11 +265,16 @@ __device__ inline void sor_setup(const SorCommon &a, const SorTile<G> &t, int ti
 // tile row w*RS + i, whose parity is the parity of i (rlo is even); its
 // colour-C point is column (i + C) & 1 of the lane's pair.  Returns the sum of
 // |residual| over the lane's updated owned points.
-template <class G, int C>
+// EDGE: the tile touches the array's left (ledge) or right (redge) z end and
+// needs the mirrors; AXM: 0 the wavefront does not hold the axis row, 1 it
+// does, 2 decided by AX at run time.
+template <class G, int C, bool EDGE, int AXM>
 __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsigned okb, const double2 up_row,
                                         const double2 dn_row, const double *cr, const double *dr, int w,
-                                        double omega, bool redge, bool AX) {
+                                        double omega, bool ledge, bool redge, bool AX) {
   const double A = a.a, E = a.e;
+  const bool ax = AXM == 2 ? AX : AXM == 1;
+  const bool lane0 = (threadIdx.x & 63) == 0;
   double q[G::RS], x[G::RS], res[G::RS];
   bool slow = false;
 #pragma unroll
@@ -279,12 +284,15 @@ __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsi
     double L, R;
     if (col == 0) {
       R = ln.P[i][1];
-      L = from_left(R, R);   // lane 0 holds iz == 0 when the tile starts at the array edge: mirror
-      if (redge) R = ln.r_mirror[0] ? L : R;
+      L = from_left(R);
+      // lane 0's column 0 is updated only as iz == 0 of a tile at the left
+      // end, whose left neighbour is the mirror of the right one
+      if (EDGE && ledge) L = lane0 ? R : L;
+      if (EDGE && redge) R = ln.r_mirror[0] ? L : R;
     } else {
       L = ln.P[i][0];
-      R = from_right(L, L);  // lane 63 at iz == nz-1 mirrors; elsewhere it is never updated
-      if (redge) R = ln.r_mirror[1] ? L : R;
+      R = from_right(L);   // lane 63's column 1 is never updated except as iz == nz-1, mirrored here
+      if (EDGE && redge) R = ln.r_mirror[1] ? L : R;
     }
     const double zs = R + L;
     const double up = i < G::RS - 1 ? ln.P[i + 1 < G::RS ? i + 1 : i][col] : (col ? up_row.y : up_row.x);
@@ -294,10 +302,10 @@ __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsi
       // row 0 of the tile may be the axis row, src/poisson.c:84-90: no "below"
       // term (added as -0.0, an exact no-op), a 4*c "above" weight, diagonal e - 2*c
       const double down = col ? dn_row.y : dn_row.x;
-      const double cu = AX ? 4 * c : c;
-      const double dn = AX ? -0.0 : dr[lr] * down;
-      diag = AX ? E - 2 * c : E;
-      y = AX ? a.y_axis : a.y_e;
+      const double cu = ax ? 4 * c : c;
+      const double dn = ax ? -0.0 : dr[lr] * down;
+      diag = ax ? E - 2 * c : E;
+      y = ax ? a.y_axis : a.y_e;
       residual = A * zs + cu * up + dn + diag * ln.P[i][col] - ln.D[i][col];
     } else {
       const double down = i > 0 ? ln.P[i > 0 ? i - 1 : 0][col] : (col ? dn_row.y : dn_row.x);
@@ -313,7 +321,7 @@ __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsi
   if (slow || !a.use_y) {
 #pragma unroll
     for (int i = 0; i < G::RS; i++) {
-      const double diag = (AX && i == 0) ? E - 2 * cr[w * G::RS] : E;
+      const double diag = (ax && i == 0) ? E - 2 * cr[w * G::RS] : E;
       if (!a.use_y || !div_fast_ok(x[i])) q[i] = x[i] / diag;
     }
   }
@@ -331,7 +339,7 @@ __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsi
 
 // One local step j of a batch, colour C, relaxation factor omega; returns the
 // lane's owned residual sum.
-template <class G, int C>
+template <class G, int C, bool EDGE, int AXM>
 __device__ inline double sor_step(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane,
                                   double omega, int j, double2 (*xrow)[G::SW][2][64], const double *cr,
                                   const double *dr, bool axis_wave) {
@@ -347,49 +355,66 @@ __device__ inline double sor_step(const SorCommon &a, const SorTile<G> &t, SorLa
   const int lo = min(max(vr0 - w * G::RS, 0), G::RS), hi = min(max(vr1 - w * G::RS, 0), G::RS);
   const unsigned rmask = hi > lo ? (((1u << (2 * hi)) - 1u) & ~((1u << (2 * lo)) - 1u)) : 0u;
   const unsigned okb = ln.upd & rmask & ((ln.zdepth[0] > j ? G::COL0 : 0u) | (ln.zdepth[1] > j ? G::COL1 : 0u));
-  return sor_half_sweep<G, C>(a, ln, okb, up_row, dn_row, cr, dr, w, omega, t.redge, axis_wave);
+  return sor_half_sweep<G, C, EDGE, AXM>(a, ln, okb, up_row, dn_row, cr, dr, w, omega, t.zlo == 0, t.redge,
+                                         axis_wave);
+}
+
+// Steps j = 0 .. k-1 of an even n0 in colour pairs (0, 1): one copy of the
+// pair loop per tile kind, so interior tiles carry no mirror selects and
+// wavefronts off the axis no axis terms.  lres[j][thread] gets the lane's
+// owned residual sum of step j.
+template <class G, bool EDGE, int AXM>
+__device__ inline void sor_pairs(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane, int k,
+                                 const double *om, double2 (*xrow)[G::SW][2][64], const double *cr,
+                                 const double *dr, double (*lres)[G::NT]) {
+#pragma unroll 1
+  for (int j = 0; j < k; j += 2) {
+    lres[j][threadIdx.x] = sor_step<G, 0, EDGE, AXM>(a, t, ln, w, lane, om[j], j, xrow, cr, dr, AXM == 1);
+    if (j + 1 < k)
+      lres[j + 1][threadIdx.x] = sor_step<G, 1, EDGE, AXM>(a, t, ln, w, lane, om[j + 1], j + 1, xrow, cr, dr,
+                                                           AXM == 1);
+  }
 }
 
 // k <= SK half-sweeps n0 .. n0+k-1 with relaxation factors om[0 .. k-1] (LDS,
-// read after each step's barrier);
-// ares[j] gets the lane's owned residual sum of step j.  For an even n0 (every
-// batch of a solve) the steps are unrolled with their colours 0, 1, 0, ...
-// known at compile time.
+// read after each step's barrier); lres[j][thread] gets the lane's owned
+// residual sum of step j.  An odd n0 (only from spf_poisson_sweeps) runs the
+// steps one by one with run-time colours.
 template <class G>
 __device__ inline void sor_sweeps(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane,
                                   int64_t n0, int k, const double *om, double2 (*xrow)[G::SW][2][64],
-                                  const double *cr, const double *dr, double (&ares)[G::SK]) {
-#pragma unroll
-  for (int j = 0; j < G::SK; j++) ares[j] = 0;
+                                  const double *cr, const double *dr, double (*lres)[G::NT]) {
   const bool axis_wave = t.rlo == 0 && w == 0;
+  const bool edge = t.zlo == 0 || t.redge;
   if ((n0 & 1) == 0) {
-#pragma unroll
-    for (int j = 0; j < G::SK; j++) {
-      if (j >= k) break;
-      ares[j] = (j & 1) ? sor_step<G, 1>(a, t, ln, w, lane, om[j], j, xrow, cr, dr, axis_wave)
-                        : sor_step<G, 0>(a, t, ln, w, lane, om[j], j, xrow, cr, dr, axis_wave);
+    // the tile kind is uniform over the workgroup, the axis flag over the wavefront
+    if (edge) {
+      if (axis_wave)
+        sor_pairs<G, true, 1>(a, t, ln, w, lane, k, om, xrow, cr, dr, lres);
+      else
+        sor_pairs<G, true, 0>(a, t, ln, w, lane, k, om, xrow, cr, dr, lres);
+    } else {
+      if (axis_wave)
+        sor_pairs<G, false, 1>(a, t, ln, w, lane, k, om, xrow, cr, dr, lres);
+      else
+        sor_pairs<G, false, 0>(a, t, ln, w, lane, k, om, xrow, cr, dr, lres);
     }
   } else {
 #pragma unroll 1
-    for (int j = 0; j < k; j++) {
-      const double omega = om[j];
-      const double acc = ((n0 + j) & 1) ? sor_step<G, 1>(a, t, ln, w, lane, omega, j, xrow, cr, dr, axis_wave)
-                                        : sor_step<G, 0>(a, t, ln, w, lane, omega, j, xrow, cr, dr, axis_wave);
-#pragma unroll
-      for (int jj = 0; jj < G::SK; jj++)
-        if (jj == j) ares[jj] = acc;
-    }
+    for (int j = 0; j < k; j++)
+      lres[j][threadIdx.x] =
+          ((n0 + j) & 1) ? sor_step<G, 1, true, 2>(a, t, ln, w, lane, om[j], j, xrow, cr, dr, axis_wave)
+                         : sor_step<G, 0, true, 2>(a, t, ln, w, lane, om[j], j, xrow, cr, dr, axis_wave);
   }
 }
 
 // the tile's owned residual sum of each step j < k into out[j * stride]:
 // butterflies within each wavefront, then the wavefronts in order
 template <class G>
-__device__ inline void sor_block_sums(const double (&ares)[G::SK], int k, int w, int lane, double (*wsum)[G::SW],
+__device__ inline void sor_block_sums(double (*lres)[G::NT], int k, int w, int lane, double (*wsum)[G::SW],
                                       double *out, int64_t stride) {
-#pragma unroll
-  for (int j = 0; j < G::SK; j++) {
-    double v = ares[j];
+  for (int j = 0; j < k; j++) {
+    double v = lres[j][threadIdx.x];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if (lane == 0) wsum[j][w] = v;
   }
@@ -515,9 +540,9 @@ __global__ __launch_bounds__(BatchCfg::NT) void sor_batch_kernel(SorArgs a) {
   sor_load_P<G>(a.m, t, w, lane, a.Pin, ln.P);
   sor_setup<G>(a.m, t, tile, w, lane, ln, a.Pin, cr, dr);
   if (done) return;
-  double ares[G::SK];
-  sor_sweeps<G>(a.m, t, ln, w, lane, a.n0, a.k, om, xrow, cr, dr, ares);
-  sor_block_sums<G>(ares, a.k, w, lane, wsum, a.partials + tile, a.m.ntiles);
+  __shared__ double lres[G::SK][G::NT];
+  sor_sweeps<G>(a.m, t, ln, w, lane, a.n0, a.k, om, xrow, cr, dr, lres);
+  sor_block_sums<G>(lres, a.k, w, lane, wsum, a.partials + tile, a.m.ntiles);
   sor_write_owned<G>(a.m, t, ln, w, lane, a.Pout);
 }
 
