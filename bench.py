@@ -219,11 +219,13 @@ def main():
     share = {"fdtd_kernel": per_step("fdtd"), "push phase": per_step("push"), "deposit phase": per_step("deposit"),
              "sor_batch_kernel": per_step("sor_batch")}
     cells_all = (g.nz + 1) * (g.nr + 1)
+    hs = ctx.lib.spf_sor_batch_half_sweeps()   # half-sweeps per SOR batch launch
     npk = 0.5 * (np0 + np1)
     algo = {  # algorithmic bytes per launch and measured ms per launch
         "fdtd_kernel": (cells * B_CELL_TMTE, t_fdtd, "%d cells x %d B (TM+TE cell-update)" % (cells, B_CELL_TMTE)),
-        "sor_batch_kernel": (8 * cells_all * B_SOR, t_sor,
-                             "8 half-sweeps x %d points x %d B (SOR point of one half-sweep)" % (cells_all, B_SOR)),
+        "sor_batch_kernel": (hs * cells_all * B_SOR, t_sor,
+                             "%d half-sweeps x %d points x %d B (SOR point of one half-sweep)"
+                             % (hs, cells_all, B_SOR)),
         "push phase": (npk * B_PUSH, t_push, "%d particles x %d B (particle-push)" % (npk, B_PUSH)),
     }
     dominant = max((k for k in share if k in algo and algo[k][1]), key=share.get)

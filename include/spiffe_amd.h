@@ -363,8 +363,9 @@ int spf_sdds_parameter(const char *path, int64_t page, const char *name, double 
 #define SPF_T_FDTD 2       /* spf_fields_advance */
 #define SPF_T_PUSH 3       /* spf_particles_push: push, compaction and event rows */
 #define SPF_T_POISSON 4    /* spf_poisson_correct, when it solves */
-#define SPF_T_SOR_BATCH 5  /* SOR batch kernel launches (up to 8 half-sweeps each), timed per
-                              group of back-to-back launches between stopping-rule checks */
+#define SPF_T_SOR_BATCH 5  /* SOR batch kernel launches (up to spf_sor_batch_half_sweeps()
+                              half-sweeps each), timed per group of back-to-back launches
+                              between stopping-rule checks */
 #define SPF_T_N 6
 /* every > 0 starts timing one call in `every` of each phase and clears the
  * accumulators; 0 stops it */
@@ -372,6 +373,9 @@ int spf_timing_enable(spf_ctx *ctx, int32_t every);
 /* synchronizing: total milliseconds over the timed launches, the number of timed
  * launches, and (may be NULL) all launches since timing started; arrays of SPF_T_N */
 int spf_timing_read(spf_ctx *ctx, double *ms, int64_t *timed, int64_t *launched);
+/* half-sweeps one SOR batch launch performs (a build-time constant of the
+ * library), for converting SPF_T_SOR_BATCH times into per-half-sweep rates */
+int32_t spf_sor_batch_half_sweeps(void);
 
 /* ---- one whole step of the loop body, src/integrate.c:134-194 ----
  * ramped imposed E (if set); time <- step*dt+start; emission from every

@@ -143,6 +143,8 @@ int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accura
 
 extern "C" {
 
+int32_t spf_sor_batch_half_sweeps(void) { return spf::sor_batch(); }
+
 int spf_poisson_setup(spf_ctx *c, int32_t run, const spf_poisson_params *p) {
   Run *R = get_run(c, run);
   if (!R || !p) return fail(c, SPF_EINVAL, "bad run or null parameters");

@@ -56,9 +56,10 @@ struct SorCfg {
   static constexpr unsigned COL1 = 0xAAAAAAAAu & ((1u << (2 * RS)) - 1u);
 };
 // tile shape (half-sweeps per batch, rows per lane, wavefronts), fixed at
-// build time; the default owns 112 x 20 points with 6 wavefronts
+// build time; the default runs 12 half-sweeps per batch on 104 x 24 owned
+// points with 8 wavefronts (48 tile rows)
 #ifndef SPF_SOR_SHAPE
-#define SPF_SOR_SHAPE 8, 6, 6
+#define SPF_SOR_SHAPE 12, 6, 8
 #endif
 using BatchCfg = SorCfg<SPF_SOR_SHAPE>;
 constexpr int CB = 1024;              // check kernel: 32 lanes per half-sweep, 32 half-sweeps
