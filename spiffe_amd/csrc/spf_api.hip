@@ -285,7 +285,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   // flags to the device layout, and per-row tables computed from them
   std::vector<int16_t> hf(d->metal_flags, d->metal_flags + (size_t)(nz + 1) * (nr + 1));
   {
-    hipError_t e = hipMemcpy(R->stage, hf.data(), hf.size() * sizeof(int16_t), hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpyAsync(R->stage, hf.data(), hf.size() * sizeof(int16_t), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = launch_transpose_in_i16((const int16_t *)R->stage, R->flags, nz, nr, R->ldz, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return bail(e, "flags upload");
@@ -357,7 +357,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   {
     hipError_t e = hipSuccess;
     auto up = [&](void *dst, const void *src, size_t bytes) {
-      if (e == hipSuccess) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
     };
     up(R->cut0, c0.data(), c0.size() * 4);
     up(R->cut1, c1.data(), c1.size() * 4);
@@ -369,6 +369,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
     up(R->sor_d, sd.data(), sd.size() * 8);
     up(R->volume, vol.data(), vol.size() * 8);
     up(R->Pdev, R->P, sizeof(R->P));
+    if (e == hipSuccess) e = hipStreamSynchronize(s);   // the host tables go out of scope below
     if (e != hipSuccess) return bail(e, "table upload");
   }
   // reuse a free slot
@@ -428,8 +429,9 @@ int spf_field_upload(spf_ctx *c, int32_t run, int32_t field, const double *host)
   double *dst = field_ptr(R, field);
   if (!dst) return fail(c, SPF_EINVAL, "unknown field id");
   size_t n = (size_t)(R->d.nz + 1) * (R->d.nr + 1);
-  HIPCALL(c, hipStreamSynchronize(c->stream));
-  HIPCALL(c, hipMemcpy(R->stage, host, n * sizeof(double), hipMemcpyHostToDevice));
+  // every copy is ordered on the run's stream (never the null stream, which
+  // the context's non-blocking stream does not wait for)
+  HIPCALL(c, hipMemcpyAsync(R->stage, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCALL(c, launch_transpose_in(R->stage, dst, R->d.nz, R->d.nr, R->ldz, c->stream));
   HIPCALL(c, hipStreamSynchronize(c->stream));
   if (field == SPF_EZ_IMPOSED || field == SPF_ER_IMPOSED) R->imposed_e = true;
@@ -443,8 +445,8 @@ int spf_field_download(spf_ctx *c, int32_t run, int32_t field, double *host) {
   if (!src) return fail(c, SPF_EINVAL, "unknown field id");
   size_t n = (size_t)(R->d.nz + 1) * (R->d.nr + 1);
   HIPCALL(c, launch_transpose_out(src, R->stage, R->d.nz, R->d.nr, R->ldz, c->stream));
+  HIPCALL(c, hipMemcpyAsync(host, R->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCALL(c, hipStreamSynchronize(c->stream));
-  HIPCALL(c, hipMemcpy(host, R->stage, n * sizeof(double), hipMemcpyDeviceToHost));
   return SPF_OK;
 }
 
@@ -631,11 +633,13 @@ int spf_particles_download(spf_ctx *c, int32_t run, const spf_particles *h, int6
   double *cols[16] = {h->Q, h->z, h->r, h->pz, h->pr, h->r_pphi, h->vz, h->vr, h->vphi,
                       h->r0, h->z0, h->t0, h->rmax, h->rmin, h->zmax, h->zmin};
   if (n > 0) {
+    hipStream_t s = c->stream;
     for (int j = 0; j < 16; j++)
-      if (cols[j]) HIPCALL(c, hipMemcpy(cols[j], R->P[j], sizeof(double) * n, hipMemcpyDeviceToHost));
-    if (h->particleID) HIPCALL(c, hipMemcpy(h->particleID, R->pid, 4 * n, hipMemcpyDeviceToHost));
-    if (h->generation) HIPCALL(c, hipMemcpy(h->generation, R->gen, 2 * n, hipMemcpyDeviceToHost));
-    if (h->status) HIPCALL(c, hipMemcpy(h->status, R->status, 4 * n, hipMemcpyDeviceToHost));
+      if (cols[j]) HIPCALL(c, hipMemcpyAsync(cols[j], R->P[j], sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (h->particleID) HIPCALL(c, hipMemcpyAsync(h->particleID, R->pid, 4 * n, hipMemcpyDeviceToHost, s));
+    if (h->generation) HIPCALL(c, hipMemcpyAsync(h->generation, R->gen, 2 * n, hipMemcpyDeviceToHost, s));
+    if (h->status) HIPCALL(c, hipMemcpyAsync(h->status, R->status, 4 * n, hipMemcpyDeviceToHost, s));
+    HIPCALL(c, hipStreamSynchronize(s));
   }
   if (n_out) *n_out = n;
   return SPF_OK;
@@ -766,8 +770,8 @@ int spf_events_drain(spf_ctx *c, int32_t run, spf_screen_row *rows, int64_t max_
   int32_t ovf = (int32_t)(c->pinned[8] & 0xffffffff);
   int64_t cs = ns < R->srow_cap ? ns : R->srow_cap, cl = nl < R->lrow_cap ? nl : R->lrow_cap;
   int64_t ks = rows ? (cs < max_rows ? cs : max_rows) : 0, kl = lost ? (cl < max_lost ? cl : max_lost) : 0;
-  if (ks > 0) HIPCALL(c, hipMemcpy(rows, R->srows, sizeof(spf_screen_row) * ks, hipMemcpyDeviceToHost));
-  if (kl > 0) HIPCALL(c, hipMemcpy(lost, R->lrows, sizeof(spf_lost_row) * kl, hipMemcpyDeviceToHost));
+  if (ks > 0) HIPCALL(c, hipMemcpyAsync(rows, R->srows, sizeof(spf_screen_row) * ks, hipMemcpyDeviceToHost, c->stream));
+  if (kl > 0) HIPCALL(c, hipMemcpyAsync(lost, R->lrows, sizeof(spf_lost_row) * kl, hipMemcpyDeviceToHost, c->stream));
   if (n_rows) *n_rows = ns;
   if (n_lost) *n_lost = nl;
   c->pinned[3] = 0;

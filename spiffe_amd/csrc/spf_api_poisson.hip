@@ -33,7 +33,8 @@ int ensure_omega(spf_ctx *c, Run *R, int64_t n) {
   if (R->omega) HIPCALL(c, hipFree(R->omega));
   R->omega = nullptr;
   HIPCALL(c, hipMalloc((void **)&R->omega, sizeof(double) * (size_t)n));
-  HIPCALL(c, hipMemcpy(R->omega, w.data(), sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+  HIPCALL(c, hipMemcpyAsync(R->omega, w.data(), sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
   R->omega_n = n;
   return SPF_OK;
 }
