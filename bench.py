@@ -34,7 +34,7 @@ B_SOR = 22.0             # SOR point of one half-sweep: Psi rhs flag read, half 
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE on this workload, scaled by
 # the counters' reading for a plain 512 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the
 # push phase is its push_kernel (the rows kernel moves a few kB); None where not measured
-TRAFFIC = {"sor_batch_kernel": 13.52e6, "fdtd_kernel": 66.04e6, "push phase": 187.68e6}
+TRAFFIC = {"sor_batch_kernel": 14.42e6, "fdtd_kernel": 66.04e6, "push phase": 187.68e6}
 TIMER_EVERY = 8
 
 
