@@ -267,7 +267,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   const int64_t cap3 = 3 * cap;
   ALLOC(R->keys, cap3); ALLOC(R->skeys, cap3); ALLOC(R->cz, cap3); ALLOC(R->scz, cap3);
   ALLOC(R->vals, cap3); ALLOC(R->gstart, cap3); ALLOC(R->long_list, cap3 / 256 + 1); ALLOC(R->gflag, cap3);
-  R->sort_tmp_bytes = deposit_sort_bytes(cap3, deposit_cell_bits(nz, nr) + 2);
+  R->sort_tmp_bytes = radix_sort_tmp_bytes(cap3);
   {
     uint8_t *t = nullptr;
     ALLOC(t, (int64_t)R->sort_tmp_bytes);

@@ -20,8 +20,6 @@
 // cannot change a bit.  The charge removal adds -Q, which equals subtracting Q.
 #include <cstring>
 
-#include <rocprim/rocprim.hpp>
-
 #include "spf_internal.h"
 
 namespace spf {
@@ -289,30 +287,6 @@ unsigned grid_for(int64_t n, int64_t cap_blocks) {
 
 }  // namespace
 
-// Radix sort of the (component, cell) keys: onesweep passes of SPF_SORT_BITS
-// key bits (block 1024 x 8 items, match ranking) for inputs of every size
-// above SPF_MERGE_LIMIT keys.  The default is onesweep throughout: the
-// charge removal's one-component sort of ~2^20 keys would otherwise take
-// rocprim's merge-sort path, which measured 0.19 ms slower per correction.
-#ifndef SPF_SORT_BITS
-#define SPF_SORT_BITS 8
-#endif
-#ifndef SPF_MERGE_LIMIT
-#define SPF_MERGE_LIMIT 0
-#endif
-using SortConfig = rocprim::radix_sort_config<
-    rocprim::default_config, rocprim::default_config,
-    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 8>, rocprim::kernel_config<1024, 8>,
-                                        SPF_SORT_BITS, rocprim::block_radix_rank_algorithm::match>,
-    SPF_MERGE_LIMIT>;
-
-size_t deposit_sort_bytes(int64_t n, int key_bits) {
-  size_t bytes = 0;
-  (void)rocprim::radix_sort_pairs<SortConfig>(nullptr, bytes, (uint32_t *)nullptr, (uint32_t *)nullptr, (double *)nullptr,
-                                  (double *)nullptr, (size_t)n, 0u, (unsigned)key_bits, (hipStream_t)0);
-  return bytes;
-}
-
 int deposit_cell_bits(int nz, int nr) {
   uint64_t cells = (uint64_t)(nz + 1) * (uint64_t)(nr + 1);
   int b = 1;
@@ -360,18 +334,18 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   hipLaunchKernelGGL(key_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, s, k);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  size_t bytes = R.sort_tmp_bytes;
-  e = rocprim::radix_sort_pairs<SortConfig>(R.sort_tmp, bytes, R.keys, R.skeys, R.cz, R.scz, (size_t)n, 0u, (unsigned)key_bits,
-                                s);
+  uint32_t *sk;
+  double *sv;
+  e = radix_sort_pairs(R.keys, R.cz, R.skeys, R.scz, n, key_bits, R.sort_tmp, R.sort_tmp_bytes, &sk, &sv, s);
   if (e != hipSuccess) return e;
-  e = segment_starts(R.skeys, n, discard, R.gstart, R.counters, R.bcount, R.boff, s);
+  e = segment_starts(sk, n, discard, R.gstart, R.counters, R.bcount, R.boff, s);
   if (e != hipSuccess) return e;
   FoldArgs f;
   f.nz = R.d.nz;
   f.ldz = R.ldz;
   f.cell_bits = cell_bits;
-  f.sk = R.skeys;
-  f.sv = R.scz;
+  f.sk = sk;
+  f.sv = sv;
   f.seg = R.gstart;
   f.counters = R.counters;
   f.long_list = R.long_list;

@@ -193,7 +193,6 @@ hipError_t launch_append_commit(Run &R, int64_t n, int64_t n_active, hipStream_t
 // and its current-density update
 int cathode_emit_step(spf_ctx *c, int32_t run, int32_t cathode, int64_t *n_emitted);
 int cathode_current_density(spf_ctx *c, int32_t run, int32_t cathode);
-// ordered compaction: out[k] = k-th index i < n with flags[i] != 0; *count = total.
 // positions of sorted keys sk[0..n) that start a segment (key differs from
 // its predecessor, not `discard`), in order, into out; counters[5] gets their
 // number, counters[6] the first discard position (preset to n)
@@ -202,8 +201,13 @@ hipError_t segment_starts(const uint32_t *sk, int64_t n, uint32_t discard, int32
 int64_t compact_blocks(int64_t n);
 // deposition
 hipError_t deposit_currents(Run &R, hipStream_t s);   // Jz, Jr, Jphi += particle currents
-size_t deposit_sort_bytes(int64_t n, int key_bits);
 int deposit_cell_bits(int nz, int nr);
+// stable radix sort of n (key, value) pairs on the low key_bits bits of the
+// keys; the input (k0, v0) and (k1, v1) alternate as pass input and output,
+// *kres/*vres name the pair holding the result
+size_t radix_sort_tmp_bytes(int64_t n);
+hipError_t radix_sort_pairs(uint32_t *k0, double *v0, uint32_t *k1, double *v1, int64_t n, int key_bits, void *tmp,
+                            size_t tmp_bytes, uint32_t **kres, double **vres, hipStream_t s);
 // Poisson
 hipError_t launch_gauss_q(Run &R, hipStream_t s);
 hipError_t launch_q_subtract(Run &R, hipStream_t s);

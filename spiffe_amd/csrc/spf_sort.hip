@@ -1,0 +1,239 @@
+// spf_sort.hip -- stable least-significant-digit radix sort of (32-bit key,
+// double value) pairs: the ordering step of the deposition engine.
+//
+// Each pass orders the pairs by one digit of RB bits (RB = 7 or 8; as many
+// passes as the key width needs) and keeps the input order among equal
+// digits, so after the last pass the pairs are ordered by key and, within a
+// key, by input position.  A pass is three launches with no waiting between
+// workgroups:
+//  1. count: each tile of TILE consecutive pairs counts its digits into
+//     cnt[digit][tile];
+//  2. scan: one workgroup per digit turns its row into exclusive tile offsets
+//     pre[digit][tile] and the digit's total;
+//  3. scatter: each tile ranks its pairs by digit in input order (per
+//     wavefront, a lane's rank among the lanes that hold the same digit comes
+//     from one ballot per digit bit), reorders them by digit in LDS and writes
+//     each digit's run to (digit start + tile offset), so the stores of a run
+//     are contiguous.
+// The count and offset tables are the only state and every pass overwrites
+// them; nothing is cleared between calls.
+#include <algorithm>
+#include <utility>
+
+#include "spf_internal.h"
+
+namespace spf {
+
+namespace {
+
+constexpr int ST = 256;          // threads per tile workgroup
+constexpr int SI = 16;           // pairs per thread
+constexpr int TILE = ST * SI;    // pairs per tile
+constexpr int NW = ST / 64;      // wavefronts per tile workgroup
+
+template <int RB>
+__global__ __launch_bounds__(ST) void sort_count_kernel(const uint32_t *keys, int64_t n, int shift, uint32_t *cnt,
+                                                        int64_t ntiles) {
+  constexpr int R = 1 << RB;
+  __shared__ uint32_t h[R];
+  for (int d = threadIdx.x; d < R; d += ST) h[d] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * TILE;
+  uint32_t k[SI];
+#pragma unroll
+  for (int i = 0; i < SI; i++) {
+    const int64_t idx = base + i * ST + threadIdx.x;
+    k[i] = idx < n ? keys[idx] : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < SI; i++)
+    if (base + i * ST + threadIdx.x < n) atomicAdd(&h[(k[i] >> shift) & (R - 1)], 1u);
+  __syncthreads();
+  for (int d = threadIdx.x; d < R; d += ST) cnt[(int64_t)d * ntiles + blockIdx.x] = h[d];
+}
+
+// inclusive scans of a and b over the workgroup's threads; returns the
+// exclusive values
+__device__ inline void block_excl_scan2(uint32_t a, uint32_t b, uint32_t &ea, uint32_t &eb, uint32_t (*ws)[NW]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t ia = a, ib = b;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t va = __shfl_up(ia, off, 64), vb = __shfl_up(ib, off, 64);
+    if (lane >= off) {
+      ia += va;
+      ib += vb;
+    }
+  }
+  if (lane == 63) {
+    ws[0][w] = ia;
+    ws[1][w] = ib;
+  }
+  __syncthreads();
+  uint32_t pa = 0, pb = 0;
+  for (int k = 0; k < w; k++) {
+    pa += ws[0][k];
+    pb += ws[1][k];
+  }
+  ea = pa + ia - a;
+  eb = pb + ib - b;
+}
+
+// one workgroup per digit d: pre[d][t] = sum of cnt[d][t'] over t' < t and
+// tot[d] = the sum of the row; thread k scans a contiguous chunk of tiles
+__global__ __launch_bounds__(ST) void sort_scan_kernel(const uint32_t *cnt, uint32_t *pre, uint32_t *tot,
+                                                       int64_t ntiles) {
+  __shared__ uint32_t ws[2][NW];
+  const uint32_t *row = cnt + (int64_t)blockIdx.x * ntiles;
+  uint32_t *out = pre + (int64_t)blockIdx.x * ntiles;
+  const int64_t per = (ntiles + ST - 1) / ST;
+  const int64_t t0 = std::min<int64_t>((int64_t)threadIdx.x * per, ntiles), t1 = std::min<int64_t>(t0 + per, ntiles);
+  uint32_t s = 0;
+  for (int64_t t = t0; t < t1; t++) s += row[t];
+  uint32_t run, unused;
+  block_excl_scan2(s, 0u, run, unused, ws);
+  for (int64_t t = t0; t < t1; t++) {
+    const uint32_t c = row[t];
+    out[t] = run;
+    run += c;
+  }
+  if (threadIdx.x == ST - 1) tot[blockIdx.x] = run;
+}
+
+// Wavefront w holds the tile's pairs w*SI*64 + j*64 + lane (j = 0..SI-1):
+// ranking item j before item j+1 and, within an item, lower lanes first,
+// follows the input order.
+template <int RB>
+__global__ __launch_bounds__(ST) void sort_scatter_kernel(const uint32_t *kin, const double *vin, uint32_t *kout,
+                                                          double *vout, int64_t n, int shift, const uint32_t *pre,
+                                                          const uint32_t *tot, int64_t ntiles) {
+  constexpr int R = 1 << RB;
+  static_assert(R <= ST, "one thread per digit");
+  __shared__ uint32_t off[NW][R];   // per-wavefront digit counts, then each wavefront's start in the tile
+  __shared__ uint32_t gbase[R];     // output position of the digit's run minus its tile-local start
+  __shared__ uint32_t ws[2][NW];
+  __shared__ uint32_t skey[TILE];
+  __shared__ double sval[TILE];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t base = (int64_t)blockIdx.x * TILE;
+  const int nvalid = (int)std::min<int64_t>(TILE, n - base);
+  for (int d = threadIdx.x; d < NW * R; d += ST) (&off[0][0])[d] = 0;
+  uint32_t key[SI], rank[SI];
+  double val[SI];
+#pragma unroll
+  for (int j = 0; j < SI; j++) {
+    const int p = (w * SI + j) * 64 + lane;
+    key[j] = p < nvalid ? kin[base + p] : 0u;
+    val[j] = p < nvalid ? vin[base + p] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < SI; j++) {
+    const int p = (w * SI + j) * 64 + lane;
+    const uint32_t d = (key[j] >> shift) & (R - 1);
+    unsigned long long peers = __ballot(p < nvalid);
+#pragma unroll
+    for (int b = 0; b < RB; b++) {
+      const bool bit = (d >> b) & 1u;
+      const unsigned long long m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const uint32_t below =
+        __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
+    // every lane reads the count before the lowest lane of each digit group
+    // stores the new one (LDS accesses of a wavefront complete in order)
+    const uint32_t old = off[w][d];
+    rank[j] = old + below;
+    if (p < nvalid && below == 0) off[w][d] = old + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  // tile count of each digit, wavefront starts, then the digit starts in the
+  // tile (exclusive over digits) and in the output (digit totals before it
+  // plus the tiles before this one)
+  const int d = threadIdx.x;
+  uint32_t tcount = 0, gcount = 0;
+  if (d < R) {
+    uint32_t run = 0;
+    for (int k = 0; k < NW; k++) {
+      const uint32_t c = off[k][d];
+      off[k][d] = run;
+      run += c;
+    }
+    tcount = run;
+    gcount = tot[d];
+  }
+  uint32_t tstart, gstart;
+  block_excl_scan2(tcount, gcount, tstart, gstart, ws);
+  if (d < R) {
+    gbase[d] = gstart + pre[(int64_t)d * ntiles + blockIdx.x] - tstart;   // modulo 2^32; n < 2^31
+    for (int k = 0; k < NW; k++) off[k][d] += tstart;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < SI; j++) {
+    const int p = (w * SI + j) * 64 + lane;
+    if (p < nvalid) {
+      const uint32_t pos = off[w][(key[j] >> shift) & (R - 1)] + rank[j];
+      skey[pos] = key[j];
+      sval[pos] = val[j];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < SI; i++) {
+    const int p = i * ST + threadIdx.x;
+    if (p < nvalid) {
+      const uint32_t k = skey[p];
+      const uint32_t dst = gbase[(k >> shift) & (R - 1)] + (uint32_t)p;
+      if (dst < (uint32_t)n) {   // always true for consistent tables; keeps stores inside the output
+        kout[dst] = k;
+        vout[dst] = sval[p];
+      }
+    }
+  }
+}
+
+template <int RB>
+hipError_t sort_pass(const uint32_t *ki, const double *vi, uint32_t *ko, double *vo, int64_t n, int shift,
+                     uint32_t *cnt, uint32_t *pre, uint32_t *tot, int64_t nt, hipStream_t s) {
+  hipLaunchKernelGGL(sort_count_kernel<RB>, dim3((unsigned)nt), dim3(ST), 0, s, ki, n, shift, cnt, nt);
+  hipLaunchKernelGGL(sort_scan_kernel, dim3(1u << RB), dim3(ST), 0, s, cnt, pre, tot, nt);
+  hipLaunchKernelGGL(sort_scatter_kernel<RB>, dim3((unsigned)nt), dim3(ST), 0, s, ki, vi, ko, vo, n, shift, pre,
+                     tot, nt);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+size_t radix_sort_tmp_bytes(int64_t n) {
+  const int64_t nt = (std::max<int64_t>(n, 1) + TILE - 1) / TILE;
+  return (size_t)(2 * 256 * nt + 256) * sizeof(uint32_t);
+}
+
+hipError_t radix_sort_pairs(uint32_t *k0, double *v0, uint32_t *k1, double *v1, int64_t n, int key_bits, void *tmp,
+                            size_t tmp_bytes, uint32_t **kres, double **vres, hipStream_t s) {
+  *kres = k0;
+  *vres = v0;
+  if (n <= 0) return hipSuccess;
+  if (n >= (1ll << 31) || key_bits < 1 || key_bits > 32 || tmp_bytes < radix_sort_tmp_bytes(n))
+    return hipErrorInvalidValue;
+  const int64_t nt = (n + TILE - 1) / TILE;
+  const int passes = (key_bits + 7) / 8;
+  const int rb = (key_bits + passes - 1) / passes;   // digit width, at most 8
+  uint32_t *cnt = static_cast<uint32_t *>(tmp), *pre = cnt + 256 * nt, *tot = pre + 256 * nt;
+  uint32_t *ki = k0, *ko = k1;
+  double *vi = v0, *vo = v1;
+  for (int p = 0; p < passes; p++) {
+    // a digit mask wider than rb overlaps the next digit, which the next
+    // (stable) pass orders anyway
+    const hipError_t e = rb <= 7 ? sort_pass<7>(ki, vi, ko, vo, n, p * rb, cnt, pre, tot, nt, s)
+                                 : sort_pass<8>(ki, vi, ko, vo, n, p * rb, cnt, pre, tot, nt, s);
+    if (e != hipSuccess) return e;
+    std::swap(ki, ko);
+    std::swap(vi, vo);
+  }
+  *kres = ki;
+  *vres = vi;
+  return hipSuccess;
+}
+
+}  // namespace spf

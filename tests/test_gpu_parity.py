@@ -154,6 +154,27 @@ def test_deposit_ngp_ordered_sums(ctx, n, seed):
     d.close()
 
 
+@pytest.mark.parametrize("n,seed", [(300_001, 5), (4096 * 3, 6)])
+def test_deposit_concentrated_bunch(ctx, n, seed):
+    """Most particles in a few cells: cell segments thousands of terms long that
+    span many sort tiles, plus a ragged last tile (or none)."""
+    g = gun_grid(modes=TMTE | abi.SPACE_CHARGE)
+    f = gun_flags(g)
+    o, d = pair(ctx, g, f, cap=n)
+    p = random_particles(g, n, seed)
+    rng = np.random.default_rng(seed + 100)
+    k = (n * 3) // 4
+    p["z"][:k] = g.zmin + g.dz * (40 + rng.random(k) * 2.0)
+    p["r"][:k] = g.dr * (1 + rng.random(k) * 2.0)
+    p["status"][::11] = 0
+    o.append(p)
+    d.append(p)
+    o.add_particle_currents()
+    d.add_particle_currents()
+    assert_fields(o, d, J_NAMES)
+    d.close()
+
+
 def test_deposit_empty(ctx):
     g = gun_grid(modes=TMTE | abi.SPACE_CHARGE)
     o, d = pair(ctx, g, gun_flags(g), cap=10)
