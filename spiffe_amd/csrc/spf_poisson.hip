@@ -100,8 +100,10 @@ __device__ inline double from_right(double v) {
 // theorem: the first step makes q faithful, the second rounds it correctly;
 // nothing underflows in that range).  For x = +-0, q0 is the correctly signed
 // zero, each remainder is +0 and +0*y = -0 (y < 0) leaves q0 unchanged.
-// Callers divide directly outside that set.  Zero residuals are common (metal,
-// and vacuum the solve has not reached), so they stay on the fast path.
+// Zero residuals are common (metal, and vacuum the solve has not reached), so
+// they stay on the fast path.  The batch kernel checks the set after the fact
+// (sor_lane_exact) and redoes a tile's batch with plain division when it was
+// left.
 __device__ inline double div_fast(double x, double d, double y) {
   double q = x * y;
   double r = __builtin_fma(-q, d, x);
@@ -110,9 +112,12 @@ __device__ inline double div_fast(double x, double d, double y) {
   return __builtin_fma(r, y, q);
 }
 
-__device__ inline bool div_fast_ok(double x) {
-  double ax = fabs(x);
-  return ax == 0 || (ax > 1e-280 && ax < 1e280);
+// frexp exponent e of x (|x| in [2^(e-1), 2^e); 0 for x = +-0, inf and NaN),
+// offset so that e in [-929, 930], i.e. |x| in [2^-930, 2^930), a subset of
+// (1e-280, 1e280), maps to [0, kExpSpan] and everything else above it
+constexpr unsigned kExpOffset = 929, kExpSpan = 1859;
+__device__ inline unsigned div_fast_exp(double x) {
+  return (unsigned)(__builtin_amdgcn_frexp_exp(x) + (int)kExpOffset);
 }
 
 // Workgroups are dealt to the 8 XCDs in turn (workgroup b runs on XCD b % 8).
@@ -168,6 +173,8 @@ struct SorTile {
 template <class G>
 struct SorLane {
   double P[G::RS][2], D[G::RS][2];
+  double cr[G::RS], dr[G::RS];   // row coefficients c, d of the lane's rows (uniform over the wavefront)
+  unsigned eacc;     // largest div_fast_exp of an updated point's x in the batch
   unsigned upd;      // in the array interior and not metal (src/poisson.c:58-120)
   unsigned own;      // in the tile's owned region
   int zdepth[2];     // distance of each column from the nearest shrinking z edge
@@ -226,12 +233,12 @@ __device__ inline void sor_lane_masks(const SorCommon &a, const SorTile<G> &t, i
   }
 }
 
-// right-hand side, masks (one stored word per lane) and row coefficients
-// (cr/dr in LDS, TRW each); without a right-hand side the loads read src
-// (same layout) and are discarded
+// right-hand side, masks (one stored word per lane) and the row coefficients
+// of the wavefront's rows, kept in registers for the whole batch; without a
+// right-hand side the loads read src (same layout) and are discarded
 template <class G>
 __device__ inline void sor_setup(const SorCommon &a, const SorTile<G> &t, int tile, int w, int lane, SorLane<G> &ln,
-                                 const double *src, double *cr, double *dr) {
+                                 const double *src) {
   const int iz_e = t.zlo + 2 * lane;
   const double *rhs = a.rhs ? a.rhs : src;
   const uint32_t m = a.masks[(int64_t)tile * G::NT + threadIdx.x];
@@ -241,11 +248,13 @@ __device__ inline void sor_setup(const SorCommon &a, const SorTile<G> &t, int ti
     ln.D[i][0] = a.rhs ? v.x : 0.0;
     ln.D[i][1] = a.rhs ? v.y : 0.0;
   }
-  for (int k = threadIdx.x; k < G::TRW; k += G::NT) {
-    const int irc = min(t.rlo + k, a.nr);
-    cr[k] = a.c[irc];
-    dr[k] = a.d[irc];
+#pragma unroll
+  for (int i = 0; i < G::RS; i++) {
+    const int irc = min(t.rlo + w * G::RS + i, a.nr);
+    ln.cr[i] = a.c[irc];
+    ln.dr[i] = a.d[irc];
   }
+  ln.eacc = 0;
   ln.upd = m & 0xffffu;
   ln.own = m >> 16;
 #pragma unroll
@@ -268,20 +277,18 @@ __device__ inline void sor_setup(const SorCommon &a, const SorTile<G> &t, int ti
 // |residual| over the lane's updated owned points.
 // EDGE: the tile touches the array's left (ledge) or right (redge) z end and
 // needs the mirrors; AXM: 0 the wavefront does not hold the axis row, 1 it
-// does, 2 decided by AX at run time.
-template <class G, int C, bool EDGE, int AXM>
+// does, 2 decided by AX at run time.  EXACT: divide directly; otherwise
+// div_fast, with each updated point's exponent folded into ln.eacc.
+template <class G, int C, bool EDGE, int AXM, bool EXACT>
 __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsigned okb, const double2 up_row,
-                                        const double2 dn_row, const double *cr, const double *dr, int w,
-                                        double omega, bool ledge, bool redge, bool AX) {
+                                        const double2 dn_row, int w, double omega, bool ledge, bool redge, bool AX) {
   const double A = a.a, E = a.e;
   const bool ax = AXM == 2 ? AX : AXM == 1;
   const bool lane0 = (threadIdx.x & 63) == 0;
   double q[G::RS], x[G::RS], res[G::RS];
-  bool slow = false;
 #pragma unroll
   for (int i = 0; i < G::RS; i++) {
     const int col = (i + C) & 1;
-    const int lr = w * G::RS + i;
     double L, R;
     if (col == 0) {
       R = ln.P[i][1];
@@ -297,14 +304,14 @@ __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsi
     }
     const double zs = R + L;
     const double up = i < G::RS - 1 ? ln.P[i + 1 < G::RS ? i + 1 : i][col] : (col ? up_row.y : up_row.x);
-    const double c = cr[lr];
+    const double c = ln.cr[i];
     double residual, diag, y;
     if (i == 0) {
       // row 0 of the tile may be the axis row, src/poisson.c:84-90: no "below"
       // term (added as -0.0, an exact no-op), a 4*c "above" weight, diagonal e - 2*c
       const double down = col ? dn_row.y : dn_row.x;
       const double cu = ax ? 4 * c : c;
-      const double dn = ax ? -0.0 : dr[lr] * down;
+      const double dn = ax ? -0.0 : ln.dr[i] * down;
       diag = ax ? E - 2 * c : E;
       y = ax ? a.y_axis : a.y_e;
       residual = A * zs + cu * up + dn + diag * ln.P[i][col] - ln.D[i][col];
@@ -312,19 +319,11 @@ __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsi
       const double down = i > 0 ? ln.P[i > 0 ? i - 1 : 0][col] : (col ? dn_row.y : dn_row.x);
       diag = E;
       y = a.y_e;
-      residual = A * zs + c * up + dr[lr] * down + E * ln.P[i][col] - ln.D[i][col];
+      residual = A * zs + c * up + ln.dr[i] * down + E * ln.P[i][col] - ln.D[i][col];
     }
     res[i] = residual;
     x[i] = omega * residual;
-    q[i] = div_fast(x[i], diag, y);
-    slow |= ((okb >> (2 * i + col)) & 1) && !div_fast_ok(x[i]);
-  }
-  if (slow || !a.use_y) {
-#pragma unroll
-    for (int i = 0; i < G::RS; i++) {
-      const double diag = (ax && i == 0) ? E - 2 * cr[w * G::RS] : E;
-      if (!a.use_y || !div_fast_ok(x[i])) q[i] = x[i] / diag;
-    }
+    q[i] = EXACT ? x[i] / diag : div_fast(x[i], diag, y);
   }
   double acc = 0;
 #pragma unroll
@@ -333,6 +332,7 @@ __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsi
     const bool ok = (okb >> (2 * i + col)) & 1;
     const double nv = ln.P[i][col] - q[i];
     ln.P[i][col] = ok ? nv : ln.P[i][col];
+    if (!EXACT) ln.eacc = max(ln.eacc, ok ? div_fast_exp(x[i]) : 0u);
     acc += (ok && ((ln.own >> (2 * i + col)) & 1)) ? fabs(res[i]) : 0.0;
   }
   return acc;
@@ -340,10 +340,9 @@ __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsi
 
 // One local step j of a batch, colour C, relaxation factor omega; returns the
 // lane's owned residual sum.
-template <class G, int C, bool EDGE, int AXM>
+template <class G, int C, bool EDGE, int AXM, bool EXACT = false>
 __device__ inline double sor_step(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane,
-                                  double omega, int j, double2 (*xrow)[G::SW][2][64], const double *cr,
-                                  const double *dr, bool axis_wave) {
+                                  double omega, int j, double2 (*xrow)[G::SW][2][64], bool axis_wave) {
   const int par = j & 1;
   // publish the first and last rows, then read the neighbouring wavefronts' rows
   xrow[par][w][0][lane] = make_double2(ln.P[0][0], ln.P[0][1]);
@@ -356,8 +355,8 @@ __device__ inline double sor_step(const SorCommon &a, const SorTile<G> &t, SorLa
   const int lo = min(max(vr0 - w * G::RS, 0), G::RS), hi = min(max(vr1 - w * G::RS, 0), G::RS);
   const unsigned rmask = hi > lo ? (((1u << (2 * hi)) - 1u) & ~((1u << (2 * lo)) - 1u)) : 0u;
   const unsigned okb = ln.upd & rmask & ((ln.zdepth[0] > j ? G::COL0 : 0u) | (ln.zdepth[1] > j ? G::COL1 : 0u));
-  return sor_half_sweep<G, C, EDGE, AXM>(a, ln, okb, up_row, dn_row, cr, dr, w, omega, t.zlo == 0, t.redge,
-                                         axis_wave);
+  return sor_half_sweep<G, C, EDGE, AXM, EXACT>(a, ln, okb, up_row, dn_row, w, omega, t.zlo == 0, t.redge,
+                                                axis_wave);
 }
 
 // Steps j = 0 .. k-1 of an even n0 in colour pairs (0, 1): one copy of the
@@ -366,47 +365,65 @@ __device__ inline double sor_step(const SorCommon &a, const SorTile<G> &t, SorLa
 // owned residual sum of step j.
 template <class G, bool EDGE, int AXM>
 __device__ inline void sor_pairs(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane, int k,
-                                 const double *om, double2 (*xrow)[G::SW][2][64], const double *cr,
-                                 const double *dr, double (*lres)[G::NT]) {
+                                 const double *om, double2 (*xrow)[G::SW][2][64], double (*lres)[G::NT]) {
 #pragma unroll 1
   for (int j = 0; j < k; j += 2) {
-    lres[j][threadIdx.x] = sor_step<G, 0, EDGE, AXM>(a, t, ln, w, lane, om[j], j, xrow, cr, dr, AXM == 1);
+    lres[j][threadIdx.x] = sor_step<G, 0, EDGE, AXM>(a, t, ln, w, lane, om[j], j, xrow, AXM == 1);
     if (j + 1 < k)
-      lres[j + 1][threadIdx.x] = sor_step<G, 1, EDGE, AXM>(a, t, ln, w, lane, om[j + 1], j + 1, xrow, cr, dr,
-                                                           AXM == 1);
+      lres[j + 1][threadIdx.x] = sor_step<G, 1, EDGE, AXM>(a, t, ln, w, lane, om[j + 1], j + 1, xrow, AXM == 1);
   }
 }
 
 // k <= SK half-sweeps n0 .. n0+k-1 with relaxation factors om[0 .. k-1] (LDS,
 // read after each step's barrier); lres[j][thread] gets the lane's owned
-// residual sum of step j.  An odd n0 (only from spf_poisson_sweeps) runs the
-// steps one by one with run-time colours.
+// residual sum of step j.  An odd n0 (only from spf_poisson_sweeps) and the
+// exact division run the steps one by one with run-time colours.
 template <class G>
 __device__ inline void sor_sweeps(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane,
                                   int64_t n0, int k, const double *om, double2 (*xrow)[G::SW][2][64],
-                                  const double *cr, const double *dr, double (*lres)[G::NT]) {
+                                  double (*lres)[G::NT], bool exact) {
   const bool axis_wave = t.rlo == 0 && w == 0;
   const bool edge = t.zlo == 0 || t.redge;
-  if ((n0 & 1) == 0) {
+  if (exact) {
+#pragma unroll 1
+    for (int j = 0; j < k; j++)
+      lres[j][threadIdx.x] =
+          ((n0 + j) & 1) ? sor_step<G, 1, true, 2, true>(a, t, ln, w, lane, om[j], j, xrow, axis_wave)
+                         : sor_step<G, 0, true, 2, true>(a, t, ln, w, lane, om[j], j, xrow, axis_wave);
+  } else if ((n0 & 1) == 0) {
     // the tile kind is uniform over the workgroup, the axis flag over the wavefront
     if (edge) {
       if (axis_wave)
-        sor_pairs<G, true, 1>(a, t, ln, w, lane, k, om, xrow, cr, dr, lres);
+        sor_pairs<G, true, 1>(a, t, ln, w, lane, k, om, xrow, lres);
       else
-        sor_pairs<G, true, 0>(a, t, ln, w, lane, k, om, xrow, cr, dr, lres);
+        sor_pairs<G, true, 0>(a, t, ln, w, lane, k, om, xrow, lres);
     } else {
       if (axis_wave)
-        sor_pairs<G, false, 1>(a, t, ln, w, lane, k, om, xrow, cr, dr, lres);
+        sor_pairs<G, false, 1>(a, t, ln, w, lane, k, om, xrow, lres);
       else
-        sor_pairs<G, false, 0>(a, t, ln, w, lane, k, om, xrow, cr, dr, lres);
+        sor_pairs<G, false, 0>(a, t, ln, w, lane, k, om, xrow, lres);
     }
   } else {
 #pragma unroll 1
     for (int j = 0; j < k; j++)
       lres[j][threadIdx.x] =
-          ((n0 + j) & 1) ? sor_step<G, 1, true, 2>(a, t, ln, w, lane, om[j], j, xrow, cr, dr, axis_wave)
-                         : sor_step<G, 0, true, 2>(a, t, ln, w, lane, om[j], j, xrow, cr, dr, axis_wave);
+          ((n0 + j) & 1) ? sor_step<G, 1, true, 2>(a, t, ln, w, lane, om[j], j, xrow, axis_wave)
+                         : sor_step<G, 0, true, 2>(a, t, ln, w, lane, om[j], j, xrow, axis_wave);
   }
+}
+
+// The fast division was valid for every updated point of the batch: each x had
+// div_fast_exp <= kExpSpan (|x| inside div_fast's set, or x = +-0, inf or NaN),
+// and none was inf or NaN, since div_fast turns those into NaN, which every
+// later update of the point keeps.
+template <class G>
+__device__ inline bool sor_lane_exact(const SorLane<G> &ln) {
+  bool ok = ln.eacc <= kExpSpan;
+#pragma unroll
+  for (int i = 0; i < G::RS; i++)
+#pragma unroll
+    for (int col = 0; col < 2; col++) ok = ok && __builtin_isfinite(ln.P[i][col]);
+  return ok;
 }
 
 // the tile's owned residual sum of each step j < k into out[j * stride]:
@@ -529,7 +546,6 @@ __global__ __launch_bounds__(BatchCfg::NT) void sor_batch_kernel(SorArgs a) {
   }
   __shared__ double2 xrow[2][G::SW][2][64];   // [parity][wave][first,last row][lane]
   __shared__ double wsum[G::SK][G::SW];
-  __shared__ double cr[G::TRW], dr[G::TRW];
   const int64_t done = a.force ? 0 : a.state[0];
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int tile = xcd_band_tile((int)blockIdx.x, a.m.ntiles);
@@ -539,10 +555,16 @@ __global__ __launch_bounds__(BatchCfg::NT) void sor_batch_kernel(SorArgs a) {
   if (threadIdx.x < G::SK) om[threadIdx.x] = a.m.omega[a.n0 + min((int)threadIdx.x, a.k - 1)];
   SorLane<G> ln;
   sor_load_P<G>(a.m, t, w, lane, a.Pin, ln.P);
-  sor_setup<G>(a.m, t, tile, w, lane, ln, a.Pin, cr, dr);
+  sor_setup<G>(a.m, t, tile, w, lane, ln, a.Pin);
   if (done) return;
   __shared__ double lres[G::SK][G::NT];
-  sor_sweeps<G>(a.m, t, ln, w, lane, a.n0, a.k, om, xrow, cr, dr, lres);
+  sor_sweeps<G>(a.m, t, ln, w, lane, a.n0, a.k, om, xrow, lres, !a.m.use_y);
+  if (a.m.use_y && __syncthreads_or(!sor_lane_exact<G>(ln))) {
+    // rare: some x left div_fast's set; the batch input is intact, so redo
+    // the tile's batch with plain division
+    sor_load_P<G>(a.m, t, w, lane, a.Pin, ln.P);
+    sor_sweeps<G>(a.m, t, ln, w, lane, a.n0, a.k, om, xrow, lres, true);
+  }
   sor_block_sums<G>(lres, a.k, w, lane, wsum, a.partials + tile, a.m.ntiles);
   sor_write_owned<G>(a.m, t, ln, w, lane, a.Pout);
 }

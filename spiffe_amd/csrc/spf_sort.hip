@@ -39,15 +39,28 @@ __global__ __launch_bounds__(ST) void sort_count_kernel(const uint32_t *keys, in
   for (int d = threadIdx.x; d < R; d += ST) h[d] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * TILE;
+  const int lane = threadIdx.x & 63;
   uint32_t k[SI];
 #pragma unroll
   for (int i = 0; i < SI; i++) {
     const int64_t idx = base + i * ST + threadIdx.x;
     k[i] = idx < n ? keys[idx] : 0u;
   }
+  // After a pass equal digits come in runs, and LDS atomics on one address
+  // serialise: the first lane of each run of equal digits adds the run's
+  // length.  Valid lanes of a wavefront are a prefix of it.
 #pragma unroll
-  for (int i = 0; i < SI; i++)
-    if (base + i * ST + threadIdx.x < n) atomicAdd(&h[(k[i] >> shift) & (R - 1)], 1u);
+  for (int i = 0; i < SI; i++) {
+    const bool valid = base + i * ST + threadIdx.x < n;
+    const uint32_t d = (k[i] >> shift) & (R - 1);
+    const uint32_t dp = __shfl_up(d, 1, 64);
+    const bool start = valid && (lane == 0 || d != dp);
+    const unsigned long long starts = __ballot(start);
+    const int nv = __popcll(__ballot(valid));
+    const unsigned long long above = starts & ~((2ull << lane) - 1ull);
+    const int next = above ? __ffsll((long long)above) - 1 : nv;
+    if (start) atomicAdd(&h[d], (uint32_t)(next - lane));
+  }
   __syncthreads();
   for (int d = threadIdx.x; d < R; d += ST) cnt[(int64_t)d * ntiles + blockIdx.x] = h[d];
 }

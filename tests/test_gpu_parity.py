@@ -369,6 +369,39 @@ def test_sor_solve_counts_and_bits(ctx, limit, acc):
     d.close()
 
 
+@pytest.mark.parametrize("case", ["tiny", "huge", "patch", "zero"])
+def test_sor_division_outside_fast_set(ctx, case):
+    """Residuals whose magnitudes leave the fast division's range (below 1e-280 or above
+    1e280) in all tiles or in a patch of them, and an all-zero solve: same bits and count."""
+    g = gun_grid(modes=abi.TM_FIELDS | abi.RADIAL_INTERP | abi.LONGIT_INTERP)
+    g.left_bc = abi.DIRICHLET
+    f = gun_flags(g)
+    o, d = pair(ctx, g, f)
+    rng = np.random.default_rng(7)
+    psi0 = rng.standard_normal(g.shape)
+    rhs = rng.standard_normal(g.shape) * 1e3
+    if case == "tiny":
+        psi0 *= 1e-292
+        rhs *= 1e-295
+    elif case == "huge":
+        psi0 *= 1e284
+        rhs *= 1e281
+    elif case == "patch":
+        psi0[60:90, 10:30] *= 1e-293
+        rhs[60:90, 10:30] = 0.0
+    else:
+        psi0[:] = 0.0
+        rhs[:] = 0.0
+    for obj in (o, d):
+        obj.set_field("Psi", psi0) if obj is o else obj.upload("Psi", psi0)
+        obj.set_field("Q", rhs) if obj is o else obj.upload("Q", rhs)
+    it_o, _ = o.solve_poisson_cyl("Psi", "Q", 1e-300, 40)
+    it_d = d.solve_poisson_cyl(0, 1e-300, 40)
+    assert it_o == it_d
+    assert_fields(o, d, ["Psi"])
+    d.close()
+
+
 def test_two_runs_one_context_match_alone(ctx):
     """Runs sharing a context and interleaving their steps give the bits each gives alone."""
     g = gun_grid(modes=TMTE | abi.SPACE_CHARGE)
