@@ -40,8 +40,8 @@ void free_run(Run *R) {
   F(R->pid); F(R->gen); F(R->status); F(R->counters); F(R->evt); F(R->evflag);
   for (int k = 0; k < 7; k++) F(R->ev[k]);
   F(R->last_old); F(R->evlist); F(R->move_src); F(R->move_dst); F(R->srows); F(R->lrows);
-  F(R->keys); F(R->skeys); F(R->vals); F(R->cz); F(R->scz); F(R->gstart); F(R->long_list); F(R->pcount); F(R->gflag); F(R->sort_tmp);
-  F(R->bcount); F(R->boff); F(R->stage); F(R->append_stage); F(R->emit_stage); F(R->Pdev); F(R->omega); F(R->red);
+  F(R->keys); F(R->skeys); F(R->vals); F(R->cz); F(R->scz); F(R->long_list); F(R->pcount); F(R->gflag); F(R->sort_tmp);
+  F(R->stage); F(R->append_stage); F(R->emit_stage); F(R->Pdev); F(R->omega); F(R->red);
   F(R->red_counter); F(R->sor_state); F(R->sor_sums); F(R->sor_masks);
   R->live = false;
 }
@@ -200,7 +200,7 @@ int64_t spf_run_footprint(const spf_run_desc *d) {
   int64_t n = d->max_particles > 0 ? d->max_particles : 1;
   int64_t part_bytes = n * (16 * 8 + 4 + 2 + 4);                         // particle SoA
   int64_t event_bytes = n * (4 + 1 + 7 * 8 + 4 + 8);                     // staging, lists
-  int64_t sort_bytes = 3 * n * (4 + 4 + 8 + 8 + 4 + 4 + 1) + 3 * n * 8;  // keys, values, segments, sort temp
+  int64_t sort_bytes = 3 * n * (4 + 4 + 8 + 8 + 8 + 1 + 1);  // keys, contributions (both sorted), flags, sort tables
   int64_t ring = default_srow_cap(d) * (int64_t)sizeof(spf_screen_row) + default_lrow_cap(d) * (int64_t)sizeof(spf_lost_row);
   return field_bytes + part_bytes + event_bytes + sort_bytes + ring;
 }
@@ -266,15 +266,13 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   // ordered scatter-add: three components (Jz, Jr, Jphi) per particle
   const int64_t cap3 = 3 * cap;
   ALLOC(R->keys, cap3); ALLOC(R->skeys, cap3); ALLOC(R->cz, cap3); ALLOC(R->scz, cap3);
-  ALLOC(R->vals, cap3); ALLOC(R->gstart, cap3); ALLOC(R->long_list, cap3 / 256 + 1); ALLOC(R->gflag, cap3);
+  ALLOC(R->vals, cap3); ALLOC(R->long_list, cap3 / 256 + 1); ALLOC(R->gflag, cap3);
   R->sort_tmp_bytes = radix_sort_tmp_bytes(cap3);
   {
     uint8_t *t = nullptr;
     ALLOC(t, (int64_t)R->sort_tmp_bytes);
     R->sort_tmp = t;
   }
-  R->nblk_cap = compact_blocks(cap3) + 1;
-  ALLOC(R->bcount, R->nblk_cap); ALLOC(R->boff, R->nblk_cap);
   R->red_n = std::max<int64_t>(sor_tiles(*R) * sor_batch() * 4 * 2, 512) + 16;   // two check groups of partials
   ALLOC(R->red, R->red_n);
   ALLOC(R->red_counter, 4); ALLOC(R->sor_state, 4); ALLOC(R->sor_sums, 8);

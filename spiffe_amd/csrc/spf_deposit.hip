@@ -11,8 +11,9 @@
 //  2. a stable radix sort of the (key, value) pairs: inputs are laid out in
 //     particle index order per component, so each cell's contributions come
 //     out contiguous and still in index order;
-//  3. each cell segment is folded sequentially onto the cell's current value
-//     by one thread, adding from register blocks of 16 values.
+//  3. each cell segment (the run of one key) is folded sequentially onto the
+//     cell's current value by one thread, or by one wavefront when it is
+//     longer than LONG_SEG values.
 // Every cell has exactly one writer, so the result is the reference's bits.
 // Contributions that are exactly zero in the reference (ir >= nr-1 for Jz and
 // Jphi, ir == 0 and ir == nr-1 for Jr) are left out: J starts each step at +0
@@ -42,8 +43,7 @@ struct KeyArgs {
   double *vals;
   uint32_t discard;
   int cell_bits;
-  int64_t *counters;            // [5], [6], [9] preset by thread 0 for the segment pass
-  int64_t n_pairs;
+  int64_t *counters;            // [9] (number of long segments) cleared by thread 0
 };
 
 // NGP value of the Jz/Jphi loops for cell row ir (0 <= ir <= nr-2), :92-100, :254-262
@@ -54,11 +54,7 @@ __device__ inline double jm_centered(double QVG, long long ir) {
 
 __global__ __launch_bounds__(256) void key_kernel(KeyArgs a) {
   int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip == 0) {
-    a.counters[5] = 0;            // number of segments
-    a.counters[6] = a.n_pairs;    // first discarded position
-    a.counters[9] = 0;            // number of long segments
-  }
+  if (ip == 0) a.counters[9] = 0;
   if (ip >= a.n_ub) return;
   int64_t np = *a.np;
   bool live = ip < np && (a.status[ip] & SPF_PART_ACTIVE);
@@ -130,11 +126,12 @@ __global__ __launch_bounds__(256) void key_kernel(KeyArgs a) {
 
 struct FoldArgs {
   int nz, ldz, cell_bits;
+  int64_t n;
+  uint32_t discard;
   const uint32_t *sk;
   const double *sv;
-  const int32_t *seg;
-  int64_t *counters;
-  int32_t *long_list;   // ids of segments longer than LONG_SEG
+  int64_t *counters;    // [9]: number of long segments
+  int32_t *long_list;   // first positions of segments longer than LONG_SEG
   double *target[4];
 };
 
@@ -145,29 +142,63 @@ __device__ inline double *target_cell(const FoldArgs &a, uint32_t key) {
   return a.target[k] + fix(a.ldz, iz, ir);
 }
 
-__device__ inline void seg_range(const FoldArgs &a, int64_t S, int64_t g, int64_t &s, int64_t &e) {
-  s = a.seg[g];
-  e = (g + 1 < S) ? a.seg[g + 1] : a.counters[6];
-}
-
-// The additions of a segment form one dependent chain.  Segments of up to
-// LONG_SEG values are folded by one thread each, from register blocks of 16
-// values with the next block's loads already in flight; longer segments are
-// listed and folded by one wavefront each (fold_long_kernel).
+// The additions of a segment (the run of one key in the sorted pairs) form
+// one dependent chain.  Each wavefront looks at 64 consecutive positions: a
+// run starts where the key differs from the previous one, and a run that
+// starts in the window ends at the next start in it; only the window's last
+// run can go on past it, and the wavefront finds where.  The lane of a
+// segment's start folds it, from register blocks of 16 values with the next
+// block's loads already in flight, unless the key at start + LONG_SEG is still
+// the same: such segments are listed and folded by one wavefront each
+// (fold_long_kernel).  The discard key's run is not folded.
 constexpr int FB = 16;
 constexpr int64_t LONG_SEG = 256;
 
 __global__ __launch_bounds__(256) void fold_kernel(FoldArgs a) {
-  const int64_t S = a.counters[5];
-  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S; g += (int64_t)gridDim.x * blockDim.x) {
-    int64_t s0, e0;
-    seg_range(a, S, g, s0, e0);
-    if (e0 - s0 > LONG_SEG) {
-      unsigned long long slot = atomicAdd((unsigned long long *)(a.counters + 9), 1ull);
-      a.long_list[slot] = (int32_t)g;
-      continue;
+  const int lane = threadIdx.x & 63;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, base = p - lane;
+  if (base >= a.n) return;
+  const bool valid = p < a.n;
+  const uint32_t key = valid ? a.sk[p] : 0u;
+  // a run starts here, or the array has ended
+  const bool edge = !valid || p == 0 || a.sk[p - 1] != key;
+  const unsigned long long em = __ballot(edge);
+  if (em == 0) return;   // the window lies inside one run
+  const unsigned long long above = em & ~((2ull << lane) - 1ull);
+  int64_t e0 = above ? base + (__ffsll((long long)above) - 1) : 0;
+  // the window's last run (from lane L) may go on past the window
+  const int L = 63 - __clzll((long long)em);
+  const int64_t sL = base + L;
+  const uint32_t kL = __shfl(key, L, 64);
+  bool longL = false;
+  int64_t eL = 0;
+  if (sL < a.n && kL != a.discard) {
+    if (sL + LONG_SEG < a.n && a.sk[sL + LONG_SEG] == kL) {
+      longL = true;
+    } else {
+      // sorted keys: the run ends within LONG_SEG positions of sL
+      for (int64_t q = base + 64;; q += 64) {
+        const int64_t i = q + lane;
+        const unsigned long long dm = __ballot(i >= a.n || a.sk[i] != kL);
+        if (dm) {
+          eL = q + (__ffsll((long long)dm) - 1);
+          break;
+        }
+      }
     }
-    double *t = target_cell(a, a.sk[s0]);
+  }
+  if (!valid || !edge || key == a.discard) return;
+  const int64_t s0 = p;
+  if (lane == L) {
+    if (longL) {
+      const unsigned long long slot = atomicAdd((unsigned long long *)(a.counters + 9), 1ull);
+      a.long_list[slot] = (int32_t)s0;
+      return;
+    }
+    e0 = eL;
+  }
+  {
+    double *t = target_cell(a, key);
     double s = *t;
     int64_t i = s0;
     if (e0 - i >= FB) {
@@ -213,43 +244,54 @@ __device__ inline double add16_in_order(double s, const double2 (&v)[8]) {
 }
 
 // One wavefront per long segment.  The wavefront loads the segment in
-// coalesced chunks of LCH values into registers, one chunk ahead, and parks
-// each chunk in LDS; lane 0 adds the parked chunk in order while the next
-// chunk's loads are in flight.
+// coalesced chunks of LCH values (with their keys, which mark where the
+// segment ends) into registers, one chunk ahead, and parks each chunk in LDS;
+// lane 0 adds the parked values in order while the next chunk's loads are in
+// flight.
 constexpr int LU = 16;            // values per lane per chunk
 constexpr int LCH = 64 * LU;      // values per chunk
 
 __global__ __launch_bounds__(64) void fold_long_kernel(FoldArgs a) {
   __shared__ __align__(16) double buf[LCH];
   const int lane = threadIdx.x;
-  const int64_t S = a.counters[5], nl = a.counters[9];
+  const int64_t nl = a.counters[9];
   for (int64_t q = blockIdx.x; q < nl; q += gridDim.x) {
-    const int64_t g = a.long_list[q];
-    int64_t s0, e0;
-    seg_range(a, S, g, s0, e0);
-    double *t = target_cell(a, a.sk[s0]);
+    const int64_t s0 = a.long_list[q];
+    const uint32_t key = a.sk[s0];
+    double *t = target_cell(a, key);
     double s = lane == 0 ? *t : 0.0;
     double v[LU];
+    uint32_t kk[LU];   // keys of the chunk's positions (past the end: not the segment's)
 #pragma unroll
     for (int u = 0; u < LU; u++) {
-      int64_t i = s0 + u * 64 + lane;
-      v[u] = i < e0 ? a.sv[i] : 0.0;
+      const int64_t i = s0 + u * 64 + lane;
+      kk[u] = i < a.n ? a.sk[i] : ~key;
+      v[u] = i < a.n ? a.sv[i] : 0.0;
     }
-    for (int64_t c0 = s0; c0 < e0; c0 += LCH) {
+    for (int64_t c0 = s0;; c0 += LCH) {
+      // the segment's values in this chunk are a prefix of it, in position
+      // order u * 64 + lane (keys are compared only here, after the previous
+      // chunk's additions, so waiting for them never delays those)
+      int m = 0;
+#pragma unroll
+      for (int u = 0; u < LU; u++) m += __popcll(__ballot(kk[u] == key));
       __syncthreads();   // lane 0 has finished with the previous chunk
 #pragma unroll
       for (int u = 0; u < LU; u++) buf[u * 64 + lane] = v[u];
       __syncthreads();
-      const int64_t c1 = c0 + LCH;
+      const bool more = m == LCH;
+      if (more) {
+        const int64_t c1 = c0 + LCH;
 #pragma unroll
-      for (int u = 0; u < LU; u++) {
-        int64_t i = c1 + u * 64 + lane;
-        v[u] = i < e0 ? a.sv[i] : 0.0;
+        for (int u = 0; u < LU; u++) {
+          const int64_t i = c1 + u * 64 + lane;
+          kk[u] = i < a.n ? a.sk[i] : ~key;
+          v[u] = i < a.n ? a.sv[i] : 0.0;
+        }
       }
       if (lane == 0) {
         // 32 parked values per round from two register sets: while one set's
         // 16 additions run, the other set is being refilled from LDS
-        const int m = (int)(e0 - c0 < LCH ? e0 - c0 : LCH);
         const double2 *b2 = reinterpret_cast<const double2 *>(buf);
         int i = 0;
         if (m >= 32) {
@@ -273,16 +315,10 @@ __global__ __launch_bounds__(64) void fold_long_kernel(FoldArgs a) {
         }
         for (; i < m; i++) s += buf[i];
       }
+      if (!more) break;
     }
     if (lane == 0) *t = s;
   }
-}
-
-unsigned grid_for(int64_t n, int64_t cap_blocks) {
-  int64_t b = (n + 255) / 256;
-  if (b < 1) b = 1;
-  if (b > cap_blocks) b = cap_blocks;
-  return (unsigned)b;
 }
 
 }  // namespace
@@ -330,7 +366,6 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   k.discard = discard;
   k.cell_bits = cell_bits;
   k.counters = R.counters;
-  k.n_pairs = n;
   hipLaunchKernelGGL(key_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, s, k);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -338,19 +373,18 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   double *sv;
   e = radix_sort_pairs(R.keys, R.cz, R.skeys, R.scz, n, key_bits, R.sort_tmp, R.sort_tmp_bytes, &sk, &sv, s);
   if (e != hipSuccess) return e;
-  e = segment_starts(sk, n, discard, R.gstart, R.counters, R.bcount, R.boff, s);
-  if (e != hipSuccess) return e;
   FoldArgs f;
   f.nz = R.d.nz;
   f.ldz = R.ldz;
   f.cell_bits = cell_bits;
+  f.n = n;
+  f.discard = discard;
   f.sk = sk;
   f.sv = sv;
-  f.seg = R.gstart;
   f.counters = R.counters;
   f.long_list = R.long_list;
   for (int i = 0; i < 4; i++) f.target[i] = i < ncomp ? targets[i] : nullptr;
-  hipLaunchKernelGGL(fold_kernel, dim3(grid_for(n, 4096)), dim3(256), 0, s, f);
+  hipLaunchKernelGGL(fold_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // at most n / LONG_SEG long segments
   int64_t nlong = n / LONG_SEG + 1;

@@ -133,16 +133,12 @@ struct Run {
   uint32_t *keys = nullptr, *skeys = nullptr;
   double *cz = nullptr, *scz = nullptr;   // contributions, unsorted and sorted
   int32_t *vals = nullptr;                // list of long segments
-  int32_t *gstart = nullptr;              // segment starts
   int32_t *pcount = nullptr;              // events per push workgroup
   bool imposed_e = false;                 // EzImposed/ErImposed may hold nonzero values
   int32_t *long_list = nullptr;           // long segments, folded one per wavefront
   uint8_t *gflag = nullptr;
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
-  // compaction scratch
-  int32_t *bcount = nullptr, *boff = nullptr;
-  int64_t nblk_cap = 0;
   // staging
   double *stage = nullptr;      // (nz+1)*(nr+1) doubles
   void *append_stage = nullptr;
@@ -193,12 +189,6 @@ hipError_t launch_append_commit(Run &R, int64_t n, int64_t n_active, hipStream_t
 // and its current-density update
 int cathode_emit_step(spf_ctx *c, int32_t run, int32_t cathode, int64_t *n_emitted);
 int cathode_current_density(spf_ctx *c, int32_t run, int32_t cathode);
-// positions of sorted keys sk[0..n) that start a segment (key differs from
-// its predecessor, not `discard`), in order, into out; counters[5] gets their
-// number, counters[6] the first discard position (preset to n)
-hipError_t segment_starts(const uint32_t *sk, int64_t n, uint32_t discard, int32_t *out, int64_t *counters,
-                          int32_t *bcount, int32_t *boff, hipStream_t s);
-int64_t compact_blocks(int64_t n);
 // deposition
 hipError_t deposit_currents(Run &R, hipStream_t s);   // Jz, Jr, Jphi += particle currents
 int deposit_cell_bits(int nz, int nr);
