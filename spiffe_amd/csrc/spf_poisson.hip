@@ -431,11 +431,19 @@ __device__ inline bool sor_lane_exact(const SorLane<G> &ln) {
 template <class G>
 __device__ inline void sor_block_sums(double (*lres)[G::NT], int k, int w, int lane, double (*wsum)[G::SW],
                                       double *out, int64_t stride) {
-  for (int j = 0; j < k; j++) {
-    double v = lres[j][threadIdx.x];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) wsum[j][w] = v;
-  }
+  // the steps' butterflies are independent: run them side by side so their
+  // cross-lane latencies overlap (each step's order of additions is the same)
+  double v[G::SK];
+#pragma unroll
+  for (int j = 0; j < G::SK; j++) v[j] = j < k ? lres[j][threadIdx.x] : 0.0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int j = 0; j < G::SK; j++) v[j] += __shfl_xor(v[j], off, 64);
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < G::SK; j++)
+      if (j < k) wsum[j][w] = v[j];
   __syncthreads();
   if (threadIdx.x < k) {
     double tot = 0;
