@@ -116,7 +116,10 @@ struct Run {
   int32_t *pid = nullptr;
   int16_t *gen = nullptr;
   uint32_t *status = nullptr;
-  int64_t *counters = nullptr;   // [0] np, [1] npActive, [2] n_ev, [3] n_srows, [4] n_lrows, [5] G, [6] nvalid, [7] spare
+  // [0] np, [1] npActive, [2] n_ev, [3] n_srows, [4] n_lrows, [7] newly active
+  // particles of a push, [8] event-ring overflow flag, [9] long segments of a
+  // deposit; [5], [6] unused
+  int64_t *counters = nullptr;
   unsigned long long *new_active = nullptr;
   int32_t *overflow = nullptr;
   // per-step events
