@@ -27,7 +27,11 @@ namespace spf {
 namespace {
 
 constexpr int ST = 256;          // threads per tile workgroup
-constexpr int SI = 16;           // pairs per thread
+// pairs per thread, fixed at build time (8, 12 and 16 measured alike)
+#ifndef SPF_SORT_ITEMS
+#define SPF_SORT_ITEMS 16
+#endif
+constexpr int SI = SPF_SORT_ITEMS;
 constexpr int TILE = ST * SI;    // pairs per tile
 constexpr int NW = ST / 64;      // wavefronts per tile workgroup
 

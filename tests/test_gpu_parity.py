@@ -175,6 +175,26 @@ def test_deposit_concentrated_bunch(ctx, n, seed):
     d.close()
 
 
+@pytest.mark.parametrize("nz,nr,n", [(5, 5, 3000), (4201, 1681, 200_000)])
+def test_deposit_key_widths(ctx, nz, nr, n):
+    """Grids whose (component, cell) keys take one radix pass (8-bit keys) and four passes
+    (25-bit keys, the 4201 x 1681-line gun), the bench grid's three being covered above."""
+    g = gun_grid(nz=nz, nr=nr, modes=TMTE | abi.SPACE_CHARGE)
+    f = np.zeros(g.shape, dtype=np.int16)
+    o, d = pair(ctx, g, f, cap=n)
+    p = random_particles(g, n, 9, margin=0.0)
+    rng = np.random.default_rng(10)
+    k = n // 3
+    p["z"][:k] = g.zmin + g.dz * (0.3 * g.nz + rng.random(k) * 1.5)
+    p["r"][:k] = g.dr * rng.random(k) * 1.5
+    o.append(p)
+    d.append(p)
+    o.add_particle_currents()
+    d.add_particle_currents()
+    assert_fields(o, d, J_NAMES)
+    d.close()
+
+
 def test_deposit_empty(ctx):
     g = gun_grid(modes=TMTE | abi.SPACE_CHARGE)
     o, d = pair(ctx, g, gun_flags(g), cap=10)
