@@ -284,8 +284,14 @@ int spf_step(spf_ctx *c, int32_t run, int32_t n_lines, const int32_t *line_dir, 
   R->time = R->time_step * R->d.dt + R->d.start_time;
   for (int32_t k = 0; k < (int32_t)R->cathodes.size(); k++)   // src/integrate.c:162-163
     if ((rc = cathode_emit_step(c, run, k, nullptr))) return rc;
-  if ((rc = spf_currents_reset(c, run))) return rc;
-  if ((R->d.modes & SPF_SPACE_CHARGE) && (rc = spf_currents_deposit(c, run))) return rc;
+  if (R->d.modes & SPF_SPACE_CHARGE) {
+    // reset and deposition (src/integrate.c:169-173) in one pass over the
+    // keys: the key kernel clears J, the folds add into it
+    PhaseTimer dt(c, SPF_T_DEPOSIT);
+    HIPCALL(c, deposit_currents(*R, c->stream, true));
+  } else if ((rc = spf_currents_reset(c, run))) {
+    return rc;
+  }
   for (int32_t k = 0; k < n_lines; k++)
     if ((rc = spf_currents_add_line(c, run, line_dir[k], line_pos[k], line_start[k], line_end[k], line_J[k])))
       return rc;

@@ -44,6 +44,8 @@ struct KeyArgs {
   uint32_t discard;
   int cell_bits;
   int64_t *counters;            // [9] (number of long segments) cleared by thread 0
+  double *clear[3];             // target arrays to set to +0 first (or null), clear_n cells each
+  int64_t clear_n;
 };
 
 // NGP value of the Jz/Jphi loops for cell row ir (0 <= ir <= nr-2), :92-100, :254-262
@@ -55,6 +57,11 @@ __device__ inline double jm_centered(double QVG, long long ir) {
 __global__ __launch_bounds__(256) void key_kernel(KeyArgs a) {
   int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (ip == 0) a.counters[9] = 0;
+  // the targets' reset (src/integrate.c:169 zeroes J before the deposition),
+  // done here to save a launch; the folds that add into them come later
+  for (int64_t q = ip; q < a.clear_n; q += (int64_t)gridDim.x * blockDim.x)
+    for (int c = 0; c < 3; c++)
+      if (a.clear[c]) a.clear[c][q] = 0.0;
   if (ip >= a.n_ub) return;
   int64_t np = *a.np;
   bool live = ip < np && (a.status[ip] & SPF_PART_ACTIVE);
@@ -331,8 +338,14 @@ int deposit_cell_bits(int nz, int nr) {
 }
 
 // ordered scatter-add of the listed components into their target arrays
-static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *const *targets, hipStream_t s) {
+// ordered scatter-add of the listed components into their target arrays; with
+// clear_first (at most three targets) the targets are set to +0 first
+static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *const *targets, hipStream_t s,
+                                  bool clear_first = false) {
   const int64_t n_ub = R.np_ub;
+  if (clear_first && (n_ub == 0 || ncomp == 0))
+    return launch_fill3(ncomp > 0 ? targets[0] : nullptr, ncomp > 1 ? targets[1] : nullptr,
+                        ncomp > 2 ? targets[2] : nullptr, R.cells, 0.0, s);
   if (n_ub == 0 || ncomp == 0) return hipSuccess;
   const int64_t n = n_ub * ncomp;
   const int cell_bits = deposit_cell_bits(R.d.nz, R.d.nr);
@@ -366,6 +379,8 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   k.discard = discard;
   k.cell_bits = cell_bits;
   k.counters = R.counters;
+  for (int i = 0; i < 3; i++) k.clear[i] = clear_first && i < ncomp ? targets[i] : nullptr;
+  k.clear_n = clear_first ? R.cells : 0;
   hipLaunchKernelGGL(key_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, s, k);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -392,7 +407,7 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   return hipGetLastError();
 }
 
-hipError_t deposit_currents(Run &R, hipStream_t s) {
+hipError_t deposit_currents(Run &R, hipStream_t s, bool clear_first) {
   int comps[3];
   double *targets[3];
   int nc = 0;
@@ -406,7 +421,7 @@ hipError_t deposit_currents(Run &R, hipStream_t s) {
     comps[nc] = COMP_JPHI;
     targets[nc++] = R.Jphi;
   }
-  return ordered_scatter(R, comps, nc, targets, s);
+  return ordered_scatter(R, comps, nc, targets, s, clear_first);
 }
 
 hipError_t launch_q_subtract(Run &R, hipStream_t s) {

@@ -193,7 +193,8 @@ hipError_t launch_append_commit(Run &R, int64_t n, int64_t n_active, hipStream_t
 int cathode_emit_step(spf_ctx *c, int32_t run, int32_t cathode, int64_t *n_emitted);
 int cathode_current_density(spf_ctx *c, int32_t run, int32_t cathode);
 // deposition
-hipError_t deposit_currents(Run &R, hipStream_t s);   // Jz, Jr, Jphi += particle currents
+// Jz, Jr, Jphi += particle currents; with clear_first they are set to +0 first
+hipError_t deposit_currents(Run &R, hipStream_t s, bool clear_first = false);
 int deposit_cell_bits(int nz, int nr);
 // stable radix sort of n (key, value) pairs on the low key_bits bits of the
 // keys; the input (k0, v0) and (k1, v1) alternate as pass input and output,
