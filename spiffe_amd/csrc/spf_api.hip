@@ -692,6 +692,7 @@ int spf_particles_push(spf_ctx *c, int32_t run, double zForcesStart) {
   a.evt = R->evt;
   a.evflag = R->evflag;
   a.pcount = R->pcount;
+  a.ev_total = (unsigned long long *)(R->counters + 5);
   a.ev_zold = R->ev[0]; a.ev_rold = R->ev[1]; a.ev_vz = R->ev[2]; a.ev_vr = R->ev[3];
   a.ev_pz0 = R->ev[4]; a.ev_pr0 = R->ev[5]; a.ev_rp0 = R->ev[6];
   a.last_old = R->last_old;
@@ -723,6 +724,7 @@ int spf_particles_push(spf_ctx *c, int32_t run, double zForcesStart) {
   w.flags = R->flags;
   w.evflag = R->evflag;
   w.pcount = R->pcount;
+  w.ev_total = (unsigned long long *)(R->counters + 5);
   w.n_ub = R->np_ub;
   w.npb = (R->np_ub + 255) / 256;
   w.evlist = R->evlist;

@@ -49,6 +49,7 @@ struct PushArgs {
   uint32_t *evt;
   uint8_t *evflag;
   int32_t *pcount;                  // events per push workgroup
+  unsigned long long *ev_total;     // events of the step, summed over the workgroups that have any
   double *ev_zold, *ev_rold, *ev_vz, *ev_vr, *ev_pz0, *ev_pr0, *ev_rp0;
   double *last_old;                 // old pz, pr, r_pphi of particle np-1
   unsigned long long *new_active;   // activations this step
@@ -65,6 +66,7 @@ struct RowsArgs {
   const int16_t *flags;
   const uint8_t *evflag;            // per particle: has an event this step
   const int32_t *pcount;            // events per push workgroup
+  unsigned long long *ev_total;     // events of the step; cleared for the next step
   int64_t npb, n_ub;                // push workgroups, particle slots they covered
   int32_t *evlist;                  // built here: particles with events, in index order
   int64_t *n_ev;
@@ -116,9 +118,9 @@ struct Run {
   int32_t *pid = nullptr;
   int16_t *gen = nullptr;
   uint32_t *status = nullptr;
-  // [0] np, [1] npActive, [2] n_ev, [3] n_srows, [4] n_lrows, [7] newly active
-  // particles of a push, [8] event-ring overflow flag, [9] long segments of a
-  // deposit; [5], [6] unused
+  // [0] np, [1] npActive, [2] n_ev, [3] n_srows, [4] n_lrows, [5] events of a
+  // push, [7] newly active particles of a push, [8] event-ring overflow flag,
+  // [9] long segments of a deposit; [6] unused
   int64_t *counters = nullptr;
   unsigned long long *new_active = nullptr;
   int32_t *overflow = nullptr;

@@ -377,7 +377,10 @@ __global__ __launch_bounds__(256) void push_kernel(PushArgs a, int64_t n_ub) {
   if (ip < np && ip < n_ub) any = push_one(a, ip, np);
   if (ip < n_ub) a.evflag[ip] = any ? 1 : 0;
   const int cnt = __syncthreads_count(any);
-  if (threadIdx.x == 0) a.pcount[blockIdx.x] = cnt;
+  if (threadIdx.x == 0) {
+    a.pcount[blockIdx.x] = cnt;
+    if (cnt) atomicAdd(a.ev_total, (unsigned long long)cnt);
+  }
 }
 
 // src/secondaryEmission.c:239-372
@@ -486,6 +489,16 @@ __global__ __launch_bounds__(ROWS_THREADS) void rows_kernel(RowsArgs a) {
   // ---- ordered event list: push workgroups in order (a scan of their
   // counts), then each workgroup's flagged particles in order (ballots)
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = ROWS_THREADS / 64;
+  if (*a.ev_total == 0) {
+    // no particle crossed a screen or was lost: no rows, nothing to compact;
+    // only this step's activations reach the counters
+    if (threadIdx.x == 0) {
+      *a.n_ev = 0;
+      a.np_active[0] += (int64_t)*a.new_active;
+      *a.new_active = 0;
+    }
+    return;
+  }
   int64_t n_ev = 0;
   for (int64_t c = 0; c < a.npb; c += blockDim.x) {
     const int64_t b = c + threadIdx.x;
@@ -649,6 +662,7 @@ __global__ __launch_bounds__(ROWS_THREADS) void rows_kernel(RowsArgs a) {
       int64_t removed = np - first;
       a.np[0] = first;
       a.np_active[0] += new_active - removed;
+      *a.ev_total = 0;
     }
     return;
   }
@@ -713,6 +727,7 @@ __global__ __launch_bounds__(ROWS_THREADS) void rows_kernel(RowsArgs a) {
   if (threadIdx.x == 0) {
     a.np[0] = new_np;
     a.np_active[0] += new_active - nlost;
+    *a.ev_total = 0;   // every thread read it before the first barrier
   }
 }
 
