@@ -161,6 +161,10 @@ int spf_field_download(spf_ctx *ctx, int32_t run, int32_t field, double *host);
 /* time_step and time = time_step*dt + start_time for the next step, src/integrate.c:138 */
 int spf_set_step(spf_ctx *ctx, int32_t run, int64_t time_step);
 int spf_get_step(spf_ctx *ctx, int32_t run, int64_t *time_step, double *time);
+/* integrate's z_forces_start (src/integrate.nl:19, default -1), which spf_step
+ * hands to the push as advance_particles' zForcesStart (src/integrate.c:186-188,
+ * src/advance_particles.c:190-200); a run starts with -1 */
+int spf_set_z_forces_start(spf_ctx *ctx, int32_t run, double z_forces_start);
 
 /* FDTD leapfrog of TM and TE fields; replaces advance_field_solutions, src/integrate.c:277-541 */
 int spf_fields_advance(spf_ctx *ctx, int32_t run);
@@ -380,8 +384,8 @@ int32_t spf_sor_batch_half_sweeps(void);
 /* ---- one whole step of the loop body, src/integrate.c:134-194 ----
  * ramped imposed E (if set); time <- step*dt+start; emission from every
  * cathode of the run; J reset; deposit (if SPF_SPACE_CHARGE); n_lines line
- * currents; Poisson correction (if set up); FDTD; push; current densities of
- * the cathodes; step++. */
+ * currents; Poisson correction (if set up); FDTD; push (with the run's
+ * z_forces_start); current densities of the cathodes; step++. */
 int spf_step(spf_ctx *ctx, int32_t run, int32_t n_lines, const int32_t *line_dir, const int64_t *line_pos,
              const int64_t *line_start, const int64_t *line_end, const double *line_J);
 

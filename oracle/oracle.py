@@ -394,7 +394,7 @@ class OracleSim:
         self.lib.oracle_compute_imposed_efield(ctypes.byref(self.f), 1 if initialize else 0, ctypes.byref(self.pc),
                                                ctypes.byref(self.last_field_factor))
 
-    def step(self, lines=(), screens=(), compat=True):
+    def step(self, lines=(), screens=(), compat=True, z_forces_start=-1.0):
         """Loop body of src/integrate.c:134-194, emission included, in the order of the device's spf_step."""
         f = self.f
         f.time = f.time_step * f.dti + f.start_time
@@ -408,7 +408,7 @@ class OracleSim:
         if self.pc is not None:
             self.add_poisson_correction((f.modes & 4) != 0)
         self.advance_field_solutions()
-        rows, lost = self.advance_particles(screens, compat)
+        rows, lost = self.advance_particles(screens, compat, z_forces_start)
         for k in range(self.n_cathodes):
             self.set_current_density(k)
         f.time_step += 1

@@ -16,12 +16,15 @@ namespace {
 
 constexpr double PI_D = 3.141592653589793238462643;
 
+// Allocation cleared on the context's stream: the run's later work on that
+// (non-blocking) stream is ordered after the clear, which a null-stream
+// hipMemset would not guarantee while other contexts keep the device busy.
 template <typename T>
-hipError_t dalloc(T **p, int64_t n) {
+hipError_t dalloc(T **p, int64_t n, hipStream_t s) {
   if (n <= 0) n = 1;
   hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)n);
   if (e != hipSuccess) return e;
-  return hipMemset(*p, 0, sizeof(T) * (size_t)n);
+  return hipMemsetAsync(*p, 0, sizeof(T) * (size_t)n, s);
 }
 
 
@@ -237,7 +240,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   };
 #define ALLOC(ptr, n)                                        \
   do {                                                       \
-    hipError_t e_ = dalloc(&(ptr), (n));                     \
+    hipError_t e_ = dalloc(&(ptr), (n), c->stream);          \
     if (e_ != hipSuccess) return bail(e_, "alloc " #ptr);    \
   } while (0)
   const int64_t cells = R->cells;
@@ -273,9 +276,9 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
     ALLOC(t, (int64_t)R->sort_tmp_bytes);
     R->sort_tmp = t;
   }
-  R->red_n = std::max<int64_t>(sor_tiles(*R) * sor_batch() * 4 * 2, 512) + 16;   // two check groups of partials
+  R->red_n = std::max<int64_t>(sor_tiles(*R) * sor_batch() * 4 * 2, 1024) + 16;   // two check groups of partials, or sum2 partials
   ALLOC(R->red, R->red_n);
-  ALLOC(R->red_counter, 4); ALLOC(R->sor_state, 4); ALLOC(R->sor_sums, 8);
+  ALLOC(R->red_counter, 4); ALLOC(R->sor_state, 4); ALLOC(R->sor_sums, 16);
   ALLOC(R->sor_masks, sor_mask_words(*R));
   ALLOC(R->Pdev, 16);
 #undef ALLOC
@@ -452,6 +455,13 @@ int spf_set_step(spf_ctx *c, int32_t run, int64_t time_step) {
   Run *R = get_run(c, run);
   if (!R) return fail(c, SPF_EINVAL, "bad run id");
   R->time_step = time_step;
+  return SPF_OK;
+}
+
+int spf_set_z_forces_start(spf_ctx *c, int32_t run, double z_forces_start) {
+  Run *R = get_run(c, run);
+  if (!R) return fail(c, SPF_EINVAL, "bad run id");
+  R->z_forces_start = z_forces_start;
   return SPF_OK;
 }
 

@@ -698,6 +698,7 @@ struct Deck {
     }
     if (opt.max_steps > 0 && n_steps > opt.max_steps) n_steps = opt.max_steps;
     create_run();
+    check(spf_set_z_forces_start(ctx, run, z_forces_start));
     // outputs
     SddsWriter lost_out;
     if (!lost_file.empty()) {

@@ -19,6 +19,7 @@
 // Jphi, ir == 0 and ir == nr-1 for Jr) are left out: J starts each step at +0
 // and a running sum of nonzero doubles from +0 is never -0, so adding a zero
 // cannot change a bit.  The charge removal adds -Q, which equals subtracting Q.
+#include <algorithm>
 #include <cstring>
 
 #include "spf_internal.h"
@@ -46,6 +47,8 @@ struct KeyArgs {
   int64_t *counters;            // [9] (number of long segments) cleared by thread 0
   double *clear[3];             // target arrays to set to +0 first (or null), clear_n cells each
   int64_t clear_n;
+  int zero_terms;               // COMP_Q: apply the signed-zero effect of the zero-weight terms
+  double *zero_target;          // the Q array those terms act on
 };
 
 // NGP value of the Jz/Jphi loops for cell row ir (0 <= ir <= nr-2), :92-100, :254-262
@@ -115,6 +118,27 @@ __global__ __launch_bounds__(256) void key_kernel(KeyArgs a) {
             iz = izh;
             ir = irh;
             v = -(a.Q[ip] * 1.0 * 1.0);
+          }
+          // The reference also subtracts Q*fb*fr, Q*fa*fl and Q*fa*fr from the
+          // cells (iz+1, ir), (iz, ir+1), (iz+1, ir+1) (src/poisson_correction.c:
+          // 222-236); without smearing these are Q*0, a zero with the sign of Q.
+          // Subtracting +0 changes no value; subtracting -0 (Q < 0) turns a -0
+          // cell into +0 and leaves every other value alone.  A cell can hold -0
+          // only while nothing but -0 has been added to a -0 start, so turning
+          // -0 into +0 before the ordered fold gives the reference's bits
+          // whatever the position of the zero terms in its sequence.
+          if (a.zero_terms && __builtin_signbit(a.Q[ip])) {
+            const long long zc[3][2] = {{izh + 1, irh}, {izh, irh + 1}, {izh + 1, irh + 1}};
+            const bool zb[3] = {(izh == 0 && a.left_neu) || (izh >= nz && a.right_neu) || (irh >= nr - 1 && a.upper_neu),
+                                (izh == 0 && a.left_neu) || (izh >= nz - 1 && a.right_neu) || (irh >= nr && a.upper_neu),
+                                (izh == 0 && a.left_neu) || (izh >= nz && a.right_neu) || (irh >= nr && a.upper_neu)};
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+              const int64_t pc = fix(a.ldz, (int)zc[t][0], (int)zc[t][1]);
+              if (zb[t] || (a.flags[pc] & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE))) continue;
+              double *q = a.zero_target + pc;
+              if (__double_as_longlong(*q) == (long long)0x8000000000000000ull) *q = 0.0;
+            }
           }
         }
       }
@@ -341,7 +365,7 @@ int deposit_cell_bits(int nz, int nr) {
 // ordered scatter-add of the listed components into their target arrays; with
 // clear_first (at most three targets) the targets are set to +0 first
 static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *const *targets, hipStream_t s,
-                                  bool clear_first = false) {
+                                  bool clear_first = false, bool zero_terms = false, double *sums = nullptr) {
   const int64_t n_ub = R.np_ub;
   if (clear_first && (n_ub == 0 || ncomp == 0))
     return launch_fill3(ncomp > 0 ? targets[0] : nullptr, ncomp > 1 ? targets[1] : nullptr,
@@ -381,9 +405,17 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   k.counters = R.counters;
   for (int i = 0; i < 3; i++) k.clear[i] = clear_first && i < ncomp ? targets[i] : nullptr;
   k.clear_n = clear_first ? R.cells : 0;
-  hipLaunchKernelGGL(key_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, s, k);
+  k.zero_terms = zero_terms ? 1 : 0;
+  k.zero_target = zero_terms ? targets[0] : nullptr;
+  // enough workgroups for the particles and, when it clears the targets, for
+  // a wide grid-stride clear (workgroups past n_ub only clear)
+  int64_t kblocks = (n_ub + 255) / 256;
+  if (clear_first) kblocks = std::max<int64_t>(kblocks, std::min<int64_t>(8192, (R.cells + 255) / 256));
+  hipLaunchKernelGGL(key_kernel, dim3((unsigned)kblocks), dim3(256), 0, s, k);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  // the first component's values in particle order, before the sort reuses their buffer
+  if (sums && (e = launch_sum2_vec(R.cz, n_ub, R.red, R.red_n, sums, s)) != hipSuccess) return e;
   uint32_t *sk;
   double *sv;
   e = radix_sort_pairs(R.keys, R.cz, R.skeys, R.scz, n, key_bits, R.sort_tmp, R.sort_tmp_bytes, &sk, &sv, s);
@@ -424,10 +456,71 @@ hipError_t deposit_currents(Run &R, hipStream_t s, bool clear_first) {
   return ordered_scatter(R, comps, nc, targets, s, clear_first);
 }
 
-hipError_t launch_q_subtract(Run &R, hipStream_t s) {
+hipError_t launch_q_subtract(Run &R, hipStream_t s, double *qreal_sums) {
   int comps[1] = {COMP_Q};
   double *targets[1] = {R.Q};
-  return ordered_scatter(R, comps, 1, targets, s);
+  if (qreal_sums && R.np_ub == 0) return launch_fill3(qreal_sums, nullptr, nullptr, 2, 0.0, s);
+  return ordered_scatter(R, comps, 1, targets, s, false, true, qreal_sums);
+}
+
+namespace {
+
+// Qreal of src/poisson_correction.c:221-224 (no smearing: fb = fl = 1) summed in
+// particle order from +0, one rounded addition at a time; the zero-weight
+// terms of :226-239 add +-0, which never changes a sum that starts at +0.
+__global__ __launch_bounds__(64) void qreal_serial_kernel(KeyArgs a, double *out) {
+  __shared__ double buf[64];
+  const int lane = threadIdx.x;
+  const int64_t np = *a.np;
+  double s = 0.0;
+  for (int64_t c = 0; c < np; c += 64) {
+    const int64_t ip = c + lane;
+    double t = 0.0;
+    bool take = false;
+    if (ip < np && (a.status[ip] & SPF_PART_ACTIVE)) {
+      const double th = (a.z[ip] - a.zmin) / a.dz + 0.5, yh = a.r[ip] / a.dr + 0.5;
+      if (th > -1.0 && th < a.nz + 1.0 && yh > -1.0 && yh < a.nr + 1.0) {
+        const long long iz = (long long)th, ir = (long long)yh;
+        if (iz >= 0 && iz < a.nz && ir >= 0 && ir < a.nr) {
+          const int16_t f = a.flags[fix(a.ldz, (int)iz, (int)ir)];
+          take = !((f & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE)) || (iz == 0 && a.left_neu) ||
+                   (iz >= a.nz - 1 && a.right_neu) || (ir >= a.nr - 1 && a.upper_neu));
+          t = a.Q[ip] * 1.0 * 1.0;
+        }
+      }
+    }
+    buf[lane] = take ? t : 0.0;
+    const unsigned long long m = __ballot(take);
+    __syncthreads();
+    if (lane == 0)
+      for (int k = 0; k < 64; k++)
+        if ((m >> k) & 1ull) s += buf[k];
+    __syncthreads();
+  }
+  if (lane == 0) *out = s;
+}
+
+}  // namespace
+
+hipError_t launch_qreal_serial(Run &R, double *out, hipStream_t s) {
+  KeyArgs k{};
+  k.nz = R.d.nz;
+  k.nr = R.d.nr;
+  k.ldz = R.ldz;
+  k.zmin = R.d.zmin;
+  k.dz = R.d.dz;
+  k.dr = R.d.dr;
+  k.Q = R.P[0];
+  k.z = R.P[1];
+  k.r = R.P[2];
+  k.status = R.status;
+  k.flags = R.flags;
+  k.left_neu = R.d.left_bc == SPF_NEUMANN;
+  k.right_neu = R.d.right_bc == SPF_NEUMANN;
+  k.upper_neu = R.d.upper_bc == SPF_NEUMANN;
+  k.np = R.counters;
+  hipLaunchKernelGGL(qreal_serial_kernel, dim3(1), dim3(64), 0, s, k, out);
+  return hipGetLastError();
 }
 
 }  // namespace spf

@@ -158,13 +158,14 @@ struct Run {
   double *red = nullptr;        // reduction partials
   int64_t red_n = 0;
   int32_t *red_counter = nullptr;
-  int64_t *sor_state = nullptr; // [0] done, [1] iterations
+  int64_t *sor_state = nullptr; // [0] done, [1] iterations, [2] stop left to the host (within rounding)
   uint32_t *sor_masks = nullptr; // per-tile lane masks of the SOR batches
   double *sor_sums = nullptr;   // [0] drive sum, [1] last residual sum, [2..] misc
   double last_field_factor = -1;
   std::vector<double> screen_z;
   std::vector<int> screen_dir;
   double ramp_time = 0, flat_top_time = 0;
+  double z_forces_start = -1;   // integrate's z_forces_start (src/integrate.nl:19), used by spf_step
   // cathode emission: host-side state, one uniform stream per run
   EmitRng emit_rng;
   std::vector<Cathode> cathodes;
@@ -206,7 +207,23 @@ hipError_t radix_sort_pairs(uint32_t *k0, double *v0, uint32_t *k1, double *v1, 
                             size_t tmp_bytes, uint32_t **kres, double **vres, hipStream_t s);
 // Poisson
 hipError_t launch_gauss_q(Run &R, hipStream_t s);
-hipError_t launch_q_subtract(Run &R, hipStream_t s);
+// Q -= particle charge (src/poisson_correction.c:184-243); with qreal_sums the
+// fixed-order sum of the removed charges' negatives and of their magnitudes
+// go to qreal_sums[0], [1]
+hipError_t launch_q_subtract(Run &R, hipStream_t s, double *qreal_sums = nullptr);
+// Qreal summed in particle order, one rounded addition at a time, as the reference does
+hipError_t launch_qreal_serial(Run &R, double *out, hipStream_t s);
+// fixed-order plain and absolute sums (out[0], out[1]) of a region or a vector;
+// partials needs 1024 entries
+hipError_t launch_sum2_region(const double *a, int ldz, int iz_n, int ir_n, double *partials, int64_t pn,
+                              double *out, hipStream_t s);
+hipError_t launch_sum2_vec(const double *a, int64_t n, double *partials, int64_t pn, double *out, hipStream_t s);
+// serial sums in the reference's loop orders: iz outer / ir inner (absval: of |a|), and ir outer / iz inner
+hipError_t launch_serial_sum_iz_ir(const double *a, int ldz, int iz_n, int ir_n, int absval, double *out,
+                                   hipStream_t s);
+hipError_t launch_serial_sum_ir_iz(const double *a, int ldz, int iz_n, int ir_n, double *out, hipStream_t s);
+// bound on how far sums of n terms with magnitude sum `mag` can differ between two summation orders
+double order_margin(int64_t n, double mag);
 hipError_t launch_q_to_rhs(Run &R, hipStream_t s);
 hipError_t launch_sum_abs(const double *a, int ldz, int iz_n, int ir_n, double *partials, int64_t pn, double *out,
                           hipStream_t s);
@@ -226,6 +243,10 @@ hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const doubl
                             int partial_row, int force, hipStream_t s, int64_t check_row = 0, int64_t check_n0 = 0,
                             int check_k = 0, double accuracy = 0);
 hipError_t launch_sor_check(Run &R, int64_t check_row, int64_t n0, int k, double accuracy, hipStream_t s);
+// half-sweep n in place with the reference's expressions, |residual| per point to dump[ir*nz+iz]
+hipError_t launch_sor_exact_sweep(Run &R, double *P, const double *rhs, int64_t n, double *dump, hipStream_t s);
+// done flag, iteration count and the host-decision flag back to "running"
+hipError_t launch_sor_state_clear(Run &R, hipStream_t s);
 hipError_t launch_field_from_psi(Run &R, int imposed, double field_factor, hipStream_t s);
 hipError_t launch_zero_psi(Run &R, hipStream_t s);
 

@@ -485,6 +485,7 @@ struct SorArgs {
   const double *check_partials;
   double *sums;          // [0] drive sum, [1] residual of the last checked half-sweep
   double accuracy;
+  double rel;            // summation-order margin of the stopping rule (sor_stop_rule)
   int64_t check_n0;
   int check_k;
 };
@@ -510,25 +511,60 @@ __device__ inline double sor_partial_total(const double *row, int ntiles, int l)
 }
 
 // The stopping rule (src/poisson.c:125-126) after half-sweeps n0, n0+1, ...
-// in order, on their totals tot[0 .. k-1]; one thread.
-__device__ inline void sor_stop_rule(const double *tot, int k, int64_t n0, double accuracy, int64_t *state,
-                                     double *sums) {
+// in order, on their totals tot[0 .. k-1]; one thread.  The reference adds
+// |residual| serially (ir outer, iz inner) and the drive sum serially (iz
+// outer, ir inner); the device's totals are fixed-order tree sums of the same
+// non-negative terms.  Two summation orders of n non-negative terms differ by
+// at most rel = 2g/(1-g) times the sum (g = (n-1)u/(1-(n-1)u), u = 2^-53), so
+// the comparison is decided here only when it holds for every value the
+// serial sums can take: residual total t in [T(1-rel), T(1+rel)], threshold
+// RN(accuracy*D) with D in [S(1-rel), S(1+rel)].  Otherwise the solve stops
+// with state[2] set, and the host decides that half-sweep from the serial
+// sums.  A NaN total (a NaN term, so NaN in any order) never converges; totals
+// near the overflow range are left to the host.
+__device__ inline void sor_stop_rule(const double *tot, int k, int64_t n0, double accuracy, double rel,
+                                     int64_t *state, double *sums) {
+  const double drive = sums[0];
+  const double u2 = 2.220446049250313e-16;
+  const double big = 0x1p1000;
+  const double thr_lo = accuracy * (drive * (1 - rel)) * (1 - u2) - 0x1p-1074;
+  const double thr_hi = accuracy * (drive * (1 + rel)) * (1 + u2) + 0x1p-1074;
+  const bool far = drive < big && accuracy * drive < big;   // NaN drive: every comparison is false
   for (int jj = 0; jj < k; jj++) {
     int64_t n = n0 + jj;
     double total = tot[jj];
     sums[1] = total;
-    if (n > 0 && (total < accuracy * sums[0] || total == 0)) {
+    if (n == 0) continue;
+    if (total == 0) {   // every term is zero, in any order
       state[0] = 1;
       state[1] = n;
       return;
     }
+    if (total != total || drive != drive) continue;   // NaN stays NaN in any order: no stop
+    if (!(total < big) || !far) {
+      state[2] = 1;
+      state[0] = 1;
+      state[1] = n;
+      return;
+    }
+    const double t_hi = total * (1 + rel) * (1 + u2), t_lo = total * (1 - rel) * (1 - u2);
+    if (t_hi < thr_lo) {
+      state[0] = 1;
+      state[1] = n;
+      return;
+    }
+    if (t_lo >= thr_hi) continue;
+    state[2] = 1;   // within rounding of the threshold: the host decides from the serial sums
+    state[0] = 1;
+    state[1] = n;
+    return;
   }
 }
 
 // all threads of a workgroup of nt threads (a multiple of 32): totals of the
 // k half-sweeps into tot, then the stopping rule
 __device__ inline void sor_group_check(const double *partials, int ntiles, int k, int64_t n0, double accuracy,
-                                       int64_t *state, double *sums, double *tot, int nt) {
+                                       double rel, int64_t *state, double *sums, double *tot, int nt) {
   if (state[0]) return;
   const int g = threadIdx.x >> 5, l = threadIdx.x & 31;
   for (int j0 = 0; j0 < k; j0 += nt / 32) {
@@ -539,7 +575,7 @@ __device__ inline void sor_group_check(const double *partials, int ntiles, int k
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) sor_stop_rule(tot, k, n0, accuracy, state, sums);
+  if (threadIdx.x == 0) sor_stop_rule(tot, k, n0, accuracy, rel, state, sums);
 }
 
 __global__ __launch_bounds__(BatchCfg::NT) void sor_batch_kernel(SorArgs a) {
@@ -549,7 +585,8 @@ __global__ __launch_bounds__(BatchCfg::NT) void sor_batch_kernel(SorArgs a) {
     // CU the tiles leave free (tiles that start after it stops the solve
     // return at once; their output is never used)
     __shared__ double tot[CB / 32];
-    sor_group_check(a.check_partials, a.m.ntiles, a.check_k, a.check_n0, a.accuracy, a.state, a.sums, tot, G::NT);
+    sor_group_check(a.check_partials, a.m.ntiles, a.check_k, a.check_n0, a.accuracy, a.rel, a.state, a.sums, tot,
+                    G::NT);
     return;
   }
   __shared__ double2 xrow[2][G::SW][2][64];   // [parity][wave][first,last row][lane]
@@ -591,15 +628,64 @@ struct CheckArgs {
   const double *partials;
   int ntiles, k;
   int64_t n0;
-  double accuracy;
-  int64_t *state;   // [0] done, [1] iterations
+  double accuracy, rel;
+  int64_t *state;   // [0] done, [1] iterations, [2] left to the host
   double *sums;     // [0] drive sum, [1] residual of the last checked half-sweep
 };
 
 // the stopping rule of one check group on its own (the last group of a solve)
 __global__ __launch_bounds__(CB) void sor_check_kernel(CheckArgs a) {
   __shared__ double tot[CB / 32];
-  sor_group_check(a.partials, a.ntiles, a.k, a.n0, a.accuracy, a.state, a.sums, tot, CB);
+  sor_group_check(a.partials, a.ntiles, a.k, a.n0, a.accuracy, a.rel, a.state, a.sums, tot, CB);
+}
+
+// One half-sweep n of src/poisson.c:58-120 written as the reference writes it,
+// in place (a colour-n point reads only points of the other colour and
+// itself), with plain division; |residual| of every point of the half-sweep's
+// loop goes to dump[ir*nz + iz] (0 where the point is not updated), so a
+// serial sum of dump in index order is the reference's abs_total_residual.
+// The result equals the batch kernel's half-sweep n bit for bit: the same
+// operations in the same order, and the batch's fast division is correctly
+// rounded where it is used.
+struct ExactSweepArgs {
+  SorCommon m;
+  double *P;
+  double *dump;
+  int64_t n;
+};
+
+__global__ __launch_bounds__(256) void sor_exact_sweep_kernel(ExactSweepArgs x) {
+  const SorCommon &a = x.m;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)a.nz * (a.nr - 1)) return;
+  const int ir = (int)(i / a.nz), iz = (int)(i % a.nz);
+  double r = 0.0;
+  const bool in_range = (iz >= 1 && iz <= a.nz - 2) || (iz == 0 && a.left_dir) || (iz == a.nz - 1 && a.right_dir);
+  const int64_t p = fix(a.ldz, iz, ir);
+  if (in_range && (ir + iz) % 2 == x.n % 2 && !(a.flags[p] & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE))) {
+    double *P = x.P;
+    const double left = iz == 0 ? P[p + 1] : P[p - 1];
+    const double right = iz == a.nz - 1 ? P[p - 1] : P[p + 1];
+    const double D = a.rhs ? a.rhs[p] : 0.0;
+    const double c = a.c[ir];
+    double residual, diag;
+    if (ir == 0) {
+      diag = a.e - 2 * c;
+      residual = a.a * (right + left) + 4 * c * P[p + a.ldz] + diag * P[p] - D;
+    } else {
+      diag = a.e;
+      residual = a.a * (right + left) + c * P[p + a.ldz] + a.d[ir] * P[p - a.ldz] + a.e * P[p] - D;
+    }
+    P[p] -= a.omega[x.n] * residual / diag;
+    r = fabs(residual);
+  }
+  x.dump[i] = r;
+}
+
+__global__ void sor_state_clear_kernel(int64_t *state) {
+  state[0] = 0;
+  state[1] = -1;
+  state[2] = 0;
 }
 
 struct QArgs {
@@ -713,6 +799,7 @@ __global__ __launch_bounds__(256) void e_from_psi_kernel(EArgs a) {
 __global__ void sor_reset_kernel(int64_t *state, int32_t *counter, double *sums) {
   state[0] = 0;
   state[1] = -1;
+  state[2] = 0;
   *counter = 0;
   sums[0] = 0;
   sums[1] = 0;
@@ -779,6 +866,10 @@ SorCommon sor_common(Run &R, const double *rhs, int oz, int orr) {
 
 }  // namespace
 
+// relative margin between summation orders of the stopping rule's sums, each
+// of at most nz*(nr-1) non-negative terms
+double sor_order_margin(const Run &R) { return order_margin((int64_t)R.d.nz * (R.d.nr - 1), 1.0) - 0x1p-1074; }
+
 int64_t sor_tiles(const Run &R) {
   int64_t tz = (R.d.nz + 1 + BatchCfg::OZ - 1) / BatchCfg::OZ, tr = (R.d.nr + 1 + BatchCfg::OR - 1) / BatchCfg::OR;
   return tz * tr;
@@ -811,6 +902,7 @@ hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const doubl
   a.check_partials = R.red + check_row * a.m.ntiles;
   a.sums = R.sor_sums;
   a.accuracy = accuracy;
+  a.rel = sor_order_margin(R);
   a.check_n0 = check_n0;
   a.check_k = check_k;
   const unsigned grid = (unsigned)a.m.ntiles + (check_k > 0 ? 1u : 0u);
@@ -825,9 +917,26 @@ hipError_t launch_sor_check(Run &R, int64_t check_row, int64_t n0, int k, double
   c.k = k;
   c.n0 = n0;
   c.accuracy = accuracy;
+  c.rel = sor_order_margin(R);
   c.state = R.sor_state;
   c.sums = R.sor_sums;
   hipLaunchKernelGGL(sor_check_kernel, dim3(1), dim3(CB), 0, s, c);
+  return hipGetLastError();
+}
+
+hipError_t launch_sor_exact_sweep(Run &R, double *P, const double *rhs, int64_t n, double *dump, hipStream_t s) {
+  ExactSweepArgs x;
+  x.m = sor_common(R, rhs, BatchCfg::OZ, BatchCfg::OR);
+  x.P = P;
+  x.dump = dump;
+  x.n = n;
+  const int64_t pts = (int64_t)R.d.nz * (R.d.nr - 1);
+  hipLaunchKernelGGL(sor_exact_sweep_kernel, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, s, x);
+  return hipGetLastError();
+}
+
+hipError_t launch_sor_state_clear(Run &R, hipStream_t s) {
+  hipLaunchKernelGGL(sor_state_clear_kernel, dim3(1), dim3(1), 0, s, R.sor_state);
   return hipGetLastError();
 }
 

@@ -173,6 +173,10 @@ class DeviceRun:
     def set_step(self, step):
         self.ctx.check(self.lib.spf_set_step(self.ctx.h, self.id, int(step)))
 
+    def set_z_forces_start(self, z):
+        """integrate's z_forces_start, used by step() (src/integrate.c:186-188)."""
+        self.ctx.check(self.lib.spf_set_z_forces_start(self.ctx.h, self.id, float(z)))
+
     def get_step(self):
         s, t = ctypes.c_int64(), ctypes.c_double()
         self.ctx.check(self.lib.spf_get_step(self.ctx.h, self.id, ctypes.byref(s), ctypes.byref(t)))
