@@ -198,6 +198,11 @@ int spf_poisson_setup(spf_ctx *ctx, int32_t run, const spf_poisson_params *p);
 /* replaces add_poisson_correction, src/poisson_correction.c:73-454; synchronizing
  * when a solve happens.  *iterations: solve result, -1 not converged, -2 no solve. */
 int spf_poisson_correct(spf_ctx *ctx, int32_t run, int32_t space_charge, int64_t *iterations);
+/* No counterpart in the reference (which prints the count when verbose,
+ * src/poisson_correction.c:279-285): the result of the last correction that
+ * solved (-1 not converged, -2 none yet), the number of corrections that
+ * solved and the half-sweeps they took in all, for telemetry */
+int spf_poisson_stats(spf_ctx *ctx, int32_t run, int64_t *last_iterations, int64_t *solves, int64_t *half_sweeps);
 /* replaces solve_poisson_cyl, src/poisson.c:17-130, on the run's Psi (which=0,
  * rhs = Q) or PsiImposed (which=1, no rhs).  Synchronizing. */
 int spf_poisson_solve(spf_ctx *ctx, int32_t run, int32_t which, double accuracy, int64_t max_iterations,

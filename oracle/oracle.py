@@ -11,6 +11,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libspf_oracle.so")
+LIB_OMP = os.path.join(HERE, "libspf_oracle_omp.so")   # the same sources with OpenMP (CPU baseline only)
 
 _d, _dp = ctypes.c_double, ctypes.POINTER(ctypes.c_double)
 _i16p = ctypes.POINTER(ctypes.c_int16)
@@ -105,6 +106,7 @@ FIELD_NAMES = ["Ez", "Jz", "Er", "Jr", "Bphi", "Bphi2", "Ephi", "Jphi", "Br", "B
                "EzImposed", "ErImposed", "PsiImposed", "BzImposed", "BrImposed", "Psi", "Q"]
 
 _lib = None
+_lib_omp = None
 
 
 def build():
@@ -115,68 +117,79 @@ def build():
     return LIB
 
 
-def load():
-    global _lib
+def load(threaded=False):
+    """The serial restatement (the checker), or with threaded=True its OpenMP
+    build (timing of the all-cores CPU baseline)."""
+    global _lib, _lib_omp
+    if threaded:
+        if _lib_omp is None:
+            if not os.path.exists(LIB_OMP):
+                build()
+            _lib_omp = _prototypes(ctypes.CDLL(LIB_OMP))
+        return _lib_omp
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        lib = ctypes.CDLL(LIB)
-        P = ctypes.POINTER
-        lib.oracle_calculate_flags.argtypes = [_i16p, ctypes.c_int32, ctypes.c_int32]
-        lib.oracle_advance_field_solutions.argtypes = [P(OFields)]
-        lib.oracle_zero_current_arrays.argtypes = [P(OFields)]
-        lib.oracle_multiply_current_arrays.argtypes = [P(OFields), _d]
-        lib.oracle_add_line_current.argtypes = [P(OFields), ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,
-                                                ctypes.c_int64, _d]
-        lib.oracle_add_particle_currents.argtypes = [P(OFields), P(OBeam)]
-        lib.oracle_check_loss_code.argtypes = [ctypes.c_int64, ctypes.c_int64, P(OFields)]
-        lib.oracle_check_loss_code.restype = ctypes.c_short
-        lib.oracle_interpolate_fields.argtypes = [_dp] * 6 + [_d, _d, P(OFields)]
-        lib.oracle_find_loss_coordinates.argtypes = [_dp, _dp, P(ctypes.c_short), _d, _d, _d, _d, P(OFields)]
-        lib.oracle_advance_particles.argtypes = [P(OFields), P(OBeam), P(OScreen), ctypes.c_int32, _d,
-                                                 ctypes.c_int32, P(OScreenRow), ctypes.c_int64,
-                                                 P(ctypes.c_int64), P(OLostRow), ctypes.c_int64,
-                                                 P(ctypes.c_int64)]
-        lib.oracle_solve_poisson_cyl.argtypes = [_dp, _dp, _i16p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
-                                                 ctypes.c_int64, _d, _d, _d, ctypes.c_int64, _dp]
-        lib.oracle_solve_poisson_cyl.restype = ctypes.c_int64
-        lib.oracle_poisson_omega_schedule.argtypes = [ctypes.c_int64, ctypes.c_int64, _d, _d, ctypes.c_int64, _dp]
-        lib.oracle_add_poisson_correction.argtypes = [P(OFields), P(OBeam), ctypes.c_int32, P(OPoisson)]
-        lib.oracle_compute_imposed_efield.argtypes = [P(OFields), ctypes.c_int32, P(OPoisson), _dp]
-        lib.oracle_antenna_current.argtypes = [_dp, _dp, ctypes.c_int64, P(ctypes.c_int64), _d, _d, _d, _d,
-                                               P(ctypes.c_int32)]
-        lib.oracle_antenna_current.restype = _d
-        u64p = ctypes.POINTER(ctypes.c_uint64)
-        lib.oracle_philox4x64_10.argtypes = [u64p, u64p, u64p]
-        lib.oracle_rng_seed.argtypes = [P(ORng), ctypes.c_uint64]
-        lib.oracle_rng_next64.argtypes = [P(ORng)]
-        lib.oracle_rng_next64.restype = ctypes.c_uint64
-        lib.oracle_rng_uniform.argtypes = [P(ORng)]
-        lib.oracle_rng_uniform.restype = _d
-        lib.oracle_rng_gauss.argtypes = [P(ORng)]
-        lib.oracle_rng_gauss.restype = _d
-        lib.oracle_radical_inverse.argtypes = [ctypes.c_uint64, ctypes.c_int32]
-        lib.oracle_radical_inverse.restype = _d
-        lib.oracle_cathodes_create.restype = ctypes.c_void_p
-        lib.oracle_cathodes_destroy.argtypes = [ctypes.c_void_p]
-        lib.oracle_cathodes_rng.argtypes = [ctypes.c_void_p]
-        lib.oracle_cathodes_rng.restype = P(ORng)
-        lib.oracle_cathodes_next_id.argtypes = [ctypes.c_void_p]
-        lib.oracle_cathodes_next_id.restype = ctypes.c_int64
-        lib.oracle_cathode_define.argtypes = [ctypes.c_void_p, P(OFields), P(OCathodeDesc), P(ctypes.c_int32),
-                                              P(ctypes.c_int32), ctypes.c_char_p, ctypes.c_int64]
-        lib.oracle_field_emission_current_density.argtypes = [_d, _d, _d]
-        lib.oracle_field_emission_current_density.restype = _d
-        lib.oracle_set_current_density.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(OFields)]
-        lib.oracle_emit_electrons_from_cathode.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(OBeam), P(OFields),
-                                                           ctypes.c_char_p, ctypes.c_int64]
-        lib.oracle_cathode_state.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, P(ctypes.c_int64)]
-        lib.oracle_ellip_ke.argtypes = [_d, _dp, _dp]
-        lib.oracle_b_ring.argtypes = [_dp, _dp, _d, _d, _d]
-        lib.oracle_b_solenoid.argtypes = [_dp, _dp, _d, _d, _d, _d, _d, ctypes.c_int64, ctypes.c_int64]
-        lib.oracle_define_solenoid.argtypes = [P(OFields), P(OSolenoidDesc), _dp, ctypes.c_char_p, ctypes.c_int64]
-        _lib = lib
+        _lib = _prototypes(ctypes.CDLL(LIB))
     return _lib
+
+
+def _prototypes(lib):
+    P = ctypes.POINTER
+    lib.oracle_calculate_flags.argtypes = [_i16p, ctypes.c_int32, ctypes.c_int32]
+    lib.oracle_advance_field_solutions.argtypes = [P(OFields)]
+    lib.oracle_zero_current_arrays.argtypes = [P(OFields)]
+    lib.oracle_multiply_current_arrays.argtypes = [P(OFields), _d]
+    lib.oracle_add_line_current.argtypes = [P(OFields), ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,
+                                            ctypes.c_int64, _d]
+    lib.oracle_add_particle_currents.argtypes = [P(OFields), P(OBeam)]
+    lib.oracle_check_loss_code.argtypes = [ctypes.c_int64, ctypes.c_int64, P(OFields)]
+    lib.oracle_check_loss_code.restype = ctypes.c_short
+    lib.oracle_interpolate_fields.argtypes = [_dp] * 6 + [_d, _d, P(OFields)]
+    lib.oracle_find_loss_coordinates.argtypes = [_dp, _dp, P(ctypes.c_short), _d, _d, _d, _d, P(OFields)]
+    lib.oracle_advance_particles.argtypes = [P(OFields), P(OBeam), P(OScreen), ctypes.c_int32, _d,
+                                             ctypes.c_int32, P(OScreenRow), ctypes.c_int64,
+                                             P(ctypes.c_int64), P(OLostRow), ctypes.c_int64,
+                                             P(ctypes.c_int64)]
+    lib.oracle_solve_poisson_cyl.argtypes = [_dp, _dp, _i16p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
+                                             ctypes.c_int64, _d, _d, _d, ctypes.c_int64, _dp]
+    lib.oracle_solve_poisson_cyl.restype = ctypes.c_int64
+    lib.oracle_poisson_omega_schedule.argtypes = [ctypes.c_int64, ctypes.c_int64, _d, _d, ctypes.c_int64, _dp]
+    lib.oracle_add_poisson_correction.argtypes = [P(OFields), P(OBeam), ctypes.c_int32, P(OPoisson)]
+    lib.oracle_compute_imposed_efield.argtypes = [P(OFields), ctypes.c_int32, P(OPoisson), _dp]
+    lib.oracle_antenna_current.argtypes = [_dp, _dp, ctypes.c_int64, P(ctypes.c_int64), _d, _d, _d, _d,
+                                           P(ctypes.c_int32)]
+    lib.oracle_antenna_current.restype = _d
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    lib.oracle_philox4x64_10.argtypes = [u64p, u64p, u64p]
+    lib.oracle_rng_seed.argtypes = [P(ORng), ctypes.c_uint64]
+    lib.oracle_rng_next64.argtypes = [P(ORng)]
+    lib.oracle_rng_next64.restype = ctypes.c_uint64
+    lib.oracle_rng_uniform.argtypes = [P(ORng)]
+    lib.oracle_rng_uniform.restype = _d
+    lib.oracle_rng_gauss.argtypes = [P(ORng)]
+    lib.oracle_rng_gauss.restype = _d
+    lib.oracle_radical_inverse.argtypes = [ctypes.c_uint64, ctypes.c_int32]
+    lib.oracle_radical_inverse.restype = _d
+    lib.oracle_cathodes_create.restype = ctypes.c_void_p
+    lib.oracle_cathodes_destroy.argtypes = [ctypes.c_void_p]
+    lib.oracle_cathodes_rng.argtypes = [ctypes.c_void_p]
+    lib.oracle_cathodes_rng.restype = P(ORng)
+    lib.oracle_cathodes_next_id.argtypes = [ctypes.c_void_p]
+    lib.oracle_cathodes_next_id.restype = ctypes.c_int64
+    lib.oracle_cathode_define.argtypes = [ctypes.c_void_p, P(OFields), P(OCathodeDesc), P(ctypes.c_int32),
+                                          P(ctypes.c_int32), ctypes.c_char_p, ctypes.c_int64]
+    lib.oracle_field_emission_current_density.argtypes = [_d, _d, _d]
+    lib.oracle_field_emission_current_density.restype = _d
+    lib.oracle_set_current_density.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(OFields)]
+    lib.oracle_emit_electrons_from_cathode.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(OBeam), P(OFields),
+                                                       ctypes.c_char_p, ctypes.c_int64]
+    lib.oracle_cathode_state.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, P(ctypes.c_int64)]
+    lib.oracle_ellip_ke.argtypes = [_d, _dp, _dp]
+    lib.oracle_b_ring.argtypes = [_dp, _dp, _d, _d, _d]
+    lib.oracle_b_solenoid.argtypes = [_dp, _dp, _d, _d, _d, _d, _d, ctypes.c_int64, ctypes.c_int64]
+    lib.oracle_define_solenoid.argtypes = [P(OFields), P(OSolenoidDesc), _dp, ctypes.c_char_p, ctypes.c_int64]
+    return lib
 
 
 def derive_flags(marks):
@@ -189,9 +202,10 @@ def derive_flags(marks):
 class OracleSim:
     """FIELDS + BEAM state of the restatement in numpy arrays ([iz][ir] layout)."""
 
-    def __init__(self, grid, flags, max_particles=0, QoMC=None, stiffness=1.0, has_solenoid=False, constants=None):
+    def __init__(self, grid, flags, max_particles=0, QoMC=None, stiffness=1.0, has_solenoid=False, constants=None,
+                 threaded=False):
         from spiffe_amd import constants as K   # the same constants the product passes to the device
-        self.lib = load()
+        self.lib = load(threaded)
         self.grid = grid
         shape = grid.shape
         self.arr = {n: np.zeros(shape, dtype=np.float64) for n in FIELD_NAMES}

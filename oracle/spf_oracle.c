@@ -107,6 +107,7 @@ void oracle_advance_field_solutions(ofields *f) {
     double *Ez = f->Ez, *Er = f->Er, *Bphi = f->Bphi, *Bphi2 = f->Bphi2, *Jz = f->Jz, *Jr = f->Jr;
     /* -- Ez, :323-359 */
     factor = 1 / (eps0 * mu0);
+#pragma omp parallel for private(iz, rp, rm, cp, cm) schedule(static)
     for (ir = 0; ir < nr; ir++) {
       rp = (ir + 0.5) * dr;
       rm = (ir - 0.5) * dr;
@@ -138,6 +139,7 @@ void oracle_advance_field_solutions(ofields *f) {
 
     /* -- Er, :361-380 */
     factor = 1 / (mu0 * eps0 * dz);
+#pragma omp parallel for private(iz) schedule(static)
     for (ir = 1; ir < nr; ir++) {
       for (iz = 0; iz < nz; iz++) {
         if (!(FL(iz, ir) & OFL_ER_ZERO))
@@ -157,6 +159,7 @@ void oracle_advance_field_solutions(ofields *f) {
 
     /* -- Bphi, :382-432 */
     factor = (f->rmax - f->dr / 2) / (f->rmax + f->dr / 2);
+#pragma omp parallel for private(iz, dB) schedule(static)
     for (ir = 1; ir < nr; ir++) {
       for (iz = 1; iz < nz; iz++) {
         if (!(FL(iz, ir) & OFL_BPHI_ZERO)) {
@@ -199,6 +202,7 @@ void oracle_advance_field_solutions(ofields *f) {
     double *Ephi = f->Ephi, *Jphi = f->Jphi, *Br = f->Br, *Br2 = f->Br2, *Bz = f->Bz, *Bz2 = f->Bz2;
     /* -- Ephi, :449-471 */
     factor = 1. / (eps0 * mu0);
+#pragma omp parallel for private(iz) schedule(static)
     for (ir = 1; ir < nr; ir++)
       for (iz = 0; iz < nz; iz++) {
         if (!(FL(iz, ir) & OFL_EPHI_ZERO))
@@ -220,6 +224,7 @@ void oracle_advance_field_solutions(ofields *f) {
         A(f, Ephi, iz, nr - 1) = 0;
 
     /* -- Br, :473-509 */
+#pragma omp parallel for private(iz, dB) schedule(static)
     for (ir = 0; ir < nr; ir++)
       for (iz = 1; iz < nz; iz++) {
         if (!(FL(iz, ir) & OFL_BR_ZERO)) {
@@ -252,6 +257,7 @@ void oracle_advance_field_solutions(ofields *f) {
       }
 
     /* -- Bz, :511-539 */
+#pragma omp parallel for private(iz, dB, factor) schedule(static)
     for (ir = 1; ir < nr; ir++) {
       factor = -2 * dti / (dr * (isqr(ir + 1) - isqr(ir)));
       for (iz = 0; iz < nz; iz++) {
@@ -863,7 +869,9 @@ int oracle_advance_particles(ofields *f, obeam *b, const oscreen *screens, int32
   *n_rows = 0;
   *n_lost = 0;
 
-  /* main loop, :131-339 */
+  /* main loop, :131-339; particles are independent, and in a threaded build
+   * pz0/pr0/r_pphi0 end with the last particle's values as in the serial loop */
+#pragma omp parallel for reduction(+ : newActiveCount) lastprivate(pz0, pr0, r_pphi0) schedule(static)
   for (ip = 0; ip < np; ip++) {
     double z, r, vz, vr, pz, pr, r_pphi, vphi, gamma, pphi, adjustFactor = 1;
     double Ez, Er, Ephi, Bz, Br, Bphi;
@@ -1116,6 +1124,10 @@ int64_t oracle_solve_poisson_cyl(double *Psi, const double *driving_term, const 
   e = -2 * (1 / sqr(dz) + 1 / sqr(dr));
   for (n = 0; n < iteration_limit; n++) {
     abs_total_residual = 0;
+    /* a threaded build splits the rows of a half-sweep (each point reads only
+     * the other colour) and adds the residual per thread: Psi is the same,
+     * the residual total can round differently */
+#pragma omp parallel for private(iz, c, d, residual) reduction(+ : abs_total_residual) schedule(static)
     for (ir = 0; ir < nr - 1; ir++) {
       if (ir != 0) {
         c = 1 / sqr(dr) * (1 + 1. / (2 * ir));

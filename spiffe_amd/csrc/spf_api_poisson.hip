@@ -331,11 +331,23 @@ int spf_poisson_correct(spf_ctx *c, int32_t run, int32_t space_charge, int64_t *
   int64_t it = -1;
   int rc = sor_solve(c, R, &R->Psi, R->Q, p.accuracy, p.maximum_iterations, &it);
   if (rc) return rc;
+  R->pc_last_iters = it;
+  R->pc_solves++;
+  R->pc_half_sweeps += it >= 0 ? it + 1 : p.maximum_iterations;
   HIPCALL(c, launch_field_from_psi(*R, 0, 1.0, s));
   // recompute the apparent charge for output, :332-439
   HIPCALL(c, launch_gauss_q(*R, s));
   if (space_charge) HIPCALL(c, launch_q_subtract(*R, s));
   if (iterations) *iterations = it;
+  return SPF_OK;
+}
+
+int spf_poisson_stats(spf_ctx *c, int32_t run, int64_t *last_iterations, int64_t *solves, int64_t *half_sweeps) {
+  Run *R = get_run(c, run);
+  if (!R) return fail(c, SPF_EINVAL, "bad run id");
+  if (last_iterations) *last_iterations = R->pc_last_iters;
+  if (solves) *solves = R->pc_solves;
+  if (half_sweeps) *half_sweeps = R->pc_half_sweeps;
   return SPF_OK;
 }
 

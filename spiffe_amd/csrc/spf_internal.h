@@ -162,6 +162,8 @@ struct Run {
   uint32_t *sor_masks = nullptr; // per-tile lane masks of the SOR batches
   double *sor_sums = nullptr;   // [0] drive sum, [1] last residual sum, [2..] misc
   double last_field_factor = -1;
+  int64_t pc_last_iters = -2;   // result of the last correction (-2: none solved)
+  int64_t pc_solves = 0, pc_half_sweeps = 0;   // corrections solved and half-sweeps they took
   std::vector<double> screen_z;
   std::vector<int> screen_dir;
   double ramp_time = 0, flat_top_time = 0;

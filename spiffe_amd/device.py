@@ -321,6 +321,13 @@ class DeviceRun:
         self.ctx.check(self.lib.spf_poisson_correct(self.ctx.h, self.id, 1 if space_charge else 0, ctypes.byref(it)))
         return it.value
 
+    def poisson_stats(self):
+        """(result of the last correction that solved, corrections solved, half-sweeps in all)."""
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self.ctx.check(self.lib.spf_poisson_stats(self.ctx.h, self.id, ctypes.byref(a), ctypes.byref(b),
+                                                  ctypes.byref(c)))
+        return a.value, b.value, c.value
+
     def solve_poisson_cyl(self, which, accuracy, maximum_iterations):
         it = ctypes.c_int64()
         self.ctx.check(self.lib.spf_poisson_solve(self.ctx.h, self.id, int(which), float(accuracy),

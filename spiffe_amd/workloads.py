@@ -1,5 +1,10 @@
 """Synthetic workloads of the benchmark configurations (seeded).
 
+gun_hires(): the high-resolution single run -- the same 1.6-cell gun at
+dr = dz = 25 um (4201 x 1681 grid lines, 7.06M cells), 16M macroparticles
+carrying 100 pC in a Gaussian bunch (sigma_t 3 ps) of radius 0.6 mm, TM+TE,
+the Gaussian solenoid, Poisson correction every 5 steps to 1e-8.
+
 gun_16cell(): the production-size single run -- a 1.6-cell S-band gun
 (half cell 0-3.5 cm, full cell 3.5-10.5 cm, cell radius 4.2 cm, iris radius
 1.25 cm, iris 0.5 cm thick), dr = dz = 0.1 mm (1051 x 421 grid lines),
@@ -85,7 +90,37 @@ def gun_bunch(grid, n, charge=1e-9, radius=1e-3, seed=7, z0=0.3e-3, length=1.5e-
             "generation": np.zeros(n, dtype=np.int16), "status": np.full(n, abi.PART_ACTIVE, dtype=np.uint32)}
 
 
+def gun_bunch_gaussian(grid, n, charge=100e-12, radius=0.6e-3, sigma_t=3e-12, seed=11, z_center=2.0e-3, pz=0.6):
+    """A bunch with a Gaussian longitudinal profile of rms duration sigma_t
+    (sigma_z = beta c sigma_t at momentum pz), uniform over a disc of the given
+    radius, just off the cathode; the tails are cut at 3 sigma."""
+    rng = np.random.default_rng(seed)
+    beta = pz / np.sqrt(1 + pz * pz)
+    sigma_z = beta * K.c_mks * sigma_t
+    u = rng.standard_normal(n)
+    u = np.where(np.abs(u) > 3, rng.uniform(-3, 3, n), u)
+    z = grid.zmin + z_center + sigma_z * u
+    r = radius * np.sqrt(rng.random(n))
+    pzv = pz * (1 + 0.02 * rng.standard_normal(n))
+    prv = 0.01 * rng.standard_normal(n)
+    g = np.sqrt(pzv * pzv + prv * prv + 1)
+    q = -charge / n
+    return {"Q": np.full(n, q), "z": z, "r": r, "pz": pzv, "pr": prv, "r_pphi": np.zeros(n),
+            "vz": K.c_mks * pzv / g, "vr": K.c_mks * prv / g, "vphi": np.zeros(n),
+            "r0": r.copy(), "z0": z.copy(), "t0": np.zeros(n), "rmax": r.copy(), "rmin": r.copy(),
+            "zmax": z.copy(), "zmin": z.copy(), "particleID": np.arange(n, dtype=np.int32),
+            "generation": np.zeros(n, dtype=np.int16), "status": np.full(n, abi.PART_ACTIVE, dtype=np.uint32)}
+
+
 GUN_SCREENS = [(0.10, 1)]
 # external solenoid: Gaussian on-axis Bz, 0.25 T peak at z = 20 cm, sigma 5 cm
 GUN_SOLENOID = dict(gaussian=1, Bz_peak=0.25, gaussian_center=0.20, gaussian_sigma=0.05)
 GUN_POISSON = dict(step_interval=10, accuracy=1e-8, maximum_iterations=3000)
+# the high-resolution run: 25 um mesh, 16M particles, 100 pC, correction every
+# 5 steps to 1e-8 (about 19 000 half-sweeps per correction once Psi is warm;
+# the cap is there for the first, cold solve)
+HIRES_DZ = 25e-6
+HIRES_PARTICLES = 16_000_000
+HIRES_CHARGE = 100e-12
+HIRES_RADIUS = 0.6e-3
+HIRES_POISSON = dict(step_interval=5, accuracy=1e-8, maximum_iterations=40000)
