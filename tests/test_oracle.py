@@ -494,3 +494,229 @@ def test_antenna_waveform_linear_interpolation():
         env = np.interp(time, t, W)
         assert err.value == 0
         assert J == pytest.approx(env * I * np.sin(2 * np.pi * f * time + 0.3), rel=1e-12, abs=1e-15)
+
+
+# ---------------------------------------------------------------- hand-drawn lattices: loss codes, gather
+# index fix-ups, loss coordinates, charge removal, field correction
+
+
+def _lattice_sim(nz, nr, marks, modes=abi.TM_FIELDS | abi.RADIAL_INTERP | abi.LONGIT_INTERP, d=1e-3, cap=0):
+    g = Grid(nz=nz, nr=nr, zmin=0.0, zmax=d * (nz - 1), rmax=d * (nr - 1), dt=courant_dt(d, d, 0.9), modes=modes)
+    return g, OracleSim(g, c_derive_flags(marks), max_particles=cap)
+
+
+def test_loss_codes_hand_lattice():
+    """checkLossCode (src/advance_particles.c:31-65) on a hand-drawn 6 x 5 lattice:
+    3 inside metal; for a surface point 4 when iz+1 > nz, else 5 when ir+1 > nr,
+    else 6 when the three points up and to the right are all metal; 0 otherwise."""
+    import ctypes
+    S, I = abi.FL_IS_SURFACE, abi.FL_IS_INSIDE
+    nz, nr = 5, 4
+    m = np.zeros((nz + 1, nr + 1), dtype=np.int16)
+    m[1, 1] = I
+    m[2, 1], m[3, 2], m[2, 2], m[3, 1] = S, S, I, S
+    m[5, 0], m[5, 4], m[0, 4] = S, S, S
+    m[4, 0], m[5, 1], m[4, 1] = S, I, S
+    g, o = _lattice_sim(nz, nr, m)
+    expected = {(1, 1): 3, (2, 1): 6, (2, 2): 3, (3, 2): 0, (3, 1): 0, (5, 0): 4, (5, 4): 4, (0, 4): 5,
+                (4, 0): 6, (5, 1): 3, (4, 1): 0}
+    for iz in range(nz + 1):
+        for ir in range(nr + 1):
+            code = o.lib.oracle_check_loss_code(iz, ir, ctypes.byref(o.f))
+            assert code == expected.get((iz, ir), 0), (iz, ir, code)
+
+
+def _labels(g, base):
+    iz, ir = np.meshgrid(np.arange(g.nz + 1), np.arange(g.nr + 1), indexing="ij")
+    return base + 100.0 * iz + ir + 1.0
+
+
+def _frac(x):
+    i = int(x)
+    return i, x - i
+
+
+@pytest.mark.parametrize("vacuum_flag", [0, abi.FL_IS_VACUUM])
+def test_gather_ez_next_to_a_constant_z_wall(vacuum_flag):
+    """The Ez gather's index shifts beside a wall along z = const
+    (src/advance_particles.c:683-699): when both rows of the cell at iz have Er
+    zeroed, iz moves to iz1 if the column before holds any flag (vacuum drawn as
+    FL_IS_VACUUM counts), else iz1 falls back to iz; when only iz1's column has
+    Er zeroed, iz1 falls back to iz.  Ez holds a distinct label per point, so the
+    gathered value names the points used."""
+    S, I = abi.FL_IS_SURFACE, abi.FL_IS_INSIDE
+    nz, nr = 10, 8
+    m = np.full((nz + 1, nr + 1), vacuum_flag, dtype=np.int16)
+    m[6, :] = S
+    m[7:, :] = I
+    g, o = _lattice_sim(nz, nr, m)
+    E = _labels(g, 0.0)
+    o.set_field("Ez", E)
+    d = g.dz
+
+    def bilinear(iz, iz1, ir, ir1, fl, fr, fa, fb):
+        return fb * (fl * E[iz, ir] + fr * E[iz1, ir]) + fa * (fl * E[iz, ir1] + fr * E[iz1, ir1])
+
+    ir, fa = _frac(2.3e-3 / d)
+    # cell column 6 has Er zeroed in both rows: iz = 6
+    iz, fr = _frac(5.6e-3 / d + 0.5)
+    assert iz == 6
+    got = o.interpolate(5.6e-3, 2.3e-3)[0]
+    if vacuum_flag:   # column 5 carries a flag: iz moves to iz1 = 7
+        want = bilinear(7, 7, ir, ir + 1, 1 - fr, fr, fa, 1 - fa)
+    else:             # column 5 is bare: iz1 falls back to 6
+        want = bilinear(6, 6, ir, ir + 1, 1 - fr, fr, fa, 1 - fa)
+    assert got == pytest.approx(want, rel=1e-12)
+    # iz = 5 with column 6 zeroed: iz1 falls back to 5
+    iz, fr = _frac(4.8e-3 / d + 0.5)
+    assert iz == 5
+    assert o.interpolate(4.8e-3, 2.3e-3)[0] == pytest.approx(bilinear(5, 5, ir, ir + 1, 1 - fr, fr, fa, 1 - fa),
+                                                              rel=1e-12)
+    # far from the wall: plain bilinear over columns 2 and 3
+    iz, fr = _frac(2.2e-3 / d + 0.5)
+    assert o.interpolate(2.2e-3, 2.3e-3)[0] == pytest.approx(bilinear(2, 3, ir, ir + 1, 1 - fr, fr, fa, 1 - fa),
+                                                              rel=1e-12)
+
+
+def test_gather_er_next_to_a_constant_r_wall():
+    """The Er gather's index shifts beside a wall along r = const
+    (src/advance_particles.c:744-747): with Ez zeroed on both columns of row ir
+    the gather moves ir to ir+1; with Ez zeroed on both columns of row ir+1 it
+    uses row ir twice; a column at iz = 0 never has Ez zeroed."""
+    S, I = abi.FL_IS_SURFACE, abi.FL_IS_INSIDE
+    nz, nr = 10, 8
+    m = np.zeros((nz + 1, nr + 1), dtype=np.int16)
+    m[:, 5] = S
+    m[:, 6:] = I
+    g, o = _lattice_sim(nz, nr, m)
+    E = _labels(g, 10000.0)
+    o.set_field("Er", E)
+    d = g.dr
+
+    def bilinear(iz, ir, ir1, fl, fr, fa, fb):
+        return fl * (fb * E[iz, ir] + fa * E[iz, ir1]) + fr * (fb * E[iz + 1, ir] + fa * E[iz + 1, ir1])
+
+    iz, fr = _frac(3.4e-3 / d)
+    ir, fa = _frac(4.3e-3 / d + 0.5)        # row 4, row 5 above has Ez zeroed: row 4 twice
+    assert (iz, ir) == (3, 4)
+    assert o.interpolate(3.4e-3, 4.3e-3)[1] == pytest.approx(bilinear(3, 4, 4, 1 - fr, fr, fa, 1 - fa), rel=1e-12)
+    ir, fa = _frac(4.6e-3 / d + 0.5)        # row 5 has Ez zeroed: rows 6 and 6
+    assert ir == 5
+    assert o.interpolate(3.4e-3, 4.6e-3)[1] == pytest.approx(bilinear(3, 6, 6, 1 - fr, fr, fa, 1 - fa), rel=1e-12)
+    iz, fr = _frac(0.4e-3 / d)              # iz = 0: point (0, 5) has no Ez zero, no shift
+    assert o.interpolate(0.4e-3, 4.6e-3)[1] == pytest.approx(bilinear(0, 5, 6, 1 - fr, fr, fa, 1 - fa), rel=1e-12)
+
+
+def _loss_coords(o, z0, r0, z1, r1):
+    import ctypes
+    zr, rr, code = ctypes.c_double(), ctypes.c_double(), ctypes.c_short()
+    o.lib.oracle_find_loss_coordinates(ctypes.byref(zr), ctypes.byref(rr), ctypes.byref(code), z0, r0, z1, r1,
+                                       ctypes.byref(o.f))
+    return zr.value, rr.value, code.value
+
+
+def test_loss_coordinates_by_direction():
+    """findLossCoordinates (src/secondaryEmission.c:239-372): the direction code
+    (1 left, 2 right, 3 up, 4 down) from the cells left and entered, and the
+    point where the straight step meets the cell face it names (a 1e-12 cell
+    inside it).  Expected points are intersections worked out by hand."""
+    S, I = abi.FL_IS_SURFACE, abi.FL_IS_INSIDE
+    nz, nr = 10, 8
+    m = np.zeros((nz + 1, nr + 1), dtype=np.int16)
+    m[4, 3] = S
+    g, o = _lattice_sim(nz, nr, m)
+    mm = 1e-3
+    # no motion: code 1 at the start
+    assert _loss_coords(o, 3.3 * mm, 2.2 * mm, 3.3 * mm, 2.2 * mm) == (3.3 * mm, 2.2 * mm, 1)
+    # same row, moving right: face z = 4 mm, r on the line there = 2.5 + 0.2/0.7*0.5
+    z, r, c = _loss_coords(o, 3.5 * mm, 2.5 * mm, 4.2 * mm, 2.7 * mm)
+    assert c == 2 and z == pytest.approx(4 * mm, abs=1e-14) and r == pytest.approx((2.5 + 0.5 * 0.2 / 0.7) * mm, rel=1e-10)
+    # same column, moving down: face r = 2 mm, z = 3.3 + 0.3 * 0.2/0.5
+    z, r, c = _loss_coords(o, 3.3 * mm, 2.2 * mm, 3.6 * mm, 1.7 * mm)
+    assert c == 4 and r == pytest.approx(2 * mm, abs=1e-14) and z == pytest.approx(3.42 * mm, rel=1e-10)
+    # same column, moving up: face r = 3 mm, z = 3.3 + 0.3 * 0.8/0.9
+    z, r, c = _loss_coords(o, 3.3 * mm, 2.2 * mm, 3.6 * mm, 3.1 * mm)
+    assert c == 3 and r == pytest.approx(3 * mm, abs=1e-14) and z == pytest.approx((3.3 + 0.3 * 0.8 / 0.9) * mm, rel=1e-10)
+    # diagonal into the metal point (4, 3): right, face z = 4 mm, r = 3.0 mm
+    z, r, c = _loss_coords(o, 3.8 * mm, 2.8 * mm, 4.3 * mm, 3.3 * mm)
+    assert c == 2 and z == pytest.approx(4 * mm, abs=1e-14) and r == pytest.approx(3.0 * mm, rel=1e-10)
+    # diagonal into vacuum: no branch names a face, the code stays 0 and the
+    # up-face formula applies: r = 3 mm, z on the line = 5.0 mm
+    z, r, c = _loss_coords(o, 4.8 * mm, 2.8 * mm, 5.3 * mm, 3.3 * mm)
+    assert c == 0 and r == pytest.approx(3 * mm, abs=1e-14) and z == pytest.approx(5.0 * mm, rel=1e-10)
+    # leaving through z < zmin: left, face z = 0, r = 2.5 + 0.7 * 0.5/0.8
+    z, r, c = _loss_coords(o, 0.5 * mm, 2.5 * mm, -0.3 * mm, 3.2 * mm)
+    assert c == 1 and z == pytest.approx(0.0, abs=1e-14) and r == pytest.approx((2.5 + 0.7 * 0.5 / 0.8) * mm, rel=1e-10)
+    # crossing the axis within a column: down, r returned as |r| at the axis face
+    z, r, c = _loss_coords(o, 3.3 * mm, 0.4 * mm, 3.6 * mm, -0.3 * mm)
+    assert c == 4 and r == pytest.approx(0.0, abs=1e-14) and z == pytest.approx((3.3 + 0.3 * 0.4 / 0.7) * mm, rel=1e-10)
+
+
+def _particles_at(zs, rs, q):
+    n = len(zs)
+    z, r = np.asarray(zs, float), np.asarray(rs, float)
+    return {"Q": np.full(n, q), "z": z, "r": r, "pz": np.zeros(n), "pr": np.zeros(n), "r_pphi": np.zeros(n),
+            "vz": np.zeros(n), "vr": np.zeros(n), "vphi": np.zeros(n), "r0": r.copy(), "z0": z.copy(),
+            "t0": np.zeros(n), "rmax": r.copy(), "rmin": r.copy(), "zmax": z.copy(), "zmin": z.copy(),
+            "particleID": np.arange(n, dtype=np.int32), "generation": np.zeros(n, dtype=np.int16),
+            "status": np.full(n, abi.PART_ACTIVE, dtype=np.uint32)}
+
+
+def test_charge_removal_lands_on_the_nearest_point():
+    """remove_particle_charge (src/poisson_correction.c:184-243) without
+    smearing: with zero fields the Gauss-law charge is zero, and each particle's
+    charge is taken from the nearest grid point (iz = (z-zmin)/dz + 0.5, ir =
+    r/dr + 0.5), except at metal points and at the Neumann rows (iz = 0 on the
+    left, iz >= nz-1 on the right, ir >= nr-1 on top); Qreal is the charge
+    removed.  A threshold above the error charge stops the correction there."""
+    S = abi.FL_IS_SURFACE
+    nz, nr = 10, 8
+    m = np.zeros((nz + 1, nr + 1), dtype=np.int16)
+    m[5, 2] = S
+    g, o = _lattice_sim(nz, nr, m, modes=abi.TM_FIELDS | abi.SPACE_CHARGE, cap=16)
+    mm = 1e-3
+    q = -2e-15
+    cases = [((3.4, 2.2), (3, 2)), ((3.6, 2.6), (4, 3)), ((7.49, 0.1), (7, 0)), ((4.9, 2.1), None),  # metal (5,2)
+             ((0.3, 3.0), None), ((8.6, 3.0), None), ((4.0, 6.6), None)]                            # Neumann rows
+    o.append(_particles_at([c[0][0] * mm for c in cases], [c[0][1] * mm for c in cases], q))
+    o.setup_poisson_correction(step_interval=1, accuracy=1e-6, maximum_iterations=10, error_charge_threshold=1.0)
+    assert o.add_poisson_correction(True) == -2
+    Q = o.field("Q")
+    want = np.zeros(g.shape)
+    for _, cell in cases:
+        if cell:
+            want[cell] -= q
+    assert np.array_equal(Q, want)
+    assert o.pc.Qreal == 3 * q
+    assert o.pc.Qerror == -3 * q
+
+
+def test_field_correction_restores_gauss_law():
+    """The correction subtracts the gradient of the error potential from E
+    (src/poisson_correction.c:289-326), so that the charge Gauss's law finds in
+    E afterwards (recomputed, :332-439) matches the particles' charge: the
+    apparent minus particle charge left in Q shrinks from the initial error to
+    the solver's residual.  A sign slip in the gradient would double the error
+    and an index slip would leave it of the same size."""
+    g = gun_grid(modes=abi.TM_FIELDS | abi.RADIAL_INTERP | abi.LONGIT_INTERP | abi.SPACE_CHARGE)
+    f = gun_flags(g)
+    o = OracleSim(g, f, max_particles=4000)
+    rng = np.random.default_rng(5)
+    for name in ("Ez", "Er"):
+        o.set_field(name, rng.standard_normal(g.shape) * 1e3)
+    o.append(random_particles(g, 4000, 6, flags=f, q=-1e-14))
+    probe = OracleSim(g, f, max_particles=4000)
+    for name in ("Ez", "Er"):
+        probe.set_field(name, o.field(name))
+    probe.append(o.particles())
+    probe.setup_poisson_correction(step_interval=1, error_charge_threshold=1e30)
+    probe.add_poisson_correction(True)
+    before = probe.field("Q")
+    o.setup_poisson_correction(step_interval=1, accuracy=1e-11, maximum_iterations=20000)
+    assert o.add_poisson_correction(True) > 0
+    after = o.field("Q")
+    interior = (f[:g.nz, :g.nr - 1] & abi.FL_IS_METAL) == 0
+    interior[0, :] = interior[g.nz - 1, :] = False
+    b, a = np.abs(before[:g.nz, :g.nr - 1][interior]), np.abs(after[:g.nz, :g.nr - 1][interior])
+    assert b.max() > 0
+    assert a.max() < 1e-6 * b.max()
