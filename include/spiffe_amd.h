@@ -207,6 +207,24 @@ int spf_poisson_stats(spf_ctx *ctx, int32_t run, int64_t *last_iterations, int64
  * rhs = Q) or PsiImposed (which=1, no rhs).  Synchronizing. */
 int spf_poisson_solve(spf_ctx *ctx, int32_t run, int32_t which, double accuracy, int64_t max_iterations,
                       int64_t *iterations);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The solve of
+ * spf_poisson_solve / solve_poisson_cyl (src/poisson.c:17-130) on caller-owned
+ * device buffers in the library's layout -- element (iz, ir) at ir*ldz + iz,
+ * ldz = nz+1 rounded up to a multiple of 32 -- Psi in/out, rhs (may be NULL)
+ * and flags (the run's flag bits) of (nr+1)*ldz elements.  Its scratch memory
+ * is one fixed array per GPU inside the library, so it calls no allocator; it
+ * holds grids with ldz*(nr+1) <= SPF_NOALLOC_MAX_POINTS and at most
+ * SPF_NOALLOC_MAX_TILES SOR tiles, and iteration_limit <=
+ * SPF_NOALLOC_MAX_ITERATIONS (larger requests return SPF_EINVAL).  Calls that
+ * overlap on one GPU share the array and race.  Synchronizing; *iterations as
+ * in spf_poisson_solve. */
+#define SPF_NOALLOC_MAX_POINTS (1 << 20)
+#define SPF_NOALLOC_MAX_TILES 1024
+#define SPF_NOALLOC_MAX_ITERATIONS (1 << 17)
+int spf_poisson_solve_noalloc(spf_ctx *ctx, double *Psi, const double *rhs, const int16_t *flags, int32_t nz,
+                              int32_t nr, double dz, double dr, int32_t left_bc, int32_t right_bc, double accuracy,
+                              int64_t iteration_limit, int64_t *iterations);
 /* exactly `count` red-black half-sweeps (half-sweep indices n0 .. n0+count-1 of the
  * omega schedule of src/poisson.c:121-124) with no stopping test: the loop body of
  * solve_poisson_cyl as a fixed-count smoother.  Stream-ordered, not synchronizing. */

@@ -176,6 +176,8 @@ SIGNATURES = {
     "spf_poisson_correct": (ctypes.c_int, [_vp, _i32, _i32, _pi64]),
     "spf_poisson_solve": (ctypes.c_int, [_vp, _i32, _i32, _d, _i64, _pi64]),
     "spf_poisson_stats": (ctypes.c_int, [_vp, _i32, _pi64, _pi64, _pi64]),
+    "spf_poisson_solve_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _d, _d, _i32, _i32, _d, _i64,
+                                                 _pi64]),
     "spf_poisson_sweeps": (ctypes.c_int, [_vp, _i32, _i32, _i64, _i64]),
     "spf_imposed_efield": (ctypes.c_int, [_vp, _i32, _i32, _d, _d]),
     "spf_timing_enable": (ctypes.c_int, [_vp, _i32]),

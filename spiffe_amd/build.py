@@ -12,7 +12,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libspiffe_amd.so")
 SOURCES = ["spf_api.hip", "spf_api_poisson.hip", "spf_fdtd.hip", "spf_push.hip", "spf_deposit.hip",
-           "spf_poisson.hip", "spf_sort.hip", "spf_util.hip", "spf_cathode.hip", "spf_solenoid.hip"]
+           "spf_poisson.hip", "spf_sort.hip", "spf_util.hip", "spf_cathode.hip", "spf_solenoid.hip",
+           "spf_noalloc.hip"]
 # host-only C++ (no HIP), built with the system C++ compiler
 HOST_SOURCES = ["spf_emit.cpp", "spf_namelist.cpp", "spf_geometry.cpp", "spf_sdds.cpp", "spf_deck.cpp"]
 # the command-line deck runner, linked against the library

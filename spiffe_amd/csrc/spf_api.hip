@@ -154,7 +154,8 @@ int spf_ctx_create(int32_t device, spf_ctx **out) {
       hipHostMalloc((void **)&c->pinned, 64 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
       hipEventCreateWithFlags(&c->stage_done, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->sor_ev[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->sor_ev[1], hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->sor_ev[1], hipEventDisableTiming) != hipSuccess ||
+      noalloc_setup(device) != SPF_OK) {
     delete c;
     return SPF_EHIP;
   }

@@ -251,6 +251,8 @@ hipError_t launch_sor_exact_sweep(Run &R, double *P, const double *rhs, int64_t 
 hipError_t launch_sor_state_clear(Run &R, hipStream_t s);
 hipError_t launch_field_from_psi(Run &R, int imposed, double field_factor, hipStream_t s);
 hipError_t launch_zero_psi(Run &R, hipStream_t s);
+// resolves, once per device, the address of the no-allocation solve's scratch array (spf_noalloc.hip)
+int noalloc_setup(int device);
 
 }  // namespace spf
 
