@@ -412,6 +412,18 @@ int32_t spf_sor_batch_half_sweeps(void);
 int spf_step(spf_ctx *ctx, int32_t run, int32_t n_lines, const int32_t *line_dir, const int64_t *line_pos,
              const int64_t *line_start, const int64_t *line_end, const double *line_J);
 
+/* ---- many runs at once ----
+ * No counterpart in the reference (one run per process): one whole step
+ * (spf_step without line currents) of each of n_runs distinct runs of the
+ * context.  The runs' phases are launched together: deposition per run, the
+ * Poisson solves of the runs correcting at this step advanced in lockstep by
+ * ragged SOR launches over all of them (runs grouped by accuracy and
+ * limit), the FDTD of every run in one ragged launch, the push, event rows
+ * and compaction of every run in one ragged launch each.  Each run's
+ * arithmetic is that of spf_step, so its results are the same bits as when
+ * it is stepped alone. */
+int spf_step_batch(spf_ctx *ctx, int32_t n_runs, const int32_t *runs);
+
 #ifdef __cplusplus
 }
 #endif

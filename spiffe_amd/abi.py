@@ -184,6 +184,7 @@ SIGNATURES = {
     "spf_sor_batch_half_sweeps": (ctypes.c_int32, []),
     "spf_timing_read": (ctypes.c_int, [_vp, _dp, _pi64, _pi64]),
     "spf_step": (ctypes.c_int, [_vp, _i32, _i32, _pi32, _pi64, _pi64, _pi64, _dp]),
+    "spf_step_batch": (ctypes.c_int, [_vp, _i32, _pi32]),
     "spf_geometry_parse": (ctypes.c_int, [ctypes.c_char_p, _d, ctypes.POINTER(BoundaryPoint), _i64, _pi64,
                                           ctypes.c_char_p, _i64]),
     "spf_geometry_draw": (ctypes.c_int, [_i32, _i32, _d, _d, _d, ctypes.POINTER(BoundaryPoint), _i64,

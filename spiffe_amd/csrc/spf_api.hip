@@ -1,6 +1,7 @@
 // spf_api.hip -- the extern "C" entry points of libspiffe_amd: context and
 // run lifecycle, host<->device layout conversion, and the per-step entry
 // points that replace the reference's hot-path functions.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -113,6 +114,47 @@ void spf::resolve_timers(spf_ctx *c, bool wait) {
   c->ev_pending.erase(c->ev_pending.begin(), c->ev_pending.begin() + k);
 }
 
+int spf::batch_table(spf_ctx *c, size_t bytes, void **host, void **dev) {
+  if (bytes > c->bt_slot) {
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    if (c->bt_host) HIPCALL(c, hipHostFree(c->bt_host));
+    if (c->bt_dev) HIPCALL(c, hipFree(c->bt_dev));
+    c->bt_host = c->bt_dev = nullptr;
+    c->bt_slot = 0;
+    const size_t slot = (std::max<size_t>(bytes, 65536) + 4095) / 4096 * 4096;
+    HIPCALL(c, hipHostMalloc((void **)&c->bt_host, slot * spf_ctx::kTableSlots, hipHostMallocDefault));
+    HIPCALL(c, hipMalloc((void **)&c->bt_dev, slot * spf_ctx::kTableSlots));
+    c->bt_slot = slot;
+    for (int i = 0; i < spf_ctx::kTableSlots; i++) c->bt_used[i] = false;
+  }
+  const int i = c->bt_next;
+  c->bt_next = (i + 1) % spf_ctx::kTableSlots;
+  if (!c->bt_ev[i]) HIPCALL(c, hipEventCreateWithFlags(&c->bt_ev[i], hipEventDisableTiming));
+  if (c->bt_used[i]) HIPCALL(c, hipEventSynchronize(c->bt_ev[i]));   // its last copy has been made
+  c->bt_cur = i;
+  *host = c->bt_host + (size_t)i * c->bt_slot;
+  *dev = c->bt_dev + (size_t)i * c->bt_slot;
+  return SPF_OK;
+}
+
+hipError_t spf::batch_table_upload(spf_ctx *c, const void *host, void *dev, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(c->bt_ev[c->bt_cur], c->stream);
+  if (e == hipSuccess) c->bt_used[c->bt_cur] = true;
+  return e;
+}
+
+int spf::batch_flags(spf_ctx *c, size_t n) {
+  if (n <= c->bw_flags_n) return SPF_OK;
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  if (c->bw_flags) HIPCALL(c, hipHostFree(c->bw_flags));
+  c->bw_flags = nullptr;
+  c->bw_flags_n = 0;
+  HIPCALL(c, hipHostMalloc((void **)&c->bw_flags, std::max<size_t>(n, 256) * sizeof(int64_t), hipHostMallocDefault));
+  c->bw_flags_n = std::max<size_t>(n, 256);
+  return SPF_OK;
+}
+
 extern "C" {
 
 int spf_timing_enable(spf_ctx *c, int32_t enable) {
@@ -173,6 +215,11 @@ int spf_ctx_destroy(spf_ctx *c) {
       delete R;
     }
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->bt_host) (void)hipHostFree(c->bt_host);
+  if (c->bt_dev) (void)hipFree(c->bt_dev);
+  for (hipEvent_t e : c->bt_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->bw_flags) (void)hipHostFree(c->bw_flags);
   if (c->hstage) (void)hipHostFree(c->hstage);
   if (c->stage_done) (void)hipEventDestroy(c->stage_done);
   for (hipEvent_t e : c->sor_ev)
@@ -474,17 +521,15 @@ int spf_get_step(spf_ctx *c, int32_t run, int64_t *time_step, double *time) {
   return SPF_OK;
 }
 
-int spf_fields_advance(spf_ctx *c, int32_t run) {
-  Run *R = get_run(c, run);
-  if (!R) return fail(c, SPF_EINVAL, "bad run id");
+}  // extern "C"
+
+void spf::fdtd_args(Run *R, FdtdArgs &a) {
   const spf_run_desc &d = R->d;
-  FdtdArgs a;
   a.nz = d.nz;
   a.nr = d.nr;
   a.ldz = R->ldz;
   a.tm = (d.modes & SPF_TM_FIELDS) != 0;
   a.te = (d.modes & SPF_TE_FIELDS) != 0;
-  if (!a.tm && !a.te) return SPF_OK;
   a.left_neu = d.left_bc == SPF_NEUMANN;
   a.right_neu = d.right_bc == SPF_NEUMANN;
   a.upper_neu = d.upper_bc == SPF_NEUMANN;
@@ -519,12 +564,26 @@ int spf_fields_advance(spf_ctx *c, int32_t run) {
   a.cp = R->cp;
   a.cm = R->cm;
   a.bzf = R->bzf;
+}
+
+void spf::fdtd_flip(Run *R) {
+  if (R->d.modes & SPF_TM_FIELDS)
+    for (int k = 0; k < 3; k++) R->cur[k] = 1 - R->cur[k];
+  if (R->d.modes & SPF_TE_FIELDS)
+    for (int k = 3; k < 6; k++) R->cur[k] = 1 - R->cur[k];
+}
+
+extern "C" {
+
+int spf_fields_advance(spf_ctx *c, int32_t run) {
+  Run *R = get_run(c, run);
+  if (!R) return fail(c, SPF_EINVAL, "bad run id");
+  FdtdArgs a;
+  fdtd_args(R, a);
+  if (!a.tm && !a.te) return SPF_OK;
   PhaseTimer pt(c, SPF_T_FDTD);
   HIPCALL(c, launch_fdtd(a, c->stream));
-  if (a.tm)
-    for (int k = 0; k < 3; k++) R->cur[k] = 1 - R->cur[k];
-  if (a.te)
-    for (int k = 3; k < 6; k++) R->cur[k] = 1 - R->cur[k];
+  fdtd_flip(R);
   return SPF_OK;
 }
 
@@ -654,12 +713,11 @@ int spf_particles_download(spf_ctx *c, int32_t run, const spf_particles *h, int6
   return SPF_OK;
 }
 
-int spf_particles_push(spf_ctx *c, int32_t run, double zForcesStart) {
-  Run *R = get_run(c, run);
-  if (!R) return fail(c, SPF_EINVAL, "bad run id");
+}  // extern "C"
+
+void spf::push_args(Run *R, double zForcesStart, PushArgs &a, RowsArgs &w) {
   const spf_run_desc &d = R->d;
   double time = R->time_step * d.dt + d.start_time;
-  PushArgs a;
   a.nz = d.nz;
   a.nr = d.nr;
   a.ldz = R->ldz;
@@ -717,9 +775,6 @@ int spf_particles_push(spf_ctx *c, int32_t run, double zForcesStart) {
     a.sz[s] = R->screen_z[s];
     a.sdir[s] = R->screen_dir[s];
   }
-  RowsArgs w;
-  PhaseTimer pt(c, SPF_T_PUSH);
-  HIPCALL(c, launch_push(a, R->np_ub, c->stream));
   w.nz = d.nz;
   w.nr = d.nr;
   w.ldz = R->ldz;
@@ -767,6 +822,18 @@ int spf_particles_push(spf_ctx *c, int32_t run, double zForcesStart) {
   w.move_dst = R->move_dst;
   w.tflag = R->gflag;
   w.overflow = (int32_t *)(R->counters + 8);
+}
+
+extern "C" {
+
+int spf_particles_push(spf_ctx *c, int32_t run, double zForcesStart) {
+  Run *R = get_run(c, run);
+  if (!R) return fail(c, SPF_EINVAL, "bad run id");
+  PushArgs a;
+  RowsArgs w;
+  push_args(R, zForcesStart, a, w);
+  PhaseTimer pt(c, SPF_T_PUSH);
+  HIPCALL(c, launch_push(a, R->np_ub, c->stream));
   HIPCALL(c, launch_rows(w, c->stream));
   return SPF_OK;
 }

@@ -284,9 +284,10 @@ __global__ __launch_bounds__(256) void fdtd_edge_kernel(FdtdArgs a, EdgeList e) 
   fdtd_edge_point(a, e, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-__global__ __launch_bounds__(64 * QW) void fdtd_kernel(FdtdArgs a, int tiles_z, int nblocks) {
+// one tile of the launch (workgroup `bid` of the run's blocks)
+__device__ inline void fdtd_tile(const FdtdArgs &a, int tiles_z, int nblocks, int bid) {
   __shared__ double sez[QW][128], ser[QW][128], sephi[QW][128];
-  int logical = xcd_band_block(blockIdx.x, nblocks);
+  int logical = xcd_band_block(bid, nblocks);
   if (logical >= nblocks) return;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int zs = (logical % tiles_z) * QZ, r0 = (logical / tiles_z) * QR;
@@ -434,6 +435,46 @@ __global__ __launch_bounds__(64 * QW) void fdtd_kernel(FdtdArgs a, int tiles_z, 
   }
 }
 
+__global__ __launch_bounds__(64 * QW) void fdtd_kernel(FdtdArgs a, int tiles_z, int nblocks) {
+  fdtd_tile(a, tiles_z, nblocks, (int)blockIdx.x);
+}
+
+// Ragged launches over several runs' FDTD steps: workgroups [first[r],
+// first[r+1]) are run r's (tile spans rounded to multiples of 8 so the XCD
+// banding of each run is the single-run one; edge spans plain).
+struct FdtdMultiRun {
+  FdtdArgs a;
+  EdgeList el;
+  int tiles_z, nblocks;
+  int64_t ne;   // edge points
+};
+
+__device__ inline int ragged_run_f(const int32_t *first, int n, int b) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (first[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+__global__ __launch_bounds__(64 * QW) void fdtd_multi_kernel(const FdtdMultiRun *__restrict__ T,
+                                                              const int32_t *__restrict__ first, int nruns) {
+  const int r = ragged_run_f(first, nruns, (int)blockIdx.x);
+  const FdtdMultiRun &x = T[r];
+  fdtd_tile(x.a, x.tiles_z, x.nblocks, (int)blockIdx.x - first[r]);
+}
+
+__global__ __launch_bounds__(256) void fdtd_edge_multi_kernel(const FdtdMultiRun *__restrict__ T,
+                                                               const int32_t *__restrict__ first, int nruns) {
+  const int r = ragged_run_f(first, nruns, (int)blockIdx.x);
+  const FdtdMultiRun &x = T[r];
+  fdtd_edge_point(x.a, x.el, (int64_t)((int)blockIdx.x - first[r]) * blockDim.x + threadIdx.x);
+}
+
 __global__ void transpose_in_kernel(const double *h, double *d, int nz, int nr, int ldz) {
   // h: [iz][ir] with row length nr+1; d: [ir][iz] with row length ldz
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -488,9 +529,7 @@ __global__ void add_line_kernel(double *J, int ldz, int direction, int64_t ipos,
 
 }  // namespace
 
-hipError_t launch_fdtd(const FdtdArgs &a, hipStream_t s) {
-  int tiles_z = (a.nz + 1 + QZ - 1) / QZ, tiles_r = (a.nr + 1 + QR - 1) / QR;
-  int nblocks = tiles_z * tiles_r;
+EdgeList edge_list(const FdtdArgs &a) {
   // every row and column outside the interior boxes, each listed once
   EdgeList el;
   el.nrows = el.ncols = 0;
@@ -505,12 +544,59 @@ hipError_t launch_fdtd(const FdtdArgs &a, hipStream_t s) {
     for (int j = 0; j < el.ncols; j++) dup = dup || el.cols[j] == c;
     if (!dup) el.cols[el.ncols++] = c;
   }
+  return el;
+}
+
+hipError_t launch_fdtd(const FdtdArgs &a, hipStream_t s) {
+  int tiles_z = (a.nz + 1 + QZ - 1) / QZ, tiles_r = (a.nr + 1 + QR - 1) / QR;
+  int nblocks = tiles_z * tiles_r;
+  EdgeList el = edge_list(a);
   const int64_t ne = (int64_t)el.nrows * (a.nz + 1) + (int64_t)el.ncols * (a.nr + 1);
   hipLaunchKernelGGL(fdtd_kernel, dim3((nblocks + 7) / 8 * 8), dim3(64 * QW), 0, s, a, tiles_z, nblocks);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   // the edge B updates read the complete new E
   hipLaunchKernelGGL(fdtd_edge_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, a, el);
+  return hipGetLastError();
+}
+
+size_t fdtd_multi_table_bytes(int nruns) {
+  return sizeof(FdtdMultiRun) * (size_t)nruns + 2 * sizeof(int32_t) * (size_t)(nruns + 1) + 64;
+}
+
+// the tables of one ragged FDTD step: per-run arguments, then the first
+// workgroup of each run in the tile launch and in the edge launch
+void fdtd_multi_table(const FdtdArgs *args, int nruns, void *host, int64_t *tile_blocks, int64_t *edge_blocks) {
+  FdtdMultiRun *T = (FdtdMultiRun *)host;
+  int32_t *ft = (int32_t *)((char *)host + sizeof(FdtdMultiRun) * (size_t)nruns);
+  int32_t *fe = ft + nruns + 1;
+  int64_t nt = 0, ne = 0;
+  for (int i = 0; i < nruns; i++) {
+    FdtdMultiRun &x = T[i];
+    x.a = args[i];
+    x.tiles_z = (args[i].nz + 1 + QZ - 1) / QZ;
+    x.nblocks = x.tiles_z * ((args[i].nr + 1 + QR - 1) / QR);
+    x.el = edge_list(args[i]);
+    x.ne = (int64_t)x.el.nrows * (args[i].nz + 1) + (int64_t)x.el.ncols * (args[i].nr + 1);
+    ft[i] = (int32_t)nt;
+    fe[i] = (int32_t)ne;
+    nt += ((int64_t)x.nblocks + 7) / 8 * 8;
+    ne += (x.ne + 255) / 256;
+  }
+  ft[nruns] = (int32_t)nt;
+  fe[nruns] = (int32_t)ne;
+  *tile_blocks = nt;
+  *edge_blocks = ne;
+}
+
+hipError_t launch_fdtd_multi(const void *dev_table, int nruns, int64_t tile_blocks, int64_t edge_blocks,
+                             hipStream_t s) {
+  const FdtdMultiRun *T = (const FdtdMultiRun *)dev_table;
+  const int32_t *ft = (const int32_t *)((const char *)dev_table + sizeof(FdtdMultiRun) * (size_t)nruns);
+  hipLaunchKernelGGL(fdtd_multi_kernel, dim3((unsigned)tile_blocks), dim3(64 * QW), 0, s, T, ft, nruns);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(fdtd_edge_multi_kernel, dim3((unsigned)edge_blocks), dim3(256), 0, s, T, ft + nruns + 1, nruns);
   return hipGetLastError();
 }
 

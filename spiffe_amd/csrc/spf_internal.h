@@ -178,6 +178,12 @@ struct Run {
 
 // ---- launchers implemented in the .hip files ----
 hipError_t launch_fdtd(const FdtdArgs &a, hipStream_t s);
+// ragged FDTD step over several runs: table into `host` (fdtd_multi_table_bytes),
+// copied to the device, then one tile launch and one edge launch
+size_t fdtd_multi_table_bytes(int nruns);
+void fdtd_multi_table(const FdtdArgs *args, int nruns, void *host, int64_t *tile_blocks, int64_t *edge_blocks);
+hipError_t launch_fdtd_multi(const void *dev_table, int nruns, int64_t tile_blocks, int64_t edge_blocks,
+                             hipStream_t s);
 hipError_t launch_transpose_in(const double *host_layout, double *dev, int nz, int nr, int ldz, hipStream_t s);
 hipError_t launch_transpose_out(const double *dev, double *host_layout, int nz, int nr, int ldz, hipStream_t s);
 hipError_t launch_transpose_in_i16(const int16_t *host_layout, int16_t *dev, int nz, int nr, int ldz, hipStream_t s);
@@ -188,6 +194,11 @@ hipError_t launch_add_line(double *J, int ldz, int direction, int64_t ipos, int6
                            hipStream_t s);
 hipError_t launch_push(const PushArgs &a, int64_t n_ub, hipStream_t s);
 hipError_t launch_rows(const RowsArgs &a, hipStream_t s);
+// ragged push of several runs (table from push_multi_table, push_multi_table_bytes
+// long, copied to the device): one push launch, then one rows workgroup per run
+size_t push_multi_table_bytes(int nruns);
+int64_t push_multi_table(const PushArgs *pa, const int64_t *n_ub, const RowsArgs *ra, int nruns, void *host);
+hipError_t launch_push_multi(const void *dev_table, int nruns, int64_t total_blocks, hipStream_t s);
 size_t append_stage_bytes(int64_t n);
 hipError_t launch_append_kernels(Run &R, const void *stage, int64_t n, int64_t n_active, double *const *Pdev,
                                  hipStream_t s);
@@ -245,6 +256,17 @@ hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const doubl
                             int partial_row, int force, hipStream_t s, int64_t check_row = 0, int64_t check_n0 = 0,
                             int check_k = 0, double accuracy = 0);
 hipError_t launch_sor_check(Run &R, int64_t check_row, int64_t n0, int k, double accuracy, hipStream_t s);
+// Ragged SOR batch launches over several runs solving in lockstep (same
+// half-sweep range, each run its own grid, ring of 6 Psi buffers, partial
+// rows and state).  sor_multi_table writes the per-run table into `host`
+// (sor_multi_table_bytes) for a copy to `dev`, and returns the launch's
+// workgroup count; launch_sor_multi runs batch index b (ring[b%6] -> ring[(b+1)%6]).
+size_t sor_multi_table_bytes(int nruns);
+int64_t sor_multi_table(Run *const *runs, const double *const *rhs, double *const (*rings)[6], int nruns,
+                        double accuracy, void *host, void *dev, const int32_t **first_dev);
+hipError_t launch_sor_multi(const void *dev_table, const int32_t *first_dev, int nruns, int64_t total_blocks,
+                            int64_t n0, int64_t b, int k, int64_t partial_row, int64_t check_row, int64_t check_n0,
+                            int check_k, hipStream_t s);
 // half-sweep n in place with the reference's expressions, |residual| per point to dump[ir*nz+iz]
 hipError_t launch_sor_exact_sweep(Run &R, double *P, const double *rhs, int64_t n, double *dump, hipStream_t s);
 // done flag, iteration count and the host-decision flag back to "running"
@@ -280,6 +302,17 @@ struct spf_ctx {
   std::vector<Pending> ev_pending;
   double t_ms[SPF_T_N] = {};
   int64_t t_count[SPF_T_N] = {};
+  // ragged (multi-run) launches: argument tables staged in pinned slots and
+  // copied to device slots, round-robin; a slot's host side is rewritten only
+  // after its copy has completed (event per slot)
+  static constexpr int kTableSlots = 16;
+  char *bt_host = nullptr, *bt_dev = nullptr;
+  size_t bt_slot = 0;
+  int bt_next = 0, bt_cur = 0;
+  hipEvent_t bt_ev[kTableSlots] = {};
+  bool bt_used[kTableSlots] = {};
+  int64_t *bw_flags = nullptr;   // pinned readback of per-run flags
+  size_t bw_flags_n = 0;
 };
 
 namespace spf {
@@ -304,6 +337,19 @@ struct PhaseTimer {
 };
 
 void resolve_timers(spf_ctx *c, bool wait);
+
+// arguments of one run's FDTD step and push (as spf_fields_advance and
+// spf_particles_push launch them); fdtd_flip swaps the E/B buffers after the step
+void fdtd_args(Run *R, FdtdArgs &a);
+void fdtd_flip(Run *R);
+void push_args(Run *R, double zForcesStart, PushArgs &a, RowsArgs &w);
+
+// a table slot of at least `bytes` for a ragged launch: host (pinned) and
+// device addresses; fill the host side, then batch_table_upload it
+int batch_table(spf_ctx *c, size_t bytes, void **host, void **dev);
+hipError_t batch_table_upload(spf_ctx *c, const void *host, void *dev, size_t bytes);
+// c->bw_flags holds at least n int64
+int batch_flags(spf_ctx *c, size_t n);
 
 inline Run *get_run(spf_ctx *c, int32_t run) {
   if (!c || run < 0 || run >= (int32_t)c->runs.size() || !c->runs[run] || !c->runs[run]->live) return nullptr;

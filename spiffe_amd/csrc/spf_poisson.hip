@@ -14,6 +14,7 @@
 // serial sum, so the iteration count can differ from the reference only when
 // the residual lands within rounding of the threshold.
 #include <cmath>
+#include <cstring>
 
 #include "spf_internal.h"
 
@@ -578,40 +579,88 @@ __device__ inline void sor_group_check(const double *partials, int ntiles, int k
   if (threadIdx.x == 0) sor_stop_rule(tot, k, n0, accuracy, rel, state, sums);
 }
 
-__global__ __launch_bounds__(BatchCfg::NT) void sor_batch_kernel(SorArgs a) {
+// One workgroup of a batch launch: tile `bid` of the batch, or with bid ==
+// ntiles the stopping-rule check of the previous check group.  The launch's
+// own input and output buffers, half-sweep range, partial rows and check
+// range come as arguments, so a ragged launch over several runs can give each
+// run its own.
+__device__ inline void sor_batch_block(const SorArgs &a, int bid, const double *Pin, double *Pout, int64_t n0, int k,
+                                       double *partials, const double *check_partials, int64_t check_n0, int check_k) {
   using G = BatchCfg;
-  if ((int)blockIdx.x == a.m.ntiles) {
+  if (bid == a.m.ntiles) {
     // the extra workgroup: stopping rule of the previous check group, on a
     // CU the tiles leave free (tiles that start after it stops the solve
     // return at once; their output is never used)
     __shared__ double tot[CB / 32];
-    sor_group_check(a.check_partials, a.m.ntiles, a.check_k, a.check_n0, a.accuracy, a.rel, a.state, a.sums, tot,
-                    G::NT);
+    sor_group_check(check_partials, a.m.ntiles, check_k, check_n0, a.accuracy, a.rel, a.state, a.sums, tot, G::NT);
     return;
   }
   __shared__ double2 xrow[2][G::SW][2][64];   // [parity][wave][first,last row][lane]
   __shared__ double wsum[G::SK][G::SW];
   const int64_t done = a.force ? 0 : a.state[0];
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int tile = xcd_band_tile((int)blockIdx.x, a.m.ntiles);
+  const int tile = xcd_band_tile(bid, a.m.ntiles);
   const SorTile<G> t(a.m, tile);
   // the batch's relaxation factors, fetched with the tile
   __shared__ double om[G::SK];
-  if (threadIdx.x < G::SK) om[threadIdx.x] = a.m.omega[a.n0 + min((int)threadIdx.x, a.k - 1)];
+  if (threadIdx.x < G::SK) om[threadIdx.x] = a.m.omega[n0 + min((int)threadIdx.x, k - 1)];
   SorLane<G> ln;
-  sor_load_P<G>(a.m, t, w, lane, a.Pin, ln.P);
-  sor_setup<G>(a.m, t, tile, w, lane, ln, a.Pin);
+  sor_load_P<G>(a.m, t, w, lane, Pin, ln.P);
+  sor_setup<G>(a.m, t, tile, w, lane, ln, Pin);
   if (done) return;
   __shared__ double lres[G::SK][G::NT];
-  sor_sweeps<G>(a.m, t, ln, w, lane, a.n0, a.k, om, xrow, lres, !a.m.use_y);
+  sor_sweeps<G>(a.m, t, ln, w, lane, n0, k, om, xrow, lres, !a.m.use_y);
   if (a.m.use_y && __syncthreads_or(!sor_lane_exact<G>(ln))) {
     // rare: some x left div_fast's set; the batch input is intact, so redo
     // the tile's batch with plain division
-    sor_load_P<G>(a.m, t, w, lane, a.Pin, ln.P);
-    sor_sweeps<G>(a.m, t, ln, w, lane, a.n0, a.k, om, xrow, lres, true);
+    sor_load_P<G>(a.m, t, w, lane, Pin, ln.P);
+    sor_sweeps<G>(a.m, t, ln, w, lane, n0, k, om, xrow, lres, true);
   }
-  sor_block_sums<G>(lres, a.k, w, lane, wsum, a.partials + tile, a.m.ntiles);
-  sor_write_owned<G>(a.m, t, ln, w, lane, a.Pout);
+  sor_block_sums<G>(lres, k, w, lane, wsum, partials + tile, a.m.ntiles);
+  sor_write_owned<G>(a.m, t, ln, w, lane, Pout);
+}
+
+__global__ __launch_bounds__(BatchCfg::NT) void sor_batch_kernel(SorArgs a) {
+  sor_batch_block(a, (int)blockIdx.x, a.Pin, a.Pout, a.n0, a.k, a.partials, a.check_partials, a.check_n0, a.check_k);
+}
+
+// A ragged batch launch over several runs' solves in lockstep: workgroups
+// [first[r], first[r+1]) belong to run r (each span a multiple of 8, so the
+// XCD banding of the tiles is the single-run one); run r's batch reads
+// ring[b % 6] and writes ring[(b+1) % 6] of its own buffer ring.  The runs
+// share the half-sweep range and the check range; everything else (grid,
+// coefficients, right-hand side, partial sums, state) is the run's own.
+struct SorMultiRun {
+  SorArgs a;               // the run's solve; Pin/Pout/n0/k/partials/check fields unused
+  const double *ring[6];
+};
+
+struct SorMultiLaunch {
+  int64_t n0, b, partial_row, check_row, check_n0;
+  int k, check_k, nruns;
+};
+
+__device__ inline int ragged_run(const int32_t *first, int n, int b) {
+  int lo = 0, hi = n;   // first[lo] <= b < first[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (first[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+__global__ __launch_bounds__(BatchCfg::NT) void sor_multi_kernel(const SorMultiRun *__restrict__ T,
+                                                                  const int32_t *__restrict__ first, SorMultiLaunch L) {
+  const int r = ragged_run(first, L.nruns, (int)blockIdx.x);
+  const SorMultiRun &x = T[r];
+  const int bid = (int)blockIdx.x - first[r];
+  if (bid > x.a.m.ntiles || (bid == x.a.m.ntiles && L.check_k == 0)) return;   // span padding
+  const int ntiles = x.a.m.ntiles;
+  sor_batch_block(x.a, bid, x.ring[L.b % 6], const_cast<double *>(x.ring[(L.b + 1) % 6]), L.n0, L.k,
+                  x.a.partials + L.partial_row * ntiles, x.a.partials + L.check_row * ntiles, L.check_n0, L.check_k);
 }
 
 // the lanes' update and ownership masks of every tile, once per solve
@@ -907,6 +956,53 @@ hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const doubl
   a.check_k = check_k;
   const unsigned grid = (unsigned)a.m.ntiles + (check_k > 0 ? 1u : 0u);
   hipLaunchKernelGGL(sor_batch_kernel, dim3(grid), dim3(BatchCfg::NT), 0, s, a);
+  return hipGetLastError();
+}
+
+// ragged launches over several runs: the per-run table (fixed for a solve),
+// then one launch per batch index b
+size_t sor_multi_table_bytes(int nruns) {
+  return sizeof(SorMultiRun) * (size_t)nruns + sizeof(int32_t) * (size_t)(nruns + 1) + 64;
+}
+
+int64_t sor_multi_table(Run *const *runs, const double *const *rhs, double *const (*rings)[6], int nruns,
+                        double accuracy, void *host, void *dev, const int32_t **first_dev) {
+  SorMultiRun *T = (SorMultiRun *)host;
+  int32_t *first = (int32_t *)((char *)host + sizeof(SorMultiRun) * (size_t)nruns);
+  int64_t total = 0;
+  for (int i = 0; i < nruns; i++) {
+    Run &R = *runs[i];
+    SorArgs &a = T[i].a;
+    memset(&a, 0, sizeof(a));
+    a.m = sor_common(R, rhs[i], BatchCfg::OZ, BatchCfg::OR);
+    a.partials = R.red;
+    a.state = R.sor_state;
+    a.sums = R.sor_sums;
+    a.accuracy = accuracy;
+    a.rel = sor_order_margin(R);
+    for (int j = 0; j < 6; j++) T[i].ring[j] = rings[i][j];
+    first[i] = (int32_t)total;
+    total += ((int64_t)a.m.ntiles + 1 + 7) / 8 * 8;
+  }
+  first[nruns] = (int32_t)total;
+  *first_dev = (const int32_t *)((char *)dev + sizeof(SorMultiRun) * (size_t)nruns);
+  return total;
+}
+
+hipError_t launch_sor_multi(const void *dev_table, const int32_t *first_dev, int nruns, int64_t total_blocks,
+                            int64_t n0, int64_t b, int k, int64_t partial_row, int64_t check_row, int64_t check_n0,
+                            int check_k, hipStream_t s) {
+  SorMultiLaunch L;
+  L.n0 = n0;
+  L.b = b;
+  L.partial_row = partial_row;
+  L.check_row = check_row;
+  L.check_n0 = check_n0;
+  L.k = k;
+  L.check_k = check_k;
+  L.nruns = nruns;
+  hipLaunchKernelGGL(sor_multi_kernel, dim3((unsigned)total_blocks), dim3(BatchCfg::NT), 0, s,
+                     (const SorMultiRun *)dev_table, first_dev, L);
   return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@
 // screen and lost rows come out in the same order as the reference's serial
 // loops regardless of scheduling.
 #include <climits>
+#include <cstring>
 
 #include "spf_internal.h"
 
@@ -370,17 +371,43 @@ __device__ inline bool push_one(const PushArgs &a, int64_t ip, int64_t np) {
 
 // one thread per particle; each workgroup also counts its events for the
 // ordered event list that rows_kernel builds
-__global__ __launch_bounds__(256) void push_kernel(PushArgs a, int64_t n_ub) {
-  const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void push_block(const PushArgs &a, int64_t n_ub, int bid) {
+  const int64_t ip = (int64_t)bid * blockDim.x + threadIdx.x;
   const int64_t np = *a.np;
   bool any = false;
   if (ip < np && ip < n_ub) any = push_one(a, ip, np);
   if (ip < n_ub) a.evflag[ip] = any ? 1 : 0;
   const int cnt = __syncthreads_count(any);
   if (threadIdx.x == 0) {
-    a.pcount[blockIdx.x] = cnt;
+    a.pcount[bid] = cnt;
     if (cnt) atomicAdd(a.ev_total, (unsigned long long)cnt);
   }
+}
+
+__global__ __launch_bounds__(256) void push_kernel(PushArgs a, int64_t n_ub) { push_block(a, n_ub, (int)blockIdx.x); }
+
+__device__ inline int ragged_run_p(const int32_t *first, int n, int b) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (first[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+// Ragged push over several runs: workgroups [first[r], first[r+1]) push run r's particles.
+struct PushMultiRun {
+  PushArgs a;
+  int64_t n_ub;
+};
+
+__global__ __launch_bounds__(256) void push_multi_kernel(const PushMultiRun *__restrict__ T,
+                                                          const int32_t *__restrict__ first, int nruns) {
+  const int r = ragged_run_p(first, nruns, (int)blockIdx.x);
+  push_block(T[r].a, T[r].n_ub, (int)blockIdx.x - first[r]);
 }
 
 // src/secondaryEmission.c:239-372
@@ -481,7 +508,7 @@ __device__ int block_flag_scan(bool f, int *wsum, int &excl) {
 
 // Builds the step's screen rows (grouped per screen, index order within a
 // screen) and lost rows (index order), then compacts the particle arrays.
-__global__ __launch_bounds__(ROWS_THREADS) void rows_kernel(RowsArgs a) {
+__device__ inline void rows_body(const RowsArgs &a) {
   __shared__ int wsum[ROWS_THREADS / 64];
   __shared__ long long sh_first_lost;
   __shared__ int qblk[ROWS_THREADS];
@@ -731,6 +758,13 @@ __global__ __launch_bounds__(ROWS_THREADS) void rows_kernel(RowsArgs a) {
   }
 }
 
+__global__ __launch_bounds__(ROWS_THREADS) void rows_kernel(RowsArgs a) { rows_body(a); }
+
+// one workgroup per run: each builds its run's rows and compacts its particles
+__global__ __launch_bounds__(ROWS_THREADS) void rows_multi_kernel(const RowsArgs *__restrict__ T) {
+  rows_body(T[blockIdx.x]);
+}
+
 __global__ void append_kernel(const uint8_t *stage, int64_t n, int64_t cap, double *const *P, int32_t *pid,
                               int16_t *gen, uint32_t *status, const int64_t *np) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -763,6 +797,44 @@ __global__ void append_commit_kernel(int64_t *counters, int64_t n, int64_t n_act
 hipError_t launch_push(const PushArgs &a, int64_t n_ub, hipStream_t s) {
   if (n_ub <= 0) return hipSuccess;
   hipLaunchKernelGGL(push_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, s, a, n_ub);
+  return hipGetLastError();
+}
+
+size_t push_multi_table_bytes(int nruns) {
+  return sizeof(PushMultiRun) * (size_t)nruns + sizeof(int32_t) * (size_t)(nruns + 1) + sizeof(RowsArgs) * (size_t)nruns +
+         64;
+}
+
+// the tables of one ragged push: per-run push arguments, first workgroups,
+// then (16-byte aligned) the per-run rows arguments
+int64_t push_multi_table(const PushArgs *pa, const int64_t *n_ub, const RowsArgs *ra, int nruns, void *host) {
+  PushMultiRun *T = (PushMultiRun *)host;
+  int32_t *first = (int32_t *)((char *)host + sizeof(PushMultiRun) * (size_t)nruns);
+  int64_t total = 0;
+  for (int i = 0; i < nruns; i++) {
+    T[i].a = pa[i];
+    T[i].n_ub = n_ub[i];
+    first[i] = (int32_t)total;
+    total += (n_ub[i] + 255) / 256;
+  }
+  first[nruns] = (int32_t)total;
+  char *rows = (char *)host + (sizeof(PushMultiRun) * (size_t)nruns + sizeof(int32_t) * (size_t)(nruns + 1) + 15) / 16 * 16;
+  memcpy(rows, ra, sizeof(RowsArgs) * (size_t)nruns);
+  return total;
+}
+
+hipError_t launch_push_multi(const void *dev_table, int nruns, int64_t total_blocks, hipStream_t s) {
+  const PushMultiRun *T = (const PushMultiRun *)dev_table;
+  const int32_t *first = (const int32_t *)((const char *)dev_table + sizeof(PushMultiRun) * (size_t)nruns);
+  const RowsArgs *R = (const RowsArgs *)((const char *)dev_table +
+                                         (sizeof(PushMultiRun) * (size_t)nruns + sizeof(int32_t) * (size_t)(nruns + 1) +
+                                          15) / 16 * 16);
+  if (total_blocks > 0) {
+    hipLaunchKernelGGL(push_multi_kernel, dim3((unsigned)total_blocks), dim3(256), 0, s, T, first, nruns);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(rows_multi_kernel, dim3((unsigned)nruns), dim3(ROWS_THREADS), 0, s, R);
   return hipGetLastError();
 }
 

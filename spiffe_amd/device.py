@@ -94,6 +94,12 @@ class Context:
         self.check(self.lib.spf_timing_read(self.h, ms, cnt, tot))
         return {name: (ms[i], cnt[i], tot[i]) for i, name in enumerate(abi.TIMERS)}
 
+    def step_batch(self, runs):
+        """One whole step of each run (DeviceRun objects of this context), their
+        phases launched together over all of them."""
+        ids = np.ascontiguousarray([r.id for r in runs], dtype=np.int32)
+        self.check(self.lib.spf_step_batch(self.h, len(ids), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+
     def close(self):
         if self.h:
             self.lib.spf_ctx_destroy(self.h)
