@@ -597,16 +597,23 @@ __device__ inline void sor_batch_block(const SorArgs &a, int bid, const double *
   }
   __shared__ double2 xrow[2][G::SW][2][64];   // [parity][wave][first,last row][lane]
   __shared__ double wsum[G::SK][G::SW];
-  const int64_t done = a.force ? 0 : a.state[0];
+  // the done flag, read once for the workgroup (the check workgroup of the
+  // same launch may set it at any moment, and the wavefronts must agree), and
+  // the batch's relaxation factors, published through LDS with the tile loads
+  __shared__ int done;
+  __shared__ double om[G::SK];
+  if (threadIdx.x == 0) done = a.force ? 0 : (int)a.state[0];
+  if (threadIdx.x < G::SK) om[threadIdx.x] = a.m.omega[n0 + min((int)threadIdx.x, k - 1)];
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int tile = xcd_band_tile(bid, a.m.ntiles);
   const SorTile<G> t(a.m, tile);
-  // the batch's relaxation factors, fetched with the tile
-  __shared__ double om[G::SK];
-  if (threadIdx.x < G::SK) om[threadIdx.x] = a.m.omega[n0 + min((int)threadIdx.x, k - 1)];
   SorLane<G> ln;
   sor_load_P<G>(a.m, t, w, lane, Pin, ln.P);
   sor_setup<G>(a.m, t, tile, w, lane, ln, Pin);
+  // every wavefront reads om[0] before the first step's own barrier: this
+  // barrier makes wavefront 0's stores visible to all of them (without it a
+  // wavefront that runs ahead reads what an earlier workgroup left there)
+  __syncthreads();
   if (done) return;
   __shared__ double lres[G::SK][G::NT];
   sor_sweeps<G>(a.m, t, ln, w, lane, n0, k, om, xrow, lres, !a.m.use_y);
