@@ -125,12 +125,116 @@ def cpu_phase_sample(g, f, bunch, pc, threaded, sor_iterations):
     return n / step, step, t
 
 
+def scan_bench(n_runs, steps):
+    """The design scan: n_runs gun runs of the seeded scan distribution, each
+    stepped `steps` times (corrections at steps 0, 10, ...), one at a time and
+    batched; runs/hour of each from the stepping time with state resident, and
+    end to end (set-up included).  Four runs are checked bit for bit between
+    the two."""
+    from spiffe_amd import scan as S
+    from spiffe_amd.device import Context
+    cfgs = S.scan_configs(n_runs)
+    out = {"metric": "scan runs/hr (batched vs one-at-a-time, one GPU)", "unit": "runs/hr", "n_gpus": 1,
+           "runs": n_runs, "steps_per_run": steps, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "64-run 1.6-cell gun scan: dz=dr U(0.05,0.1) mm, particles logU(1e5,4e6), "
+                                  "phase U(0,60) deg, gradient U(80,140) MV/m, solenoid U(0.15,0.35) T, "
+                                  "charge logU(10 pC,2 nC), cathode radius U(0.5,2) mm; Poisson every 10 steps "
+                                  "to 1e-8 capped at 3000 half-sweeps", "seed": S.SCAN_SEED}}
+    with Context(0) as ctx:
+        grids = {}
+        out["total_particles"] = int(sum(c["particles"] for c in cfgs))
+        seq_step, seq_all = S.one_at_a_time(ctx, cfgs, steps, grids)
+        # the autotuner picks the sub-batch size on the scan's own resident runs
+        # (steps 1-8 of every run, between corrections), then the scan runs batched
+        runs = [S.build_run(ctx, c, grids) for c in cfgs]
+        ctx.step_batch(runs)
+        best, timings = S.tune_batch(ctx, runs, steps=2)
+        for r in runs:
+            r.close()
+        bat_step, bat_all, kept = S.batched(ctx, cfgs, steps, best, grids, keep=True)
+        # bit-for-bit check of four runs: batched state against a run stepped alone
+        checked = []
+        for i in (0, n_runs // 3, 2 * n_runs // 3, n_runs - 1):
+            alone = S.build_run(ctx, cfgs[i], grids)
+            for _ in range(steps):
+                alone.step()
+            same = all(np.array_equal(alone.download(k).view(np.uint64), kept[i].download(k).view(np.uint64))
+                       for k in ("Ez", "Er", "Bphi", "Psi", "Jz"))
+            pa, pb = alone.particles(), kept[i].particles()
+            same = same and all(np.array_equal(pa[k].view(np.uint64), pb[k].view(np.uint64))
+                                for k in ("z", "r", "pz", "pr"))
+            same = same and alone.poisson_stats() == kept[i].poisson_stats()
+            checked.append({"run": i, "bitwise_equal": bool(same)})
+            alone.close()
+        for r in kept:
+            r.close()
+    out.update({"value": n_runs / bat_step * 3600, "one_at_a_time_runs_per_hr": n_runs / seq_step * 3600,
+                "speedup": seq_step / bat_step, "batched_s": bat_step, "one_at_a_time_s": seq_step,
+                "end_to_end": {"batched_runs_per_hr": n_runs / bat_all * 3600,
+                               "one_at_a_time_runs_per_hr": n_runs / seq_all * 3600,
+                               "batched_s": bat_all, "one_at_a_time_s": seq_all},
+                "autotune": {"sub_batch": best, "probe_s": timings}, "bitwise_checks": checked})
+    return out
+
+
+def device_free_bytes():
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+        return 288e9
+    return float(free.value)
+
+
+def serve_bench(n_requests, steps):
+    """The request stream, time-compressed: each request is `steps` whole steps
+    of its run (a full run to the screen is about 50x longer), so arrivals come
+    50x faster than the configured 20 runs/min to keep the load the GPU sees.
+    Both schedulers serve the same stream; every request's result digest is
+    compared between them."""
+    from spiffe_amd import serve as V
+    from spiffe_amd.device import Context
+    compression = 2000.0 / steps
+    rate = 20.0 / 60.0 * compression
+    reqs = V.request_stream(n_requests, rate, steps=steps)
+    with Context(0) as ctx:
+        budget = 0.9 * device_free_bytes()
+        done_c, st_c = V.serve_continuous(ctx, reqs, budget)
+        print("continuous batching served %d requests" % len(reqs), file=sys.stderr, flush=True)
+        done_1, st_1 = V.serve_one_at_a_time(ctx, reqs)
+        print("one-at-a-time queue served %d requests" % len(reqs), file=sys.stderr, flush=True)
+        differ = [r["id"] for r in reqs if done_c[r["id"]]["digest"] != done_1[r["id"]]["digest"]]
+        serial = V.serve_serial(ctx, reqs, differ)
+    same = len(reqs) - len(differ)
+    mismatches = [{"id": i, "kind": reqs[i]["kind"], "cfg": reqs[i]["cfg"],
+                   "continuous_is_serial": done_c[i]["digest"] == serial[i],
+                   "one_at_a_time_is_serial": done_1[i]["digest"] == serial[i]} for i in differ]
+    return {"metric": "serving runs/hr + p50/p99 time-to-result (continuous batching vs one-at-a-time queue)",
+            "value": st_c["runs_per_hr"], "unit": "runs/hr", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "%d requests, Poisson arrivals at 20 runs/min time-compressed %gx (%.2f/s); "
+                                   "80%% 1.6-cell gun runs from the scan distribution, 20%% DC-gun runs (100 kV, "
+                                   "2 cm gap, 3 mm cathode, 2 mm bore, 0.05 mm mesh, 200k macroparticles 50 pC "
+                                   "10 ps flat-top emitted); %d steps per request"
+                                   % (n_requests, compression, rate, steps),
+                       "memory_budget_bytes": budget, "dc_requests": sum(r["kind"] == "dc" for r in reqs)},
+            "continuous": st_c, "one_at_a_time": st_1,
+            "p50_s": st_c["p50_s"], "p99_s": st_c["p99_s"],
+            "results_identical": "%d/%d" % (same, len(reqs)), "mismatches": mismatches}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["hires", "gun"], default="hires")
+    ap.add_argument("--workload", choices=["hires", "gun", "scan", "serve"], default="hires",
+                    help="hires: the high-resolution single run (the headline); gun: the production-size run; "
+                         "scan: the 64-run design scan, batched against one at a time; serve: the request "
+                         "stream, continuous batching against a one-at-a-time queue")
+    ap.add_argument("--serve-requests", type=int, default=512)
+    ap.add_argument("--serve-steps", type=int, default=40)
+    ap.add_argument("--scan-runs", type=int, default=64)
+    ap.add_argument("--scan-steps", type=int, default=40)
     ap.add_argument("--particles", type=int, default=0, help="0: the workload's own count")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -150,6 +254,13 @@ def main():
               "and this build measures the single-GPU path only (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "scan":
+        print(json.dumps(scan_bench(args.scan_runs, args.scan_steps)))
+        return
+    if args.workload == "serve":
+        print(json.dumps(serve_bench(args.serve_requests, args.serve_steps)))
+        return
 
     from spiffe_amd import workloads as W
     from spiffe_amd.device import Context

@@ -152,6 +152,14 @@ void *spf_stream(spf_ctx *ctx);
 /* ---- run lifecycle; replaces the allocations of src/geometry.c:78-107 ---- */
 int spf_run_create(spf_ctx *ctx, const spf_run_desc *desc, int32_t *run_out);
 int spf_run_destroy(spf_ctx *ctx, int32_t run);
+/* No counterpart in the reference: hands a run (all its device state) from
+ * one context to another on the same device, after the run's queued work in
+ * `from` has finished; *run_out is its id in `to`.  A serving loop builds runs
+ * in a context of its own (uploads, imposed-field solves) while its main
+ * context steps the resident ones, then moves them in; the run tables of
+ * both contexts are locked for the move, so `to` may be in use by another
+ * thread. */
+int spf_run_move(spf_ctx *from, int32_t run, spf_ctx *to, int32_t *run_out);
 /* bytes of HBM a run with this description would hold (admission control) */
 int64_t spf_run_footprint(const spf_run_desc *desc);
 

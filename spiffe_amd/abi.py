@@ -155,6 +155,7 @@ SIGNATURES = {
     "spf_stream": (_vp, [_vp]),
     "spf_run_create": (ctypes.c_int, [_vp, ctypes.POINTER(RunDesc), _pi32]),
     "spf_run_destroy": (ctypes.c_int, [_vp, _i32]),
+    "spf_run_move": (ctypes.c_int, [_vp, _i32, _vp, _pi32]),
     "spf_run_footprint": (_i64, [ctypes.POINTER(RunDesc)]),
     "spf_field_upload": (ctypes.c_int, [_vp, _i32, _i32, _dp]),
     "spf_field_download": (ctypes.c_int, [_vp, _i32, _i32, _dp]),

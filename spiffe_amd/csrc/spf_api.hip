@@ -421,19 +421,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
     if (e == hipSuccess) e = hipStreamSynchronize(s);   // the host tables go out of scope below
     if (e != hipSuccess) return bail(e, "table upload");
   }
-  // reuse a free slot
-  int32_t id = -1;
-  for (size_t k = 0; k < c->runs.size(); k++)
-    if (!c->runs[k]) {
-      id = (int32_t)k;
-      break;
-    }
-  if (id < 0) {
-    id = (int32_t)c->runs.size();
-    c->runs.push_back(nullptr);
-  }
-  c->runs[id] = R;
-  *run_out = id;
+  *run_out = put_run(c, R);
   return SPF_OK;
 }
 
@@ -441,9 +429,19 @@ int spf_run_destroy(spf_ctx *c, int32_t run) {
   Run *R = get_run(c, run);
   if (!R) return fail(c, SPF_EINVAL, "bad run id");
   HIPCALL(c, hipStreamSynchronize(c->stream));
+  clear_run(c, run);
   free_run(R);
   delete R;
-  c->runs[run] = nullptr;
+  return SPF_OK;
+}
+
+int spf_run_move(spf_ctx *from, int32_t run, spf_ctx *to, int32_t *run_out) {
+  Run *R = get_run(from, run);
+  if (!R || !to || !run_out) return fail(from, SPF_EINVAL, "bad run id or null argument");
+  if (to->device != from->device) return fail(from, SPF_EINVAL, "a run moves only between contexts of one device");
+  HIPCALL(from, hipStreamSynchronize(from->stream));   // the run's queued work is done
+  clear_run(from, run);
+  *run_out = put_run(to, R);
   return SPF_OK;
 }
 

@@ -124,7 +124,6 @@ int sor_run_batches(spf_ctx *c, SorPass *const *ps, int np) {
   if (np > 1) {
     std::vector<Run *> runs(np);
     std::vector<const double *> rhs(np);
-    std::vector<const double *> rings_flat;
     double *rings[64][6];
     if (np > 64) return fail(c, SPF_EINVAL, "at most 64 runs solve in one ragged batch");
     for (int i = 0; i < np; i++) {

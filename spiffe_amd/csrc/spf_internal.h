@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -283,6 +284,7 @@ struct spf_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<spf::Run *> runs;
+  std::mutex runs_mu;   // guards `runs`: spf_run_move may insert from another thread
   std::string err;
   int64_t *pinned = nullptr;   // small readbacks
   void *hstage = nullptr;      // pinned staging for appends
@@ -352,8 +354,27 @@ hipError_t batch_table_upload(spf_ctx *c, const void *host, void *dev, size_t by
 int batch_flags(spf_ctx *c, size_t n);
 
 inline Run *get_run(spf_ctx *c, int32_t run) {
-  if (!c || run < 0 || run >= (int32_t)c->runs.size() || !c->runs[run] || !c->runs[run]->live) return nullptr;
+  if (!c) return nullptr;
+  std::lock_guard<std::mutex> lock(c->runs_mu);
+  if (run < 0 || run >= (int32_t)c->runs.size() || !c->runs[run] || !c->runs[run]->live) return nullptr;
   return c->runs[run];
+}
+
+// stores R in a free slot of c's run table and returns its id
+inline int32_t put_run(spf_ctx *c, Run *R) {
+  std::lock_guard<std::mutex> lock(c->runs_mu);
+  for (size_t k = 0; k < c->runs.size(); k++)
+    if (!c->runs[k]) {
+      c->runs[k] = R;
+      return (int32_t)k;
+    }
+  c->runs.push_back(R);
+  return (int32_t)c->runs.size() - 1;
+}
+
+inline void clear_run(spf_ctx *c, int32_t run) {
+  std::lock_guard<std::mutex> lock(c->runs_mu);
+  c->runs[run] = nullptr;
 }
 
 }  // namespace spf

@@ -65,6 +65,7 @@ class Context:
 
     def __init__(self, device=0):
         self.lib = abi.load()
+        self.device = device
         h = ctypes.c_void_p()
         rc = self.lib.spf_ctx_create(device, ctypes.byref(h))
         if rc != 0:
@@ -164,6 +165,22 @@ class DeviceRun:
 
     def footprint(self):
         return self.lib.spf_run_footprint(ctypes.byref(self.desc))
+
+    def move_to(self, ctx):
+        """Hands the run to another context of the same device (spf_run_move)."""
+        rid = ctypes.c_int32()
+        self.ctx.check(self.lib.spf_run_move(self.ctx.h, self.id, ctx.h, ctypes.byref(rid)))
+        self.ctx, self.id = ctx, rid.value
+
+    @staticmethod
+    def footprint_for(grid, max_particles, n_screens):
+        """HBM bytes a run of this grid, capacity and screen count would hold
+        (spf_run_footprint), before creating it."""
+        d = abi.RunDesc()
+        d.nz, d.nr = grid.nz, grid.nr
+        d.max_particles = int(max_particles)
+        d.n_screens = int(n_screens)
+        return abi.load().spf_run_footprint(ctypes.byref(d))
 
     def upload(self, name, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64)

@@ -1,0 +1,338 @@
+"""A request loop serving a stream of independent gun runs on one GPU.
+
+Requests arrive as a Poisson process.  80 % are 1.6-cell RF-gun runs drawn
+from the design-scan distribution (spiffe_amd.scan), 20 % DC-gun runs: 100 kV
+across a 2 cm gap between a flat 3 mm cathode and an anode with a 2 mm bore,
+0.05 mm mesh, a 10 ps flat-top pulse of 50 pC in 200 k macroparticles emitted
+from the cathode, the imposed electrostatic field solved on the device.  Each
+request is a fixed number of whole steps of its run.
+
+Host work per request (geometry, fields, bunch, uploads) runs on builder
+threads, each with a context of its own, which hand finished runs to the
+serving context (spf_run_move); a finisher thread downloads results.  The GPU
+side is one of two schedulers:
+
+  serve_continuous: continuous batching.  Runs stay resident in HBM from
+    admission to completion; a request is admitted as soon as its inputs are
+    ready and its footprint (spf_run_footprint) fits in what the resident and
+    queued runs leave of the memory budget (a builder reserves the footprint
+    before it allocates); every loop iteration advances all resident
+    runs by one step together (spf_step_batch: ragged launches over the runs),
+    and a run leaves (its memory freed) when its steps are done, so new
+    requests join while others are mid-flight.
+  serve_one_at_a_time: a plain FIFO queue, one run on the GPU at a time
+    (with the same builders and finisher, so only the GPU scheduling differs).
+
+Both report per-request time to result (completion - arrival) and a digest
+of each result (the bits of its final fields, particles and screen rows),
+which is the same under either scheduler.
+"""
+import hashlib
+import math
+import time
+
+import numpy as np
+
+from . import constants as K
+from . import scan as S
+from . import workloads as W
+from .device import DeviceRun
+
+DC_POISSON = dict(step_interval=10, accuracy=1e-6, maximum_iterations=2000)
+DC_IMPOSED_ITERATIONS = 2000
+
+
+def request_stream(n, rate_per_s, seed=7, steps=40, dc_fraction=0.2):
+    """n requests with exponential inter-arrival times of mean 1/rate_per_s (seconds from 0)."""
+    rng = np.random.default_rng(seed)
+    gun_cfgs = S.scan_configs(n, seed=seed + 1)
+    t = 0.0
+    out = []
+    for k in range(n):
+        t += rng.exponential(1.0 / rate_per_s)
+        kind = "dc" if rng.random() < dc_fraction else "gun"
+        cfg = gun_cfgs[k] if kind == "gun" else {"index": k, "seed": int(rng.integers(1, 2 ** 31 - 1))}
+        out.append({"id": k, "kind": kind, "cfg": cfg, "arrival": t, "steps": steps})
+    return out
+
+
+def prepare(req, grids):
+    """Host-side inputs of a request (no GPU)."""
+    cfg = req["cfg"]
+    if req["kind"] == "gun":
+        key = ("gun", cfg["dz"])
+        if key not in grids:
+            g = W.gun_grid(dz=cfg["dz"])
+            grids[key] = (g, W.gun_flags(g), None)
+        g, f, _ = grids[key]
+        ez = W.gun_fields(g, E0=cfg["gradient"])["Ez"] * math.cos(math.radians(cfg["phase_deg"]))
+        bunch = W.gun_bunch_gaussian(g, cfg["particles"], charge=cfg["charge"], radius=cfg["radius"],
+                                     seed=cfg["seed"])
+        return {"grid": g, "flags": f, "fields": {"Ez": ez}, "bunch": bunch, "cap": cfg["particles"],
+                "solenoid": dict(W.GUN_SOLENOID, Bz_peak=cfg["solenoid"]), "poisson": S.SCAN_POISSON,
+                "screens": W.GUN_SCREENS}
+    key = ("dc",)
+    if key not in grids:
+        g = W.dc_gun_grid()
+        f, psi = W.dc_gun_geometry(g)
+        grids[key] = (g, f, psi)
+    g, f, psi = grids[key]
+    q, n, tf = 50e-12, 200_000, 10e-12
+    cathode = dict(z_position=0.0, outer_radius=3e-3, current_density=q / tf / (math.pi * 3e-3 ** 2),
+                   electrons_per_macroparticle=q / K.e_mks / n, initial_pz=1e-3, zoned_emission=0,
+                   random_number_seed=cfg["seed"])
+    prof = (np.array([0.0, 0.5e-12, 10.5e-12, 11e-12]), np.array([0.0, 1.0, 1.0, 0.0]))
+    return {"grid": g, "flags": f, "fields": {"PsiImposed": psi}, "bunch": None, "cap": int(n * 1.2),
+            "cathode": (cathode, prof), "poisson": DC_POISSON, "screens": [(0.010, 1), (0.025, 1)]}
+
+
+def build(ctx, prep):
+    """The resident run of a prepared request (uploads; the DC gun's imposed field solve)."""
+    d = DeviceRun(ctx, prep["grid"], prep["flags"], max_particles=prep["cap"], screens=prep["screens"],
+                  compat=False)
+    for k, v in prep["fields"].items():
+        d.upload(k, v)
+    if "solenoid" in prep:
+        d.define_solenoid(**prep["solenoid"])
+    if prep.get("bunch") is not None:
+        d.append(prep["bunch"])
+    if "cathode" in prep:
+        d.setup_poisson_correction(**dict(prep["poisson"], maximum_iterations=DC_IMPOSED_ITERATIONS))
+        d.compute_imposed_Efield(True)
+        c, prof = prep["cathode"]
+        d.define_cathode(profile=prof, **c)
+    d.setup_poisson_correction(**prep["poisson"])
+    return d
+
+
+def digest(d, rows):
+    """sha1 of the result: final Ez, Jz, Psi and particle bits and every screen row."""
+    h = hashlib.sha1()
+    for name in ("Ez", "Jz", "Psi"):
+        h.update(d.download(name).tobytes())
+    p = d.particles()
+    for name in ("z", "r", "pz", "pr", "particleID"):
+        h.update(np.ascontiguousarray(p[name]).tobytes())
+    for r in rows:
+        h.update(r.tobytes())
+    return h.hexdigest()
+
+
+class _Pipeline:
+    """Host work around the GPU loop, on threads with contexts of their own:
+    builders take arrived requests in arrival order, reserve their footprint
+    in the memory budget (waiting while it is full), prepare the host inputs,
+    build the run in their context and move it into the serving context; a
+    finisher takes completed runs, downloads their results (digest), destroys
+    them and returns their memory.  The serving loop itself only steps runs."""
+
+    def __init__(self, reqs, ctx, memory_budget, builders=3, want_digest=True, max_built=64):
+        import threading
+        from .device import Context
+        self.reqs, self.ctx, self.budget, self.want_digest = reqs, ctx, memory_budget, want_digest
+        self.t0 = time.perf_counter()
+        self.lock = threading.Condition()
+        self.next_req = 0
+        self.reserved = 0.0
+        self.built = {}          # request id -> (run, footprint)
+        self.max_built = max_built
+        self.results = {}
+        self.fin_queue = []
+        self.stop = False
+        self.errors = []
+        self.grids = {}
+        self.threads = [threading.Thread(target=self._builder, args=(Context(ctx_device(ctx)),), daemon=True)
+                        for _ in range(builders)]
+        self.fin_ctx = Context(ctx_device(ctx))
+        self.threads.append(threading.Thread(target=self._finisher, daemon=True))
+        for t in self.threads:
+            t.start()
+
+    def now(self):
+        return time.perf_counter() - self.t0
+
+    def _builder(self, bctx):
+        try:
+            while True:
+                with self.lock:
+                    while not self.stop and (self.next_req >= len(self.reqs) or len(self.built) >= self.max_built):
+                        self.lock.wait(0.01)
+                    if self.stop:
+                        return
+                    r = self.reqs[self.next_req]
+                    self.next_req += 1
+                wait = r["arrival"] - self.now()
+                if wait > 0:
+                    time.sleep(wait)
+                p = prepare(r, self.grids)
+                fp = DeviceRun.footprint_for(p["grid"], p["cap"], len(p["screens"]))
+                with self.lock:
+                    while not self.stop and self.reserved > 0 and self.reserved + fp > self.budget:
+                        self.lock.wait(0.01)
+                    if self.stop:
+                        return
+                    self.reserved += fp
+                d = build(bctx, p)
+                d.move_to(self.ctx)
+                with self.lock:
+                    self.built[r["id"]] = (d, fp)
+                    self.lock.notify_all()
+        except Exception as e:   # surfaced by the serving loop
+            self.errors.append(e)
+        finally:
+            bctx.close()
+
+    def take(self, rid):
+        """The built run of request rid, or None if it is not ready yet."""
+        with self.lock:
+            if self.errors:
+                raise self.errors[0]
+            return self.built.pop(rid, None)
+
+    def take_ready(self, limit):
+        """Up to `limit` built runs, oldest request first, as (request, run, footprint)."""
+        with self.lock:
+            if self.errors:
+                raise self.errors[0]
+            ids = sorted(self.built)[:max(limit, 0)]
+            return [(self.reqs[i], *self.built.pop(i)) for i in ids]
+
+    def finish(self, r, d, fp):
+        """Hands a completed run (in the serving context) to the finisher."""
+        d.move_to(self.fin_ctx)
+        with self.lock:
+            self.fin_queue.append((r, d, fp, self.now()))
+            self.lock.notify_all()
+
+    def _finisher(self):
+        try:
+            while True:
+                with self.lock:
+                    while not self.fin_queue and not self.stop:
+                        self.lock.wait(0.01)
+                    if not self.fin_queue and self.stop:
+                        return
+                    r, d, fp, t_dev = self.fin_queue.pop(0)
+                dg = digest(d, [d.drain()[0]]) if self.want_digest else None
+                d.close()
+                t = self.now()
+                with self.lock:
+                    self.results[r["id"]] = {"latency": t - r["arrival"], "finish": t, "device_done": t_dev,
+                                             "digest": dg}
+                    self.reserved -= fp
+                    self.lock.notify_all()
+        except Exception as e:
+            self.errors.append(e)
+
+    def wait_all(self):
+        with self.lock:
+            while len(self.results) < len(self.reqs) and not self.errors:
+                self.lock.wait(0.01)
+            self.stop = True
+            self.lock.notify_all()
+        for t in self.threads:
+            t.join()
+        self.fin_ctx.close()
+        if self.errors:
+            raise self.errors[0]
+        return self.results
+
+
+def serve_serial(ctx, reqs, ids):
+    """Requests `ids` built, stepped alone (spf_step) and digested one after
+    another in the calling thread, with no other context or thread involved:
+    the plain result to check either scheduler against."""
+    out = {}
+    for i in ids:
+        r = reqs[i]
+        d = build(ctx, prepare(r, {}))
+        for _ in range(r["steps"]):
+            d.step()
+        out[i] = digest(d, [d.drain()[0]])
+        d.close()
+    return out
+
+
+def replay_schedule(ctx, reqs, schedule):
+    """The iterations of a recorded continuous-batching schedule repeated in
+    the calling thread on one context: each run is built when it first
+    appears, stepped by spf_step_batch with the same company, digested and
+    destroyed after its last step.  Returns {request id: digest}."""
+    left, runs, out = {}, {}, {}
+    for ids in schedule:
+        for i in ids:
+            if i not in runs:
+                runs[i] = build(ctx, prepare(reqs[i], {}))
+                left[i] = reqs[i]["steps"]
+        ctx.step_batch([runs[i] for i in ids])
+        for i in ids:
+            left[i] -= 1
+            if left[i] == 0:
+                d = runs.pop(i)
+                out[i] = digest(d, [d.drain()[0]])
+                d.close()
+    return out
+
+
+def ctx_device(ctx):
+    return getattr(ctx, "device", 0)
+
+
+def _stats(reqs, done, t_first, t_last):
+    lat = np.array([done[r["id"]]["latency"] for r in reqs])
+    span = t_last - t_first
+    return {"requests": len(reqs), "runs_per_hr": len(reqs) / span * 3600 if span > 0 else None,
+            "makespan_s": span, "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean())}
+
+
+def serve_one_at_a_time(ctx, reqs, memory_budget=200e9, want_digest=True):
+    """FIFO queue, one run on the GPU at a time (host work overlapped as in serve_continuous)."""
+    pipe = _Pipeline(reqs, ctx, memory_budget, want_digest=want_digest, max_built=4)
+    for r in reqs:
+        while True:
+            got = pipe.take(r["id"])
+            if got:
+                break
+            time.sleep(0.0002)
+        d, fp = got
+        for _ in range(r["steps"]):
+            d.step()
+        ctx.synchronize()
+        pipe.finish(r, d, fp)
+    done = pipe.wait_all()
+    return done, _stats(reqs, done, reqs[0]["arrival"], max(v["finish"] for v in done.values()))
+
+
+def serve_continuous(ctx, reqs, memory_budget, max_active=64, want_digest=True, schedule=None):
+    """Continuous batching: admission by footprint, every resident run stepped
+    each iteration.  With a list for `schedule`, the request ids stepped by
+    each iteration are appended to it (replay_schedule repeats them)."""
+    pipe = _Pipeline(reqs, ctx, memory_budget, want_digest=want_digest)
+    active = []          # [request, run, steps left, footprint]
+    admitted = 0
+    peak = 0
+    while admitted < len(reqs) or active:
+        # admit every built run (oldest first): a run was built only once its
+        # footprint was reserved in the budget, so any of them fits
+        for r, d, fp in pipe.take_ready(max_active - len(active)):
+            active.append([r, d, r["steps"], fp])
+            admitted += 1
+        peak = max(peak, len(active))
+        if not active:
+            time.sleep(0.0002)
+            continue
+        ctx.step_batch([a[1] for a in active])
+        if schedule is not None:
+            schedule.append([a[0]["id"] for a in active])
+        for a in active:
+            a[2] -= 1
+        if any(a[2] == 0 for a in active):
+            for a in active:
+                if a[2] == 0:
+                    pipe.finish(a[0], a[1], a[3])
+            active = [a for a in active if a[2] > 0]
+    done = pipe.wait_all()
+    st = _stats(reqs, done, reqs[0]["arrival"], max(v["finish"] for v in done.values()))
+    st["peak_resident_runs"] = peak
+    return done, st

@@ -95,3 +95,34 @@ def test_batched_steps_match_single_steps():
         assert sum(len(r) for rr in batch_rows for r, _ in rr) > 0
         for d in runs:
             d.close()
+
+
+def test_moved_run_matches_unmoved_run():
+    """A run built in one context, moved (spf_run_move) to a second and stepped
+    there, then moved to a third and read, has the bits of a run that never
+    moved; the old ids are gone from the contexts it left."""
+    sc = scenarios()[0]
+    steps = 7
+    with Context(0) as a, Context(0) as b, Context(0) as c:
+        ref = build(a, sc)
+        for _ in range(steps):
+            ref.step()
+        want = state(ref)
+        ref.close()
+        d = build(a, sc)
+        old_id = d.id
+        d.move_to(b)
+        assert d.ctx is b
+        for _ in range(steps):
+            d.step()
+        d.move_to(c)
+        got = state(d)
+        with pytest.raises(RuntimeError):
+            a.check(a.lib.spf_run_destroy(a.h, old_id))
+        for k, v in want.items():
+            if k == "stats":
+                assert got[k] == v
+                continue
+            x, y = np.asarray(v), np.asarray(got[k])
+            assert np.array_equal(bits(x) if x.dtype == np.float64 else x, bits(y) if y.dtype == np.float64 else y), k
+        d.close()
