@@ -8,7 +8,9 @@
  * point returns 0 on success or a nonzero SPF_E* code; spf_last_error() gives
  * the message.  Calls are stream-ordered on the context's HIP stream and do
  * not block unless documented.  A context is used from one host thread and
- * holds any number of runs.
+ * holds any number of runs; several contexts may work at once from several
+ * threads of one process (each run's results are the same bits as when it
+ * runs alone), and spf_run_move hands a run from one context to another.
  *
  * Host arrays crossing this boundary use the reference's layout: a 2-D field
  * is (nz+1) x (nr+1) doubles with element [iz][ir] at iz*(nr+1)+ir
