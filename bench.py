@@ -297,7 +297,7 @@ def main():
         ms, n, launched = tm[name]
         return ms / n * launched / args.steps if n else 0.0
 
-    hs = ctx.lib.spf_sor_batch_half_sweeps()   # half-sweeps per SOR batch launch
+    hs = run.sor_batch_half_sweeps()   # half-sweeps per SOR batch launch of this run's tile shape
     corrections = solves1 - solves0
     sor_half_sweeps = hs1 - hs0
     t_fdtd, t_push, t_dep, t_sor = avg("fdtd"), avg("push"), avg("deposit"), avg("sor_batch")

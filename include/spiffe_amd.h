@@ -401,8 +401,8 @@ int spf_sdds_parameter(const char *path, int64_t page, const char *name, double 
 #define SPF_T_PUSH 3       /* spf_particles_push: push, compaction and event rows */
 #define SPF_T_POISSON 4    /* spf_poisson_correct, when it solves */
 #define SPF_T_SOR_BATCH 5  /* SOR batch kernel launches (up to spf_sor_batch_half_sweeps()
-                              half-sweeps each), timed per group of back-to-back launches
-                              between stopping-rule checks */
+                              of the run half-sweeps each), timed per group of back-to-back
+                              launches between stopping-rule checks */
 #define SPF_T_N 6
 /* every > 0 starts timing one call in `every` of each phase and clears the
  * accumulators; 0 stops it */
@@ -410,9 +410,10 @@ int spf_timing_enable(spf_ctx *ctx, int32_t every);
 /* synchronizing: total milliseconds over the timed launches, the number of timed
  * launches, and (may be NULL) all launches since timing started; arrays of SPF_T_N */
 int spf_timing_read(spf_ctx *ctx, double *ms, int64_t *timed, int64_t *launched);
-/* half-sweeps one SOR batch launch performs (a build-time constant of the
- * library), for converting SPF_T_SOR_BATCH times into per-half-sweep rates */
-int32_t spf_sor_batch_half_sweeps(void);
+/* half-sweeps one SOR batch launch of this run performs (the library picks
+ * one of two build-time tile shapes from the grid size when the run is
+ * created), for converting SPF_T_SOR_BATCH times into per-half-sweep rates */
+int spf_sor_batch_half_sweeps(spf_ctx *ctx, int32_t run, int32_t *half_sweeps);
 
 /* ---- one whole step of the loop body, src/integrate.c:134-194 ----
  * ramped imposed E (if set); time <- step*dt+start; emission from every

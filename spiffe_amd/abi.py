@@ -182,7 +182,7 @@ SIGNATURES = {
     "spf_poisson_sweeps": (ctypes.c_int, [_vp, _i32, _i32, _i64, _i64]),
     "spf_imposed_efield": (ctypes.c_int, [_vp, _i32, _i32, _d, _d]),
     "spf_timing_enable": (ctypes.c_int, [_vp, _i32]),
-    "spf_sor_batch_half_sweeps": (ctypes.c_int32, []),
+    "spf_sor_batch_half_sweeps": (ctypes.c_int, [_vp, _i32, _pi32]),
     "spf_timing_read": (ctypes.c_int, [_vp, _dp, _pi64, _pi64]),
     "spf_step": (ctypes.c_int, [_vp, _i32, _i32, _pi32, _pi64, _pi64, _pi64, _dp]),
     "spf_step_batch": (ctypes.c_int, [_vp, _i32, _pi32]),

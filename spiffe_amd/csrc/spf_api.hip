@@ -324,7 +324,8 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
     ALLOC(t, (int64_t)R->sort_tmp_bytes);
     R->sort_tmp = t;
   }
-  R->red_n = std::max<int64_t>(sor_tiles(*R) * sor_batch() * 4 * 2, 1024) + 16;   // two check groups of partials, or sum2 partials
+  R->sor_shape = sor_shape_for(nz, nr);
+  R->red_n = std::max<int64_t>(sor_tiles(*R) * sor_batch(*R) * 4 * 2, 1024) + 16;   // two check groups of partials, or sum2 partials
   ALLOC(R->red, R->red_n);
   ALLOC(R->red_counter, 4); ALLOC(R->sor_state, 4); ALLOC(R->sor_sums, 16);
   ALLOC(R->sor_masks, sor_mask_words(*R));

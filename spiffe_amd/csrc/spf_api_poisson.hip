@@ -112,7 +112,7 @@ int sor_pass_init(spf_ctx *c, SorPass &p, Run *R, double *Psi, const double *rhs
 // each pass's converged (-1: ran to the limit), ambiguous and nb.
 int sor_run_batches(spf_ctx *c, SorPass *const *ps, int np) {
   hipStream_t s = c->stream;
-  const int K = sor_batch(), KC = sor_check_group();
+  const int K = sor_batch(*ps[0]->R), KC = sor_check_group(*ps[0]->R);
   const int64_t group_rows = (int64_t)KC * K;   // partial rows of one check group
   const int64_t start = ps[0]->start, limit = ps[0]->limit;
   const int64_t nb = (limit - start + K - 1) / K;
@@ -182,7 +182,8 @@ int sor_run_batches(spf_ctx *c, SorPass *const *ps, int np) {
             HIPCALL(c, launch_sor_batch(*ps[0]->R, ps[0]->ring[b % kRing], ps[0]->ring[(b + 1) % kRing], ps[0]->rhs,
                                         n0, k, (int)prow, 0, s, prev_row, prev_n0, ck, accuracy));
           } else {
-            HIPCALL(c, launch_sor_multi(tdev, first_dev, np, total_blocks, n0, b, k, prow, prev_row, prev_n0, ck, s));
+            HIPCALL(c, launch_sor_multi(ps[0]->R->sor_shape, tdev, first_dev, np, total_blocks, n0, b, k, prow, prev_row,
+                                        prev_n0, ck, s));
           }
           if (b == g0) prev_k = 0;
         }
@@ -226,7 +227,7 @@ int sor_run_batches(spf_ctx *c, SorPass *const *ps, int np) {
 int sor_settle(spf_ctx *c, SorPass &p) {
   hipStream_t s = c->stream;
   Run *R = p.R;
-  const int K = sor_batch();
+  const int K = sor_batch(*R);
   const int64_t converged = p.converged;
   if (converged < 0) {
     p.out = (int)(p.nb % kRing);
@@ -299,7 +300,7 @@ int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accura
   return SPF_OK;
 }
 
-// The solves of several runs at once (same accuracy and limit): ragged
+// The solves of several runs at once (same accuracy, limit and tile shape): ragged
 // launches advance them in lockstep until each stops; a run whose stop the
 // host decides against resumes on its own.  Each run's half-sweeps, partial
 // sums and stopping decisions are those of its single-run solve.
@@ -308,6 +309,8 @@ int sor_solve_multi(spf_ctx *c, Run *const *runs, const double *const *rhs, doub
   std::vector<SorPass> ps(np);
   std::vector<SorPass *> pp(np);
   int rc;
+  for (int i = 1; i < np; i++)
+    if (runs[i]->sor_shape != runs[0]->sor_shape) return fail(c, SPF_EINVAL, "runs of one ragged solve share a tile shape");
   for (int i = 0; i < np; i++) {
     if ((rc = sor_pass_init(c, ps[i], runs[i], runs[i]->Psi, rhs[i], accuracy, limit))) return rc;
     pp[i] = &ps[i];
@@ -428,7 +431,12 @@ int poisson_post(spf_ctx *c, Run *R, int space_charge, int64_t it) {
 
 extern "C" {
 
-int32_t spf_sor_batch_half_sweeps(void) { return spf::sor_batch(); }
+int spf_sor_batch_half_sweeps(spf_ctx *c, int32_t run, int32_t *half_sweeps) {
+  Run *R = get_run(c, run);
+  if (!R || !half_sweeps) return fail(c, SPF_EINVAL, "bad run id or null argument");
+  *half_sweeps = sor_batch(*R);
+  return SPF_OK;
+}
 
 int spf_poisson_setup(spf_ctx *c, int32_t run, const spf_poisson_params *p) {
   Run *R = get_run(c, run);
@@ -468,7 +476,7 @@ int spf_poisson_sweeps(spf_ctx *c, int32_t run, int32_t which, int64_t n0, int64
   if (rc) return rc;
   double **Psi = which == 0 ? &R->Psi : &R->PsiImp;
   const double *rhs = which == 0 ? R->Q : nullptr;
-  const int K = sor_batch();
+  const int K = sor_batch(*R);
   HIPCALL(c, launch_sor_masks(*R, c->stream));
   double *buf[2] = {*Psi, R->PsiSpare[0]};
   int64_t nb = (count + K - 1) / K;
@@ -605,7 +613,8 @@ int spf_step_batch(spf_ctx *c, int32_t n_runs, const int32_t *run_ids) {
       return rc;
     }
   }
-  // Poisson corrections: the runs due now solve together, grouped by accuracy and limit
+  // Poisson corrections: the runs due now solve together, grouped by accuracy,
+  // limit and SOR tile shape
   {
     std::vector<int32_t> solving;
     for (int32_t i = 0; i < n_runs; i++) {
@@ -624,7 +633,8 @@ int spf_step_batch(spf_ctx *c, int32_t n_runs, const int32_t *run_ids) {
       std::vector<size_t> idx;
       for (size_t h = g; h < solving.size(); h++) {
         const spf_poisson_params &q = runs[solving[h]]->pc;
-        if (!done[h] && q.accuracy == p.accuracy && q.maximum_iterations == p.maximum_iterations) {
+        if (!done[h] && q.accuracy == p.accuracy && q.maximum_iterations == p.maximum_iterations &&
+            runs[solving[h]]->sor_shape == runs[solving[g]]->sor_shape) {
           done[h] = 1;
           grp.push_back(runs[solving[h]]);
           rhs.push_back(runs[solving[h]]->Q);

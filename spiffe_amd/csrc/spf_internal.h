@@ -160,6 +160,7 @@ struct Run {
   int64_t red_n = 0;
   int32_t *red_counter = nullptr;
   int64_t *sor_state = nullptr; // [0] done, [1] iterations, [2] stop left to the host (within rounding)
+  int sor_shape = 0;            // SOR tile shape (sor_shape_for), fixed at creation
   uint32_t *sor_masks = nullptr; // per-tile lane masks of the SOR batches
   double *sor_sums = nullptr;   // [0] drive sum, [1] last residual sum, [2..] misc
   double last_field_factor = -1;
@@ -244,11 +245,13 @@ hipError_t launch_sum_abs(const double *a, int ldz, int iz_n, int ir_n, double *
 hipError_t launch_sum_plain(const double *a, int ldz, int iz_n, int ir_n, double *partials, int64_t pn, double *out,
                             hipStream_t s);
 hipError_t launch_sor_reset(Run &R, hipStream_t s);
+int sor_shape_for(int nz, int nr);   // the SOR tile shape a grid's runs use (0 small, 1 large)
+int sor_max_batch();                 // the most half-sweeps a batch launch performs, over both shapes
 int64_t sor_tiles(const Run &R);
-int sor_check_group();   // batches per stopping-rule check (<= 4)
+int sor_check_group(const Run &R);   // batches per stopping-rule check (<= 4)
 int64_t sor_mask_words(const Run &R);
 hipError_t launch_sor_masks(Run &R, hipStream_t s);   // before the batches of a solve
-int sor_batch();
+int sor_batch(const Run &R);         // half-sweeps per batch launch of the run's shape
 
 // check_k > 0: the launch carries one extra workgroup that applies the
 // stopping rule to the previous check group (check_k half-sweeps from
@@ -265,9 +268,9 @@ hipError_t launch_sor_check(Run &R, int64_t check_row, int64_t n0, int k, double
 size_t sor_multi_table_bytes(int nruns);
 int64_t sor_multi_table(Run *const *runs, const double *const *rhs, double *const (*rings)[6], int nruns,
                         double accuracy, void *host, void *dev, const int32_t **first_dev);
-hipError_t launch_sor_multi(const void *dev_table, const int32_t *first_dev, int nruns, int64_t total_blocks,
-                            int64_t n0, int64_t b, int k, int64_t partial_row, int64_t check_row, int64_t check_n0,
-                            int check_k, hipStream_t s);
+hipError_t launch_sor_multi(int shape, const void *dev_table, const int32_t *first_dev, int nruns,
+                            int64_t total_blocks, int64_t n0, int64_t b, int k, int64_t partial_row, int64_t check_row,
+                            int64_t check_n0, int check_k, hipStream_t s);   // every run of the table has `shape`
 // half-sweep n in place with the reference's expressions, |residual| per point to dump[ir*nz+iz]
 hipError_t launch_sor_exact_sweep(Run &R, double *P, const double *rhs, int64_t n, double *dump, hipStream_t s);
 // done flag, iteration count and the host-decision flag back to "running"

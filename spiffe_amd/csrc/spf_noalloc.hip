@@ -122,7 +122,7 @@ int spf_poisson_solve_noalloc(spf_ctx *c, double *Psi, const double *rhs, const 
                                    std::to_string(kMaxPoints) + ", SOR tiles <= " + std::to_string(kMaxTiles) + ")");
   if (iteration_limit > kMaxIter)
     return fail(c, SPF_EINVAL, "iteration_limit above the no-allocation limit " + std::to_string(kMaxIter));
-  if (sor_batch() > kPartialRows) return fail(c, SPF_EINVAL, "SOR batch deeper than the partial rows");
+  if (sor_batch(R) > kPartialRows) return fail(c, SPF_EINVAL, "SOR batch deeper than the partial rows");
   R.flags = const_cast<int16_t *>(flags);
   R.sor_masks = reinterpret_cast<uint32_t *>(S + OFF_MASKS);
   R.red = S + OFF_PARTIALS;
@@ -158,7 +158,7 @@ int spf_poisson_solve_noalloc(spf_ctx *c, double *Psi, const double *rhs, const 
   // each followed by its stopping-rule check; once a check stops the solve,
   // later batches return without writing, so the stopping batch's input is
   // still intact and it is re-run for exactly the half-sweeps needed.
-  const int K = sor_batch();
+  const int K = sor_batch(R);
   double *buf[2] = {Psi, S + OFF_SPARE};
   int cur = 0;                 // buf[cur] holds Psi before half-sweep `start`
   int64_t start = 0;

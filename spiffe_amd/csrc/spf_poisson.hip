@@ -57,17 +57,37 @@ struct SorCfg {
   static constexpr unsigned COL0 = 0x55555555u & ((1u << (2 * RS)) - 1u);
   static constexpr unsigned COL1 = 0xAAAAAAAAu & ((1u << (2 * RS)) - 1u);
 };
-// tile shape (half-sweeps per batch, rows per lane, wavefronts), fixed at
-// build time; the default runs 12 half-sweeps per batch on 104 x 24 owned
-// points with 8 wavefronts (48 tile rows)
-#ifndef SPF_SOR_SHAPE
-#define SPF_SOR_SHAPE 12, 6, 8
+// Tile shapes (half-sweeps per batch, rows per lane, wavefronts), fixed at
+// build time; a run takes one of two by its grid size (sor_shape_for).  Small
+// grids run 12 half-sweeps per batch on 104 x 24 owned points (more tiles to
+// fill the GPU, fewer launches); grids of SPF_SOR_LARGE_POINTS points or more
+// run 8 half-sweeps on 112 x 32 owned points (a smaller halo share of the
+// work).  Both have 8 wavefronts, 48 tile rows.  Measured per half-sweep on
+// the 1.6-cell gun: at 1051 x 421 lines the small shape is 16% faster, at
+// 2101 x 841 the large one 7%, at 4201 x 1681 the large one 13%.
+#ifndef SPF_SOR_SHAPE_S
+#define SPF_SOR_SHAPE_S 12, 6, 8
 #endif
-using BatchCfg = SorCfg<SPF_SOR_SHAPE>;
+#ifndef SPF_SOR_SHAPE_L
+#define SPF_SOR_SHAPE_L 8, 6, 8
+#endif
+#ifndef SPF_SOR_LARGE_POINTS
+#define SPF_SOR_LARGE_POINTS 1200000
+#endif
+using SorSmall = SorCfg<SPF_SOR_SHAPE_S>;
+using SorLarge = SorCfg<SPF_SOR_SHAPE_L>;
+// wavefronts per SIMD the batch kernels are compiled for (sets their register budget)
+#ifndef SPF_SOR_WAVES
+#define SPF_SOR_WAVES 4
+#endif
 constexpr int CB = 1024;              // check kernel: 32 lanes per half-sweep, 32 half-sweeps
 // batches per stopping-rule check: as many as one check launch covers, at most 4
-constexpr int kCheckGroup = (CB / 32) / BatchCfg::SK < 4 ? (CB / 32) / BatchCfg::SK : 4;
-static_assert(kCheckGroup >= 1, "a check covers at least one batch of half-sweeps");
+template <class G>
+constexpr int check_group_of() {
+  return (CB / 32) / G::SK < 4 ? (CB / 32) / G::SK : 4;
+}
+static_assert(check_group_of<SorSmall>() >= 1 && check_group_of<SorLarge>() >= 1,
+              "a check covers at least one batch of half-sweeps");
 
 // fixed-order sum over a workgroup: butterfly within each wavefront, then the
 // wavefront sums in order (the same operations every launch)
@@ -583,9 +603,9 @@ __device__ inline void sor_group_check(const double *partials, int ntiles, int k
 // own input and output buffers, half-sweep range, partial rows and check
 // range come as arguments, so a ragged launch over several runs can give each
 // run its own.
+template <class G>
 __device__ inline void sor_batch_block(const SorArgs &a, int bid, const double *Pin, double *Pout, int64_t n0, int k,
                                        double *partials, const double *check_partials, int64_t check_n0, int check_k) {
-  using G = BatchCfg;
   if (bid == a.m.ntiles) {
     // the extra workgroup: stopping rule of the previous check group, on a
     // CU the tiles leave free (tiles that start after it stops the solve
@@ -629,9 +649,10 @@ __device__ inline void sor_batch_block(const SorArgs &a, int bid, const double *
 // CU, so one hides the other's barrier and DPP latencies.  The compiler keeps
 // a few batch constants in scratch to fit; measured 1.37x faster per batch
 // on the 4201 x 1681-line grid than at 147 VGPRs and one workgroup per CU.
-__global__ __launch_bounds__(BatchCfg::NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void sor_batch_kernel(
+template <class G>
+__global__ __launch_bounds__(G::NT) __attribute__((amdgpu_waves_per_eu(SPF_SOR_WAVES, SPF_SOR_WAVES))) void sor_batch_kernel(
     SorArgs a) {
-  sor_batch_block(a, (int)blockIdx.x, a.Pin, a.Pout, a.n0, a.k, a.partials, a.check_partials, a.check_n0, a.check_k);
+  sor_batch_block<G>(a, (int)blockIdx.x, a.Pin, a.Pout, a.n0, a.k, a.partials, a.check_partials, a.check_n0, a.check_k);
 }
 
 // A ragged batch launch over several runs' solves in lockstep: workgroups
@@ -662,21 +683,21 @@ __device__ inline int ragged_run(const int32_t *first, int n, int b) {
   return __builtin_amdgcn_readfirstlane(lo);
 }
 
-__global__ __launch_bounds__(BatchCfg::NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void sor_multi_kernel(
-    const SorMultiRun *__restrict__ T,
-                                                                  const int32_t *__restrict__ first, SorMultiLaunch L) {
+template <class G>
+__global__ __launch_bounds__(G::NT) __attribute__((amdgpu_waves_per_eu(SPF_SOR_WAVES, SPF_SOR_WAVES))) void sor_multi_kernel(
+    const SorMultiRun *__restrict__ T, const int32_t *__restrict__ first, SorMultiLaunch L) {
   const int r = ragged_run(first, L.nruns, (int)blockIdx.x);
   const SorMultiRun &x = T[r];
   const int bid = (int)blockIdx.x - first[r];
   if (bid > x.a.m.ntiles || (bid == x.a.m.ntiles && L.check_k == 0)) return;   // span padding
   const int ntiles = x.a.m.ntiles;
-  sor_batch_block(x.a, bid, x.ring[L.b % 6], const_cast<double *>(x.ring[(L.b + 1) % 6]), L.n0, L.k,
+  sor_batch_block<G>(x.a, bid, x.ring[L.b % 6], const_cast<double *>(x.ring[(L.b + 1) % 6]), L.n0, L.k,
                   x.a.partials + L.partial_row * ntiles, x.a.partials + L.check_row * ntiles, L.check_n0, L.check_k);
 }
 
 // the lanes' update and ownership masks of every tile, once per solve
-__global__ __launch_bounds__(BatchCfg::NT) void sor_masks_kernel(SorCommon a, uint32_t *out) {
-  using G = BatchCfg;
+template <class G>
+__global__ __launch_bounds__(G::NT) void sor_masks_kernel(SorCommon a, uint32_t *out) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const SorTile<G> t(a, (int)blockIdx.x);
   unsigned upd, own;
@@ -930,28 +951,54 @@ SorCommon sor_common(Run &R, const double *rhs, int oz, int orr) {
 // of at most nz*(nr-1) non-negative terms
 double sor_order_margin(const Run &R) { return order_margin((int64_t)R.d.nz * (R.d.nr - 1), 1.0) - 0x1p-1074; }
 
+namespace {
+// calls f with the run's tile shape (an empty SorCfg value)
+template <class F>
+auto with_shape(const Run &R, F &&f) {
+  return R.sor_shape ? f(SorLarge{}) : f(SorSmall{});
+}
+}  // namespace
+
+int sor_shape_for(int nz, int nr) { return (int64_t)(nz + 1) * (nr + 1) >= SPF_SOR_LARGE_POINTS ? 1 : 0; }
+
+int sor_max_batch() { return SorSmall::SK > SorLarge::SK ? SorSmall::SK : SorLarge::SK; }
+
 int64_t sor_tiles(const Run &R) {
-  int64_t tz = (R.d.nz + 1 + BatchCfg::OZ - 1) / BatchCfg::OZ, tr = (R.d.nr + 1 + BatchCfg::OR - 1) / BatchCfg::OR;
-  return tz * tr;
+  return with_shape(R, [&](auto g) {
+    using G = decltype(g);
+    int64_t tz = (R.d.nz + 1 + G::OZ - 1) / G::OZ, tr = (R.d.nr + 1 + G::OR - 1) / G::OR;
+    return tz * tr;
+  });
 }
 
-int sor_batch() { return BatchCfg::SK; }
+int sor_batch(const Run &R) {
+  return with_shape(R, [](auto g) { return decltype(g)::SK; });
+}
 
-int64_t sor_mask_words(const Run &R) { return sor_tiles(R) * BatchCfg::NT; }
+int64_t sor_mask_words(const Run &R) {
+  return with_shape(R, [&](auto g) { return sor_tiles(R) * decltype(g)::NT; });
+}
 
 hipError_t launch_sor_masks(Run &R, hipStream_t s) {
-  SorCommon a = sor_common(R, nullptr, BatchCfg::OZ, BatchCfg::OR);
-  hipLaunchKernelGGL(sor_masks_kernel, dim3((unsigned)a.ntiles), dim3(BatchCfg::NT), 0, s, a, R.sor_masks);
-  return hipGetLastError();
+  return with_shape(R, [&](auto g) {
+    using G = decltype(g);
+    SorCommon a = sor_common(R, nullptr, G::OZ, G::OR);
+    hipLaunchKernelGGL(sor_masks_kernel<G>, dim3((unsigned)a.ntiles), dim3(G::NT), 0, s, a, R.sor_masks);
+    return hipGetLastError();
+  });
 }
 
-int sor_check_group() { return kCheckGroup; }
+int sor_check_group(const Run &R) {
+  return with_shape(R, [](auto g) { return check_group_of<decltype(g)>(); });
+}
 
 hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const double *rhs, int64_t n0, int k,
                             int partial_row, int force, hipStream_t s, int64_t check_row, int64_t check_n0,
                             int check_k, double accuracy) {
+  return with_shape(R, [&](auto g) {
+  using G = decltype(g);
   SorArgs a;
-  a.m = sor_common(R, rhs, BatchCfg::OZ, BatchCfg::OR);
+  a.m = sor_common(R, rhs, G::OZ, G::OR);
   a.Pin = Pin;
   a.Pout = Pout;
   a.partials = R.red + (int64_t)partial_row * a.m.ntiles;
@@ -966,8 +1013,9 @@ hipError_t launch_sor_batch(Run &R, const double *Pin, double *Pout, const doubl
   a.check_n0 = check_n0;
   a.check_k = check_k;
   const unsigned grid = (unsigned)a.m.ntiles + (check_k > 0 ? 1u : 0u);
-  hipLaunchKernelGGL(sor_batch_kernel, dim3(grid), dim3(BatchCfg::NT), 0, s, a);
+  hipLaunchKernelGGL(sor_batch_kernel<G>, dim3(grid), dim3(G::NT), 0, s, a);
   return hipGetLastError();
+  });
 }
 
 // ragged launches over several runs: the per-run table (fixed for a solve),
@@ -985,7 +1033,7 @@ int64_t sor_multi_table(Run *const *runs, const double *const *rhs, double *cons
     Run &R = *runs[i];
     SorArgs &a = T[i].a;
     memset(&a, 0, sizeof(a));
-    a.m = sor_common(R, rhs[i], BatchCfg::OZ, BatchCfg::OR);
+    a.m = with_shape(R, [&](auto g) { return sor_common(R, rhs[i], decltype(g)::OZ, decltype(g)::OR); });
     a.partials = R.red;
     a.state = R.sor_state;
     a.sums = R.sor_sums;
@@ -1000,9 +1048,9 @@ int64_t sor_multi_table(Run *const *runs, const double *const *rhs, double *cons
   return total;
 }
 
-hipError_t launch_sor_multi(const void *dev_table, const int32_t *first_dev, int nruns, int64_t total_blocks,
-                            int64_t n0, int64_t b, int k, int64_t partial_row, int64_t check_row, int64_t check_n0,
-                            int check_k, hipStream_t s) {
+hipError_t launch_sor_multi(int shape, const void *dev_table, const int32_t *first_dev, int nruns,
+                            int64_t total_blocks, int64_t n0, int64_t b, int k, int64_t partial_row, int64_t check_row,
+                            int64_t check_n0, int check_k, hipStream_t s) {
   SorMultiLaunch L;
   L.n0 = n0;
   L.b = b;
@@ -1012,8 +1060,12 @@ hipError_t launch_sor_multi(const void *dev_table, const int32_t *first_dev, int
   L.k = k;
   L.check_k = check_k;
   L.nruns = nruns;
-  hipLaunchKernelGGL(sor_multi_kernel, dim3((unsigned)total_blocks), dim3(BatchCfg::NT), 0, s,
-                     (const SorMultiRun *)dev_table, first_dev, L);
+  if (shape)
+    hipLaunchKernelGGL(sor_multi_kernel<SorLarge>, dim3((unsigned)total_blocks), dim3(SorLarge::NT), 0, s,
+                       (const SorMultiRun *)dev_table, first_dev, L);
+  else
+    hipLaunchKernelGGL(sor_multi_kernel<SorSmall>, dim3((unsigned)total_blocks), dim3(SorSmall::NT), 0, s,
+                       (const SorMultiRun *)dev_table, first_dev, L);
   return hipGetLastError();
 }
 
@@ -1033,7 +1085,7 @@ hipError_t launch_sor_check(Run &R, int64_t check_row, int64_t n0, int k, double
 
 hipError_t launch_sor_exact_sweep(Run &R, double *P, const double *rhs, int64_t n, double *dump, hipStream_t s) {
   ExactSweepArgs x;
-  x.m = sor_common(R, rhs, BatchCfg::OZ, BatchCfg::OR);
+  x.m = with_shape(R, [&](auto g) { return sor_common(R, rhs, decltype(g)::OZ, decltype(g)::OR); });
   x.P = P;
   x.dump = dump;
   x.n = n;

@@ -344,6 +344,12 @@ class DeviceRun:
         self.ctx.check(self.lib.spf_poisson_correct(self.ctx.h, self.id, 1 if space_charge else 0, ctypes.byref(it)))
         return it.value
 
+    def sor_batch_half_sweeps(self):
+        """Half-sweeps one SOR batch launch of this run performs (its tile shape)."""
+        v = ctypes.c_int32()
+        self.ctx.check(self.lib.spf_sor_batch_half_sweeps(self.ctx.h, self.id, ctypes.byref(v)))
+        return v.value
+
     def poisson_stats(self):
         """(result of the last correction that solved, corrections solved, half-sweeps in all)."""
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
