@@ -41,7 +41,7 @@ void free_run(Run *R) {
   F(R->EzImp); F(R->ErImp); F(R->PsiImp); F(R->BzImp); F(R->BrImp); F(R->flags); F(R->cut0); F(R->cut1);
   F(R->cp); F(R->cm); F(R->bzf); F(R->sor_c); F(R->sor_d); F(R->volume);
   for (int k = 0; k < 16; k++) F(R->P[k]);
-  F(R->pid); F(R->gen); F(R->status); F(R->counters); F(R->evt); F(R->evflag);
+  F(R->pid); F(R->gen); F(R->status); F(R->counters); F(R->krange); F(R->evt); F(R->evflag);
   for (int k = 0; k < 7; k++) F(R->ev[k]);
   F(R->last_old); F(R->evlist); F(R->move_src); F(R->move_dst); F(R->srows); F(R->lrows);
   F(R->keys); F(R->skeys); F(R->vals); F(R->cz); F(R->scz); F(R->long_list); F(R->pcount); F(R->gflag); F(R->sort_tmp);
@@ -308,6 +308,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   for (int k = 0; k < 16; k++) ALLOC(R->P[k], cap);
   ALLOC(R->pid, cap); ALLOC(R->gen, cap); ALLOC(R->status, cap);
   ALLOC(R->counters, 16);
+  ALLOC(R->krange, 2 * 256 + 2);
   ALLOC(R->evt, cap); ALLOC(R->evflag, cap); ALLOC(R->pcount, cap / 256 + 2);
   for (int k = 0; k < 7; k++) ALLOC(R->ev[k], cap);
   ALLOC(R->last_old, 4); ALLOC(R->evlist, cap); ALLOC(R->move_src, cap); ALLOC(R->move_dst, cap);
@@ -324,11 +325,12 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
     ALLOC(t, (int64_t)R->sort_tmp_bytes);
     R->sort_tmp = t;
   }
-  R->sor_shape = sor_shape_for(nz, nr);
+  R->sor_shape = 0;   // the small shape needs the most tiles, mask words and partial rows
   R->red_n = std::max<int64_t>(sor_tiles(*R) * sor_batch(*R) * 4 * 2, 1024) + 16;   // two check groups of partials, or sum2 partials
   ALLOC(R->red, R->red_n);
   ALLOC(R->red_counter, 4); ALLOC(R->sor_state, 4); ALLOC(R->sor_sums, 16);
   ALLOC(R->sor_masks, sor_mask_words(*R));
+  R->sor_shape = sor_shape_for((int64_t)(nz + 1) * (nr + 1));
   ALLOC(R->Pdev, 16);
 #undef ALLOC
   hipStream_t s = c->stream;

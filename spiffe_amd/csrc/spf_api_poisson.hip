@@ -87,11 +87,13 @@ struct SorPass {
   int64_t converged = -1, ambiguous = 0, nb = 0;   // what the last pass of batches reported
 };
 
-int sor_pass_init(spf_ctx *c, SorPass &p, Run *R, double *Psi, const double *rhs, double accuracy, int64_t limit) {
+int sor_pass_init(spf_ctx *c, SorPass &p, Run *R, double *Psi, const double *rhs, double accuracy, int64_t limit,
+                  int shape) {
   if (accuracy <= 0) return fail(c, SPF_EINVAL, "accuracy is <= 0 in solve_poisson_cyl");
   if (limit <= 1) return fail(c, SPF_EINVAL, "iteration_limit is <= 1 in solve_poisson_cyl");
   int rc = ensure_omega(c, R, limit);
   if (rc) return rc;
+  R->sor_shape = shape;   // the masks below and every launch of this solve use it
   hipStream_t s = c->stream;
   HIPCALL(c, launch_sor_reset(*R, s));
   HIPCALL(c, launch_sor_masks(*R, s));
@@ -293,14 +295,14 @@ int sor_finish(spf_ctx *c, SorPass &p) {
 
 int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accuracy, int64_t limit, int64_t *iters) {
   SorPass p;
-  int rc = sor_pass_init(c, p, R, *Psi, rhs, accuracy, limit);
+  int rc = sor_pass_init(c, p, R, *Psi, rhs, accuracy, limit, sor_shape_for((int64_t)(R->d.nz + 1) * (R->d.nr + 1)));
   if (rc || (rc = sor_finish(c, p))) return rc;
   *iters = p.iters;
   adopt_result(R, Psi, p.ring, kRing, p.out);
   return SPF_OK;
 }
 
-// The solves of several runs at once (same accuracy, limit and tile shape): ragged
+// The solves of several runs at once (same accuracy and limit): ragged
 // launches advance them in lockstep until each stops; a run whose stop the
 // host decides against resumes on its own.  Each run's half-sweeps, partial
 // sums and stopping decisions are those of its single-run solve.
@@ -309,10 +311,13 @@ int sor_solve_multi(spf_ctx *c, Run *const *runs, const double *const *rhs, doub
   std::vector<SorPass> ps(np);
   std::vector<SorPass *> pp(np);
   int rc;
-  for (int i = 1; i < np; i++)
-    if (runs[i]->sor_shape != runs[0]->sor_shape) return fail(c, SPF_EINVAL, "runs of one ragged solve share a tile shape");
+  // one tile shape for the whole ragged solve, chosen by the points it covers:
+  // together the runs fill the GPU as one large grid would
+  int64_t points = 0;
+  for (int i = 0; i < np; i++) points += (int64_t)(runs[i]->d.nz + 1) * (runs[i]->d.nr + 1);
+  const int shape = sor_shape_for(points);
   for (int i = 0; i < np; i++) {
-    if ((rc = sor_pass_init(c, ps[i], runs[i], runs[i]->Psi, rhs[i], accuracy, limit))) return rc;
+    if ((rc = sor_pass_init(c, ps[i], runs[i], runs[i]->Psi, rhs[i], accuracy, limit, shape))) return rc;
     pp[i] = &ps[i];
   }
   for (int i0 = 0; i0 < np; i0 += 64) {
@@ -613,8 +618,7 @@ int spf_step_batch(spf_ctx *c, int32_t n_runs, const int32_t *run_ids) {
       return rc;
     }
   }
-  // Poisson corrections: the runs due now solve together, grouped by accuracy,
-  // limit and SOR tile shape
+  // Poisson corrections: the runs due now solve together, grouped by accuracy and limit
   {
     std::vector<int32_t> solving;
     for (int32_t i = 0; i < n_runs; i++) {
@@ -633,8 +637,7 @@ int spf_step_batch(spf_ctx *c, int32_t n_runs, const int32_t *run_ids) {
       std::vector<size_t> idx;
       for (size_t h = g; h < solving.size(); h++) {
         const spf_poisson_params &q = runs[solving[h]]->pc;
-        if (!done[h] && q.accuracy == p.accuracy && q.maximum_iterations == p.maximum_iterations &&
-            runs[solving[h]]->sor_shape == runs[solving[g]]->sor_shape) {
+        if (!done[h] && q.accuracy == p.accuracy && q.maximum_iterations == p.maximum_iterations) {
           done[h] = 1;
           grp.push_back(runs[solving[h]]);
           rhs.push_back(runs[solving[h]]->Q);

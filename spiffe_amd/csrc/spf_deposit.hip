@@ -7,10 +7,13 @@
 // index order, one pass per component.  Floating-point addition is not
 // associative, so the device keeps exactly that order:
 //  1. key: for every active particle and component c the target cell and the
-//     contribution value, key = (c, cell), value = the contribution;
-//  2. a stable radix sort of the (key, value) pairs: inputs are laid out in
-//     particle index order per component, so each cell's contributions come
-//     out contiguous and still in index order;
+//     contribution value, key = cell * 4 + c, value = the contribution; the
+//     kernel also finds the smallest and largest key in use;
+//  2. a stable radix sort of the (key, value) pairs over that key range (a
+//     bunch covers a band of grid rows, so fewer 7-bit passes cover it than
+//     the whole key space would need): inputs
+//     are laid out in particle index order per component, so each cell's
+//     contributions come out contiguous and still in index order;
 //  3. each cell segment (the run of one key) is folded sequentially onto the
 //     cell's current value by one thread, or by one wavefront when it is
 //     longer than LONG_SEG values.
@@ -45,6 +48,7 @@ struct KeyArgs {
   uint32_t discard;
   int cell_bits;
   int64_t *counters;            // [9] (number of long segments) cleared by thread 0
+  unsigned long long *krange;   // [0, 256): per-slot smallest key, [256, 512): largest, [512], [513]: the range
   double *clear[3];             // target arrays to set to +0 first (or null), clear_n cells each
   int64_t clear_n;
   int zero_terms;               // COMP_Q: apply the signed-zero effect of the zero-weight terms
@@ -57,6 +61,33 @@ __device__ inline double jm_centered(double QVG, long long ir) {
   return 1.0 * QVG / ((ir + 0.5) * (ir + 0.5) - (ir - 0.5) * (ir - 0.5));
 }
 
+// The key range: each key workgroup folds its keys' smallest and largest into
+// one of 256 slots (integer min/max: the same result in any order; the slots
+// keep the atomics off one address), and one workgroup then folds the slots.
+constexpr int kRangeSlots = 256;
+__global__ __launch_bounds__(kRangeSlots) void key_range_reset_kernel(unsigned long long *krange) {
+  krange[threadIdx.x] = ~0ull;
+  krange[kRangeSlots + threadIdx.x] = 0ull;
+}
+
+__global__ __launch_bounds__(kRangeSlots) void key_range_final_kernel(unsigned long long *krange) {
+  __shared__ unsigned long long lo[kRangeSlots], hi[kRangeSlots];
+  lo[threadIdx.x] = krange[threadIdx.x];
+  hi[threadIdx.x] = krange[kRangeSlots + threadIdx.x];
+  __syncthreads();
+  for (int off = kRangeSlots / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      lo[threadIdx.x] = min(lo[threadIdx.x], lo[threadIdx.x + off]);
+      hi[threadIdx.x] = max(hi[threadIdx.x], hi[threadIdx.x + off]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    krange[2 * kRangeSlots] = lo[0];
+    krange[2 * kRangeSlots + 1] = hi[0];
+  }
+}
+
 __global__ __launch_bounds__(256) void key_kernel(KeyArgs a) {
   int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (ip == 0) a.counters[9] = 0;
@@ -65,9 +96,12 @@ __global__ __launch_bounds__(256) void key_kernel(KeyArgs a) {
   for (int64_t q = ip; q < a.clear_n; q += (int64_t)gridDim.x * blockDim.x)
     for (int c = 0; c < 3; c++)
       if (a.clear[c]) a.clear[c][q] = 0.0;
-  if (ip >= a.n_ub) return;
-  int64_t np = *a.np;
-  bool live = ip < np && (a.status[ip] & SPF_PART_ACTIVE);
+  // smallest and largest key this lane writes (every lane takes part in the
+  // wavefront's reduction below, so no lane returns early)
+  uint32_t kmin = 0xffffffffu, kmax = 0u;
+  const bool in = ip < a.n_ub;
+  int64_t np = in ? *a.np : 0;
+  bool live = in && ip < np && (a.status[ip] & SPF_PART_ACTIVE);
   long long izh = -1, izA = -1, irh = -1, irA = -1;
   double Qo = 0;
   if (live) {
@@ -83,7 +117,7 @@ __global__ __launch_bounds__(256) void key_kernel(KeyArgs a) {
     Qo = a.Q[ip] * a.one_over_piDzDr2;
   }
   const int nz = a.nz, nr = a.nr;
-  for (int k = 0; k < a.ncomp; k++) {
+  for (int k = 0; in && k < a.ncomp; k++) {
     int c = a.comp[k];
     long long iz = -1, ir = -1;
     double v = 0;
@@ -145,12 +179,37 @@ __global__ __launch_bounds__(256) void key_kernel(KeyArgs a) {
     }
     int64_t slot = (int64_t)k * a.n_ub + ip;
     if (iz >= 0) {
-      uint32_t cell = (uint32_t)(ir * (nz + 1) + iz);
-      a.keys[slot] = ((uint32_t)k << a.cell_bits) | cell;
+      const uint32_t cell = (uint32_t)(ir * (nz + 1) + iz);
+      const uint32_t key = (cell << 2) | (uint32_t)k;
+      a.keys[slot] = key;
       a.vals[slot] = v;
+      kmin = min(kmin, key);
+      kmax = max(kmax, key);
     } else {
       a.keys[slot] = a.discard;
       a.vals[slot] = 0.0;
+    }
+  }
+  if (!a.krange) return;   // the sort covers the whole key space
+  for (int off = 32; off > 0; off >>= 1) {
+    kmin = min(kmin, (uint32_t)__shfl_xor((int)kmin, off, 64));
+    kmax = max(kmax, (uint32_t)__shfl_xor((int)kmax, off, 64));
+  }
+  __shared__ uint32_t wmin[4], wmax[4];
+  if ((threadIdx.x & 63) == 0) {
+    wmin[threadIdx.x >> 6] = kmin;
+    wmax[threadIdx.x >> 6] = kmax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; w++) {
+      kmin = min(kmin, wmin[w]);
+      kmax = max(kmax, wmax[w]);
+    }
+    if (kmin <= kmax) {
+      const int slot = blockIdx.x % kRangeSlots;
+      atomicMin(a.krange + slot, (unsigned long long)kmin);
+      atomicMax(a.krange + kRangeSlots + slot, (unsigned long long)kmax);
     }
   }
 }
@@ -159,16 +218,20 @@ struct FoldArgs {
   int nz, ldz, cell_bits;
   int64_t n;
   uint32_t discard;
-  const uint32_t *sk;
+  const uint32_t *sk;           // sorted pairs: the sort's first buffer, or sk1/sv1 (ranged_sort_in_k1)
   const double *sv;
+  const uint32_t *sk1;
+  const double *sv1;
+  const uint64_t *range;        // key range of the ranged sort (null: sk/sv hold the result)
   int64_t *counters;    // [9]: number of long segments
   int32_t *long_list;   // first positions of segments longer than LONG_SEG
   double *target[4];
 };
 
+
 __device__ inline double *target_cell(const FoldArgs &a, uint32_t key) {
-  uint32_t k = key >> a.cell_bits;
-  uint32_t cell = key & ((1u << a.cell_bits) - 1u);
+  uint32_t k = key & 3u;
+  uint32_t cell = key >> 2;
   int iz = (int)(cell % (uint32_t)(a.nz + 1)), ir = (int)(cell / (uint32_t)(a.nz + 1));
   return a.target[k] + fix(a.ldz, iz, ir);
 }
@@ -186,13 +249,17 @@ constexpr int FB = 16;
 constexpr int64_t LONG_SEG = 256;
 
 __global__ __launch_bounds__(256) void fold_kernel(FoldArgs a) {
+  // the buffer the ranged sort left its result in
+  const bool in1 = a.range && ranged_sort_in_k1(a.range);
+  const uint32_t *__restrict__ sk = in1 ? a.sk1 : a.sk;
+  const double *__restrict__ sv = in1 ? a.sv1 : a.sv;
   const int lane = threadIdx.x & 63;
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, base = p - lane;
   if (base >= a.n) return;
   const bool valid = p < a.n;
-  const uint32_t key = valid ? a.sk[p] : 0u;
+  const uint32_t key = valid ? sk[p] : 0u;
   // a run starts here, or the array has ended
-  const bool edge = !valid || p == 0 || a.sk[p - 1] != key;
+  const bool edge = !valid || p == 0 || sk[p - 1] != key;
   const unsigned long long em = __ballot(edge);
   if (em == 0) return;   // the window lies inside one run
   const unsigned long long above = em & ~((2ull << lane) - 1ull);
@@ -204,13 +271,13 @@ __global__ __launch_bounds__(256) void fold_kernel(FoldArgs a) {
   bool longL = false;
   int64_t eL = 0;
   if (sL < a.n && kL != a.discard) {
-    if (sL + LONG_SEG < a.n && a.sk[sL + LONG_SEG] == kL) {
+    if (sL + LONG_SEG < a.n && sk[sL + LONG_SEG] == kL) {
       longL = true;
     } else {
       // sorted keys: the run ends within LONG_SEG positions of sL
       for (int64_t q = base + 64;; q += 64) {
         const int64_t i = q + lane;
-        const unsigned long long dm = __ballot(i >= a.n || a.sk[i] != kL);
+        const unsigned long long dm = __ballot(i >= a.n || sk[i] != kL);
         if (dm) {
           eL = q + (__ffsll((long long)dm) - 1);
           break;
@@ -235,10 +302,10 @@ __global__ __launch_bounds__(256) void fold_kernel(FoldArgs a) {
     if (e0 - i >= FB) {
       double cur[FB], nxt[FB];
 #pragma unroll
-      for (int k = 0; k < FB; k++) cur[k] = a.sv[i + k];
+      for (int k = 0; k < FB; k++) cur[k] = sv[i + k];
       for (; i + 2 * FB <= e0; i += FB) {
 #pragma unroll
-        for (int k = 0; k < FB; k++) nxt[k] = a.sv[i + FB + k];
+        for (int k = 0; k < FB; k++) nxt[k] = sv[i + FB + k];
 #pragma unroll
         for (int k = 0; k < FB; k++) s += cur[k];
 #pragma unroll
@@ -248,7 +315,7 @@ __global__ __launch_bounds__(256) void fold_kernel(FoldArgs a) {
       for (int k = 0; k < FB; k++) s += cur[k];
       i += FB;
     }
-    for (; i < e0; i++) s += a.sv[i];
+    for (; i < e0; i++) s += sv[i];
     *t = s;
   }
 }
@@ -283,12 +350,15 @@ constexpr int LU = 16;            // values per lane per chunk
 constexpr int LCH = 64 * LU;      // values per chunk
 
 __global__ __launch_bounds__(64) void fold_long_kernel(FoldArgs a) {
+  const bool in1 = a.range && ranged_sort_in_k1(a.range);
+  const uint32_t *__restrict__ sk = in1 ? a.sk1 : a.sk;
+  const double *__restrict__ sv = in1 ? a.sv1 : a.sv;
   __shared__ __align__(16) double buf[LCH];
   const int lane = threadIdx.x;
   const int64_t nl = a.counters[9];
   for (int64_t q = blockIdx.x; q < nl; q += gridDim.x) {
     const int64_t s0 = a.long_list[q];
-    const uint32_t key = a.sk[s0];
+    const uint32_t key = sk[s0];
     double *t = target_cell(a, key);
     double s = lane == 0 ? *t : 0.0;
     double v[LU];
@@ -296,8 +366,8 @@ __global__ __launch_bounds__(64) void fold_long_kernel(FoldArgs a) {
 #pragma unroll
     for (int u = 0; u < LU; u++) {
       const int64_t i = s0 + u * 64 + lane;
-      kk[u] = i < a.n ? a.sk[i] : ~key;
-      v[u] = i < a.n ? a.sv[i] : 0.0;
+      kk[u] = i < a.n ? sk[i] : ~key;
+      v[u] = i < a.n ? sv[i] : 0.0;
     }
     for (int64_t c0 = s0;; c0 += LCH) {
       // the segment's values in this chunk are a prefix of it, in position
@@ -316,8 +386,8 @@ __global__ __launch_bounds__(64) void fold_long_kernel(FoldArgs a) {
 #pragma unroll
         for (int u = 0; u < LU; u++) {
           const int64_t i = c1 + u * 64 + lane;
-          kk[u] = i < a.n ? a.sk[i] : ~key;
-          v[u] = i < a.n ? a.sv[i] : 0.0;
+          kk[u] = i < a.n ? sk[i] : ~key;
+          v[u] = i < a.n ? sv[i] : 0.0;
         }
       }
       if (lane == 0) {
@@ -374,7 +444,7 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   const int64_t n = n_ub * ncomp;
   const int cell_bits = deposit_cell_bits(R.d.nz, R.d.nr);
   const int key_bits = cell_bits + 2;
-  const uint32_t discard = (1u << key_bits) - 1u;   // (3 << cell_bits) | all-ones cell: never a real key
+  const uint32_t discard = (1u << key_bits) - 1u;   // all-ones cell with slot 3: never a real key
   KeyArgs k;
   k.nz = R.d.nz;
   k.nr = R.d.nr;
@@ -411,14 +481,25 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   // a wide grid-stride clear (workgroups past n_ub only clear)
   int64_t kblocks = (n_ub + 255) / 256;
   if (clear_first) kblocks = std::max<int64_t>(kblocks, std::min<int64_t>(8192, (R.cells + 255) / 256));
+  // Over keys wider than three 7-bit digits the sort takes the key range
+  // found by the key kernel (a bunch spans a band of rows, so it often needs a
+  // pass fewer); narrower keys keep the plain sort, whose three passes the
+  // range could not shorten and whose set-up it would only add to.
+  const bool ranged = key_bits > 21;
+  k.krange = ranged ? reinterpret_cast<unsigned long long *>(R.krange) : nullptr;
+  if (ranged) hipLaunchKernelGGL(key_range_reset_kernel, dim3(1), dim3(kRangeSlots), 0, s, k.krange);
   hipLaunchKernelGGL(key_kernel, dim3((unsigned)kblocks), dim3(256), 0, s, k);
+  if (ranged) hipLaunchKernelGGL(key_range_final_kernel, dim3(1), dim3(kRangeSlots), 0, s, k.krange);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   // the first component's values in particle order, before the sort reuses their buffer
   if (sums && (e = launch_sum2_vec(R.cz, n_ub, R.red, R.red_n, sums, s)) != hipSuccess) return e;
-  uint32_t *sk;
-  double *sv;
-  e = radix_sort_pairs(R.keys, R.cz, R.skeys, R.scz, n, key_bits, R.sort_tmp, R.sort_tmp_bytes, &sk, &sv, s);
+  const uint64_t *range = ranged ? reinterpret_cast<const uint64_t *>(R.krange + 2 * kRangeSlots) : nullptr;
+  uint32_t *sk = R.keys;
+  double *sv = R.cz;
+  e = ranged ? radix_sort_pairs_ranged(R.keys, R.cz, R.skeys, R.scz, n, key_bits, range, discard, R.sort_tmp,
+                                       R.sort_tmp_bytes, s)
+             : radix_sort_pairs(R.keys, R.cz, R.skeys, R.scz, n, key_bits, R.sort_tmp, R.sort_tmp_bytes, &sk, &sv, s);
   if (e != hipSuccess) return e;
   FoldArgs f;
   f.nz = R.d.nz;
@@ -428,6 +509,9 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   f.discard = discard;
   f.sk = sk;
   f.sv = sv;
+  f.sk1 = R.skeys;
+  f.sv1 = R.scz;
+  f.range = range;
   f.counters = R.counters;
   f.long_list = R.long_list;
   for (int i = 0; i < 4; i++) f.target[i] = i < ncomp ? targets[i] : nullptr;

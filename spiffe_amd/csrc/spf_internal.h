@@ -123,6 +123,7 @@ struct Run {
   // push, [7] newly active particles of a push, [8] event-ring overflow flag,
   // [9] long segments of a deposit; [6] unused
   int64_t *counters = nullptr;
+  int64_t *krange = nullptr;    // deposition key range: 256 min and 256 max slots, then the range (2)
   unsigned long long *new_active = nullptr;
   int32_t *overflow = nullptr;
   // per-step events
@@ -160,7 +161,7 @@ struct Run {
   int64_t red_n = 0;
   int32_t *red_counter = nullptr;
   int64_t *sor_state = nullptr; // [0] done, [1] iterations, [2] stop left to the host (within rounding)
-  int sor_shape = 0;            // SOR tile shape (sor_shape_for), fixed at creation
+  int sor_shape = 0;            // SOR tile shape of the current or last solve (buffers are sized for shape 0)
   uint32_t *sor_masks = nullptr; // per-tile lane masks of the SOR batches
   double *sor_sums = nullptr;   // [0] drive sum, [1] last residual sum, [2..] misc
   double last_field_factor = -1;
@@ -220,6 +221,21 @@ int deposit_cell_bits(int nz, int nr);
 size_t radix_sort_tmp_bytes(int64_t n);
 hipError_t radix_sort_pairs(uint32_t *k0, double *v0, uint32_t *k1, double *v1, int64_t n, int key_bits, void *tmp,
                             size_t tmp_bytes, uint32_t **kres, double **vres, hipStream_t s);
+// The same sort over keys whose values other than `discard` lie in the device
+// range [range[0], range[1]] (uint64; range[1] < range[0]: none): 7-bit digits
+// of the key's offset in the range (discard after every real key), and only
+// the passes the range needs do work.  The result is in (k0, v0) when an even
+// number of passes worked (ranged_sort_in_k1 on the device), else in (k1, v1).
+inline int sorted_passes_max(int key_bits) { return (key_bits + 6) / 7; }
+hipError_t radix_sort_pairs_ranged(uint32_t *k0, double *v0, uint32_t *k1, double *v1, int64_t n, int key_bits,
+                                   const uint64_t *range, uint32_t discard, void *tmp, size_t tmp_bytes,
+                                   hipStream_t s);
+__device__ inline bool ranged_sort_in_k1(const uint64_t *range) {
+  const uint64_t lo = range[0], hi = range[1];
+  const uint32_t span = hi >= lo ? (uint32_t)(hi - lo) + 1u : 0u;
+  const int nbits = span ? 32 - __builtin_clz(span) : 0;
+  return ((nbits + 6) / 7) & 1;
+}
 // Poisson
 hipError_t launch_gauss_q(Run &R, hipStream_t s);
 // Q -= particle charge (src/poisson_correction.c:184-243); with qreal_sums the
@@ -245,7 +261,7 @@ hipError_t launch_sum_abs(const double *a, int ldz, int iz_n, int ir_n, double *
 hipError_t launch_sum_plain(const double *a, int ldz, int iz_n, int ir_n, double *partials, int64_t pn, double *out,
                             hipStream_t s);
 hipError_t launch_sor_reset(Run &R, hipStream_t s);
-int sor_shape_for(int nz, int nr);   // the SOR tile shape a grid's runs use (0 small, 1 large)
+int sor_shape_for(int64_t points);   // SOR tile shape of a solve over this many grid points (0 small, 1 large)
 int sor_max_batch();                 // the most half-sweeps a batch launch performs, over both shapes
 int64_t sor_tiles(const Run &R);
 int sor_check_group(const Run &R);   // batches per stopping-rule check (<= 4)

@@ -58,13 +58,15 @@ struct SorCfg {
   static constexpr unsigned COL1 = 0xAAAAAAAAu & ((1u << (2 * RS)) - 1u);
 };
 // Tile shapes (half-sweeps per batch, rows per lane, wavefronts), fixed at
-// build time; a run takes one of two by its grid size (sor_shape_for).  Small
-// grids run 12 half-sweeps per batch on 104 x 24 owned points (more tiles to
-// fill the GPU, fewer launches); grids of SPF_SOR_LARGE_POINTS points or more
-// run 8 half-sweeps on 112 x 32 owned points (a smaller halo share of the
-// work).  Both have 8 wavefronts, 48 tile rows.  Measured per half-sweep on
-// the 1.6-cell gun: at 1051 x 421 lines the small shape is 16% faster, at
-// 2101 x 841 the large one 7%, at 4201 x 1681 the large one 13%.
+// build time; each solve takes one of two by the grid points it covers
+// (sor_shape_for: the run's, or the sum over the runs of a ragged solve).
+// Small solves run 12 half-sweeps per batch on 104 x 24 owned points (more
+// tiles to fill the GPU, fewer launches); from SPF_SOR_LARGE_POINTS points up
+// they run 8 half-sweeps on 112 x 32 owned points (a smaller halo share of the
+// work).  Both have 8 wavefronts, 48 tile rows; the large shape never needs
+// more tiles, mask words or partial rows than the small one.  Measured per
+// half-sweep on the 1.6-cell gun: at 1051 x 421 lines the small shape is 16%
+// faster, at 2101 x 841 the large one 7%, at 4201 x 1681 the large one 13%.
 #ifndef SPF_SOR_SHAPE_S
 #define SPF_SOR_SHAPE_S 12, 6, 8
 #endif
@@ -88,6 +90,9 @@ constexpr int check_group_of() {
 }
 static_assert(check_group_of<SorSmall>() >= 1 && check_group_of<SorLarge>() >= 1,
               "a check covers at least one batch of half-sweeps");
+static_assert(SorLarge::OZ >= SorSmall::OZ && SorLarge::OR >= SorSmall::OR && SorLarge::NT <= SorSmall::NT &&
+                  SorLarge::SK <= SorSmall::SK,
+              "buffers sized for the small shape must hold the large shape's tiles, masks and partial rows");
 
 // fixed-order sum over a workgroup: butterfly within each wavefront, then the
 // wavefront sums in order (the same operations every launch)
@@ -959,7 +964,7 @@ auto with_shape(const Run &R, F &&f) {
 }
 }  // namespace
 
-int sor_shape_for(int nz, int nr) { return (int64_t)(nz + 1) * (nr + 1) >= SPF_SOR_LARGE_POINTS ? 1 : 0; }
+int sor_shape_for(int64_t points) { return points >= SPF_SOR_LARGE_POINTS ? 1 : 0; }
 
 int sor_max_batch() { return SorSmall::SK > SorLarge::SK ? SorSmall::SK : SorLarge::SK; }
 

@@ -20,8 +20,11 @@ constexpr uint32_t ACTIVE = SPF_PART_ACTIVE, LOST = SPF_PART_LOST;
 constexpr uint32_t EV_LOST = 1u, EV_RNEG = 2u;  // screen s crossing: bit 2+s
 
 // C conversion double -> long truncates toward zero; out-of-range values and
-// NaN give LONG_MIN, as the x86-64 conversion instruction does.
+// NaN give LONG_MIN, as the x86-64 conversion instruction does.  Grid indices
+// are far inside 32 bits, where the one-instruction 32-bit conversion
+// truncates to the same value; the 64-bit sequence runs only for the rest.
 __device__ inline long long trunc_l(double x) {
+  if (__builtin_expect(__builtin_fabs(x) < 2147483648.0, 1)) return (long long)(int)x;
   if (!(x > -9.2233720368547758e18 && x < 9.2233720368547758e18)) return LLONG_MIN;
   return (long long)x;
 }

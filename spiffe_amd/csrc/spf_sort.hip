@@ -17,6 +17,13 @@
 //     are contiguous.
 // The count and offset tables are the only state and every pass overwrites
 // them; nothing is cleared between calls.
+//
+// Ranged sort (a device range [kmin, kmax] of the keys other than `discard`):
+// every pass orders by digits of rel(k) = k - kmin (discard: kmax - kmin + 1,
+// after every real key), 7 bits a pass, and a pass whose digits lie above
+// rel's top bit returns at once, so a sort over a narrow range of a wide key
+// space costs the passes the range needs.  The passes that ran decide which
+// buffer holds the result (sorted_keys); the host never waits for the range.
 #include <algorithm>
 #include <utility>
 
@@ -35,10 +42,34 @@ constexpr int SI = SPF_SORT_ITEMS;
 constexpr int TILE = ST * SI;    // pairs per tile
 constexpr int NW = ST / 64;      // wavefronts per tile workgroup
 
+// the key map of one pass: digits of rel(k) from `shift`; active: the pass runs
+struct KeyMap {
+  uint32_t kmin, discard_rel, discard;
+  bool ranged, active;
+  __device__ KeyMap(const uint64_t *range, uint32_t discard_key, int shift) {
+    ranged = range != nullptr;
+    discard = discard_key;
+    kmin = 0;
+    discard_rel = 0;
+    active = true;
+    if (ranged) {
+      const uint64_t lo = range[0], hi = range[1];
+      const uint32_t span = hi >= lo ? (uint32_t)(hi - lo) + 1u : 0u;   // rel of discard
+      kmin = hi >= lo ? (uint32_t)lo : 0u;
+      discard_rel = span;
+      const int nbits = span ? 32 - __builtin_clz(span) : 0;
+      active = shift < nbits;
+    }
+  }
+  __device__ uint32_t rel(uint32_t k) const { return !ranged ? k : (k == discard ? discard_rel : k - kmin); }
+};
+
 template <int RB>
 __global__ __launch_bounds__(ST) void sort_count_kernel(const uint32_t *keys, int64_t n, int shift, uint32_t *cnt,
-                                                        int64_t ntiles) {
+                                                        int64_t ntiles, const uint64_t *range, uint32_t discard) {
   constexpr int R = 1 << RB;
+  const KeyMap km(range, discard, shift);
+  if (!km.active) return;
   __shared__ uint32_t h[R];
   for (int d = threadIdx.x; d < R; d += ST) h[d] = 0;
   __syncthreads();
@@ -48,7 +79,7 @@ __global__ __launch_bounds__(ST) void sort_count_kernel(const uint32_t *keys, in
 #pragma unroll
   for (int i = 0; i < SI; i++) {
     const int64_t idx = base + i * ST + threadIdx.x;
-    k[i] = idx < n ? keys[idx] : 0u;
+    k[i] = idx < n ? km.rel(keys[idx]) : 0u;
   }
   // After a pass equal digits come in runs, and LDS atomics on one address
   // serialise: the first lane of each run of equal digits adds the run's
@@ -98,7 +129,9 @@ __device__ inline void block_excl_scan2(uint32_t a, uint32_t b, uint32_t &ea, ui
 // one workgroup per digit d: pre[d][t] = sum of cnt[d][t'] over t' < t and
 // tot[d] = the sum of the row; thread k scans a contiguous chunk of tiles
 __global__ __launch_bounds__(ST) void sort_scan_kernel(const uint32_t *cnt, uint32_t *pre, uint32_t *tot,
-                                                       int64_t ntiles) {
+                                                       int64_t ntiles, int shift, const uint64_t *range,
+                                                       uint32_t discard) {
+  if (!KeyMap(range, discard, shift).active) return;
   __shared__ uint32_t ws[2][NW];
   const uint32_t *row = cnt + (int64_t)blockIdx.x * ntiles;
   uint32_t *out = pre + (int64_t)blockIdx.x * ntiles;
@@ -122,8 +155,11 @@ __global__ __launch_bounds__(ST) void sort_scan_kernel(const uint32_t *cnt, uint
 template <int RB>
 __global__ __launch_bounds__(ST) void sort_scatter_kernel(const uint32_t *kin, const double *vin, uint32_t *kout,
                                                           double *vout, int64_t n, int shift, const uint32_t *pre,
-                                                          const uint32_t *tot, int64_t ntiles) {
+                                                          const uint32_t *tot, int64_t ntiles, const uint64_t *range,
+                                                          uint32_t discard) {
   constexpr int R = 1 << RB;
+  const KeyMap km(range, discard, shift);
+  if (!km.active) return;
   static_assert(R <= ST, "one thread per digit");
   __shared__ uint32_t off[NW][R];   // per-wavefront digit counts, then each wavefront's start in the tile
   __shared__ uint32_t gbase[R];     // output position of the digit's run minus its tile-local start
@@ -146,7 +182,7 @@ __global__ __launch_bounds__(ST) void sort_scatter_kernel(const uint32_t *kin, c
 #pragma unroll
   for (int j = 0; j < SI; j++) {
     const int p = (w * SI + j) * 64 + lane;
-    const uint32_t d = (key[j] >> shift) & (R - 1);
+    const uint32_t d = (km.rel(key[j]) >> shift) & (R - 1);
     unsigned long long peers = __ballot(p < nvalid);
 #pragma unroll
     for (int b = 0; b < RB; b++) {
@@ -189,7 +225,7 @@ __global__ __launch_bounds__(ST) void sort_scatter_kernel(const uint32_t *kin, c
   for (int j = 0; j < SI; j++) {
     const int p = (w * SI + j) * 64 + lane;
     if (p < nvalid) {
-      const uint32_t pos = off[w][(key[j] >> shift) & (R - 1)] + rank[j];
+      const uint32_t pos = off[w][(km.rel(key[j]) >> shift) & (R - 1)] + rank[j];
       skey[pos] = key[j];
       sval[pos] = val[j];
     }
@@ -200,7 +236,7 @@ __global__ __launch_bounds__(ST) void sort_scatter_kernel(const uint32_t *kin, c
     const int p = i * ST + threadIdx.x;
     if (p < nvalid) {
       const uint32_t k = skey[p];
-      const uint32_t dst = gbase[(k >> shift) & (R - 1)] + (uint32_t)p;
+      const uint32_t dst = gbase[(km.rel(k) >> shift) & (R - 1)] + (uint32_t)p;
       if (dst < (uint32_t)n) {   // always true for consistent tables; keeps stores inside the output
         kout[dst] = k;
         vout[dst] = sval[p];
@@ -211,11 +247,12 @@ __global__ __launch_bounds__(ST) void sort_scatter_kernel(const uint32_t *kin, c
 
 template <int RB>
 hipError_t sort_pass(const uint32_t *ki, const double *vi, uint32_t *ko, double *vo, int64_t n, int shift,
-                     uint32_t *cnt, uint32_t *pre, uint32_t *tot, int64_t nt, hipStream_t s) {
-  hipLaunchKernelGGL(sort_count_kernel<RB>, dim3((unsigned)nt), dim3(ST), 0, s, ki, n, shift, cnt, nt);
-  hipLaunchKernelGGL(sort_scan_kernel, dim3(1u << RB), dim3(ST), 0, s, cnt, pre, tot, nt);
+                     uint32_t *cnt, uint32_t *pre, uint32_t *tot, int64_t nt, const uint64_t *range, uint32_t discard,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(sort_count_kernel<RB>, dim3((unsigned)nt), dim3(ST), 0, s, ki, n, shift, cnt, nt, range, discard);
+  hipLaunchKernelGGL(sort_scan_kernel, dim3(1u << RB), dim3(ST), 0, s, cnt, pre, tot, nt, shift, range, discard);
   hipLaunchKernelGGL(sort_scatter_kernel<RB>, dim3((unsigned)nt), dim3(ST), 0, s, ki, vi, ko, vo, n, shift, pre,
-                     tot, nt);
+                     tot, nt, range, discard);
   return hipGetLastError();
 }
 
@@ -242,14 +279,33 @@ hipError_t radix_sort_pairs(uint32_t *k0, double *v0, uint32_t *k1, double *v1, 
   for (int p = 0; p < passes; p++) {
     // a digit mask wider than rb overlaps the next digit, which the next
     // (stable) pass orders anyway
-    const hipError_t e = rb <= 7 ? sort_pass<7>(ki, vi, ko, vo, n, p * rb, cnt, pre, tot, nt, s)
-                                 : sort_pass<8>(ki, vi, ko, vo, n, p * rb, cnt, pre, tot, nt, s);
+    const hipError_t e = rb <= 7 ? sort_pass<7>(ki, vi, ko, vo, n, p * rb, cnt, pre, tot, nt, nullptr, 0u, s)
+                                 : sort_pass<8>(ki, vi, ko, vo, n, p * rb, cnt, pre, tot, nt, nullptr, 0u, s);
     if (e != hipSuccess) return e;
     std::swap(ki, ko);
     std::swap(vi, vo);
   }
   *kres = ki;
   *vres = vi;
+  return hipSuccess;
+}
+
+hipError_t radix_sort_pairs_ranged(uint32_t *k0, double *v0, uint32_t *k1, double *v1, int64_t n, int key_bits,
+                                   const uint64_t *range, uint32_t discard, void *tmp, size_t tmp_bytes,
+                                   hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (n >= (1ll << 31) || key_bits < 1 || key_bits > 32 || tmp_bytes < radix_sort_tmp_bytes(n))
+    return hipErrorInvalidValue;
+  const int64_t nt = (n + TILE - 1) / TILE;
+  uint32_t *cnt = static_cast<uint32_t *>(tmp), *pre = cnt + 256 * nt, *tot = pre + 256 * nt;
+  uint32_t *ki = k0, *ko = k1;
+  double *vi = v0, *vo = v1;
+  for (int p = 0; p < sorted_passes_max(key_bits); p++) {
+    const hipError_t e = sort_pass<7>(ki, vi, ko, vo, n, 7 * p, cnt, pre, tot, nt, range, discard, s);
+    if (e != hipSuccess) return e;
+    std::swap(ki, ko);
+    std::swap(vi, vo);
+  }
   return hipSuccess;
 }
 
