@@ -49,10 +49,12 @@ struct SorCfg {
   static constexpr int TRW = RS * SW;       // tile rows
   static constexpr int OR = TRW - 2 * SK;   // owned rows per tile
   static constexpr int NT = SW * 64;        // workgroup size
+  static constexpr int SWL = SW * 32;       // residual slots of a workgroup per step (lane pairs)
   static_assert(OZ % 2 == 0 && OR % 2 == 0 && SK % 2 == 0 && RS % 2 == 0,
                 "tile origins and lane rows must be even so each row's colour column is fixed");
   static_assert(OR >= SK && OZ >= SK, "a halo must come from the adjacent tiles only");
   static_assert(2 * RS <= 16, "one mask bit per point, two masks in a word");
+  static_assert(SK * 32 <= NT, "the block sums take 32 threads per half-sweep");
   // mask bits of the lane's even (column 0) and odd (column 1) points
   static constexpr unsigned COL0 = 0x55555555u & ((1u << (2 * RS)) - 1u);
   static constexpr unsigned COL1 = 0xAAAAAAAAu & ((1u << (2 * RS)) - 1u);
@@ -201,11 +203,14 @@ template <class G>
 struct SorLane {
   double P[G::RS][2], D[G::RS][2];
   double cr[G::RS], dr[G::RS];   // row coefficients c, d of the lane's rows (uniform over the wavefront)
-  unsigned eacc;     // largest div_fast_exp of an updated point's x in the batch
-  unsigned upd;      // in the array interior and not metal (src/poisson.c:58-120)
-  unsigned own;      // in the tile's owned region
-  int zdepth[2];     // distance of each column from the nearest shrinking z edge
-  bool r_mirror[2];  // column is iz == nz-1: its right neighbour mirrors the left one
+  unsigned eacc;      // largest div_fast_exp of an updated point's x in the batch
+  unsigned upd;       // in the array interior and not metal (src/poisson.c:58-120)
+  unsigned own;       // in the tile's owned region
+  int zdepth[2];      // distance of each column from the nearest shrinking z edge
+  bool r_mirror[2];   // column is iz == nz-1: its right neighbour mirrors the left one
+  uint64_t updm[G::RS][2];   // the update mask as wavefront masks (bit = lane), uniform
+  uint64_t colown[2]; // lanes whose column 0 / 1 lies in the owned columns
+  unsigned ownrows;   // bit i: lane row i lies in the owned rows (uniform)
 };
 
 // Column pair of a lane: the lane's two columns iz_e, iz_e+1 (iz_e even) are
@@ -260,9 +265,10 @@ __device__ inline void sor_lane_masks(const SorCommon &a, const SorTile<G> &t, i
   }
 }
 
-// right-hand side, masks (one stored word per lane) and the row coefficients
-// of the wavefront's rows, kept in registers for the whole batch; without a
-// right-hand side the loads read src (same layout) and are discarded
+// right-hand side, masks (one stored word per lane, and as wavefront
+// masks), ownership and the row coefficients of the wavefront's rows, kept in
+// registers for the whole batch; without a right-hand side the loads read src
+// (same layout) and are discarded
 template <class G>
 __device__ inline void sor_setup(const SorCommon &a, const SorTile<G> &t, int tile, int w, int lane, SorLane<G> &ln,
                                  const double *src) {
@@ -285,37 +291,54 @@ __device__ inline void sor_setup(const SorCommon &a, const SorTile<G> &t, int ti
   ln.upd = m & 0xffffu;
   ln.own = m >> 16;
 #pragma unroll
-  for (int col = 0; col < 2; col++) {
-    const int lz = 2 * lane + col;
-    ln.zdepth[col] = min(t.sz_lo ? lz : G::SK + 1, t.sz_hi ? t.LZ - 1 - lz : G::SK + 1);
-    ln.r_mirror[col] = iz_e + col == a.nz - 1;
-  }
-#pragma unroll
   for (int i = 0; i < G::RS; i++)
 #pragma unroll
     for (int col = 0; col < 2; col++)
       if (!((ln.upd >> (2 * i + col)) & 1u)) ln.D[i][col] = 0.0;
+#pragma unroll
+  for (int i = 0; i < G::RS; i++)
+#pragma unroll
+    for (int col = 0; col < 2; col++) ln.updm[i][col] = __ballot((ln.upd >> (2 * i + col)) & 1u);
+#pragma unroll
+  for (int col = 0; col < 2; col++) {
+    const int lz = 2 * lane + col;
+    ln.zdepth[col] = min(t.sz_lo ? lz : G::SK + 1, t.sz_hi ? t.LZ - 1 - lz : G::SK + 1);
+    ln.r_mirror[col] = iz_e + col == a.nz - 1;
+    ln.colown[col] = __ballot(iz_e + col >= t.z0 && iz_e + col < t.zo1);
+  }
+  ln.ownrows = 0;
+#pragma unroll
+  for (int i = 0; i < G::RS; i++) {
+    const int ir = t.rlo + w * G::RS + i;
+    if (ir >= t.r0 && ir < t.ro1) ln.ownrows |= 1u << i;
+  }
 }
 
-// One half-sweep of colour C on the registers of this lane; AX: this
-// wavefront holds the axis row (tile row 0 at ir = 0).  Row i of the lane is
-// tile row w*RS + i, whose parity is the parity of i (rlo is even); its
-// colour-C point is column (i + C) & 1 of the lane's pair.  Returns the sum of
-// |residual| over the lane's updated owned points.
-// EDGE: the tile touches the array's left (ledge) or right (redge) z end and
-// needs the mirrors; AXM: 0 the wavefront does not hold the axis row, 1 it
-// does, 2 decided by AX at run time.  EXACT: divide directly; otherwise
-// div_fast, with each updated point's exponent folded into ln.eacc.
+// One half-sweep of colour C on the registers of this lane.  Row i of the
+// lane is tile row w*RS + i, whose parity is the parity of i (rlo is even);
+// its colour-C point is column (i + C) & 1 of the lane's pair.  rv: the rows
+// valid at this step (uniform; the others are skipped); zv0, zv1: the lanes
+// whose column 0 / 1 is valid at this step.  A point is computed only on the
+// lanes that update it (the update mask and zv, as the execution mask), so
+// nothing is selected afterwards.  Returns the sum of |residual| over the
+// lane's updated owned points, rows in order.  EDGE: the tile touches the
+// array's left (ledge) or right (redge) z end and needs the mirrors; AXM: 0
+// the wavefront does not hold the axis row, 1 it does, 2 decided by AX at run
+// time.  EXACT: divide directly; otherwise div_fast, with each updated point's
+// exponent folded into ln.eacc.
 template <class G, int C, bool EDGE, int AXM, bool EXACT>
-__device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsigned okb, const double2 up_row,
-                                        const double2 dn_row, int w, double omega, bool ledge, bool redge, bool AX) {
+__device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, int lo, int hi, uint64_t zv0, uint64_t zv1,
+                                        const double2 up_row, const double2 dn_row, double omega, bool ledge,
+                                        bool redge, bool AX) {
   const double A = a.a, E = a.e;
   const bool ax = AXM == 2 ? AX : AXM == 1;
   const bool lane0 = (threadIdx.x & 63) == 0;
-  double q[G::RS], x[G::RS], res[G::RS];
+  double acc = 0;
 #pragma unroll
   for (int i = 0; i < G::RS; i++) {
+    if (i < lo || i >= hi) continue;
     const int col = (i + C) & 1;
+    // the other-colour neighbours in the row; the shifts run on every lane
     double L, R;
     if (col == 0) {
       R = ln.P[i][1];
@@ -329,6 +352,7 @@ __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsi
       R = from_right(L);   // lane 63's column 1 is never updated except as iz == nz-1, mirrored here
       if (EDGE && redge) R = ln.r_mirror[1] ? L : R;
     }
+    if (!__builtin_amdgcn_inverse_ballot_w64(ln.updm[i][col] & (col ? zv1 : zv0))) continue;
     const double zs = R + L;
     const double up = i < G::RS - 1 ? ln.P[i + 1 < G::RS ? i + 1 : i][col] : (col ? up_row.y : up_row.x);
     const double c = ln.cr[i];
@@ -343,24 +367,16 @@ __device__ inline double sor_half_sweep(const SorCommon &a, SorLane<G> &ln, unsi
       y = ax ? a.y_axis : a.y_e;
       residual = A * zs + cu * up + dn + diag * ln.P[i][col] - ln.D[i][col];
     } else {
-      const double down = i > 0 ? ln.P[i > 0 ? i - 1 : 0][col] : (col ? dn_row.y : dn_row.x);
+      const double down = ln.P[i > 0 ? i - 1 : 0][col];
       diag = E;
       y = a.y_e;
       residual = A * zs + c * up + ln.dr[i] * down + E * ln.P[i][col] - ln.D[i][col];
     }
-    res[i] = residual;
-    x[i] = omega * residual;
-    q[i] = EXACT ? x[i] / diag : div_fast(x[i], diag, y);
-  }
-  double acc = 0;
-#pragma unroll
-  for (int i = 0; i < G::RS; i++) {
-    const int col = (i + C) & 1;
-    const bool ok = (okb >> (2 * i + col)) & 1;
-    const double nv = ln.P[i][col] - q[i];
-    ln.P[i][col] = ok ? nv : ln.P[i][col];
-    if (!EXACT) ln.eacc = max(ln.eacc, ok ? div_fast_exp(x[i]) : 0u);
-    acc += (ok && ((ln.own >> (2 * i + col)) & 1)) ? fabs(res[i]) : 0.0;
+    const double x = omega * residual;
+    const double q = EXACT ? x / diag : div_fast(x, diag, y);
+    ln.P[i][col] = ln.P[i][col] - q;
+    if (!EXACT) ln.eacc = max(ln.eacc, div_fast_exp(x));
+    if (((ln.ownrows >> i) & 1u) && __builtin_amdgcn_inverse_ballot_w64(ln.colown[col])) acc += fabs(residual);
   }
   return acc;
 }
@@ -377,77 +393,81 @@ __device__ inline double sor_step(const SorCommon &a, const SorTile<G> &t, SorLa
   __syncthreads();
   const double2 up_row = w + 1 < G::SW ? xrow[par][w + 1][0][lane] : make_double2(0.0, 0.0);
   const double2 dn_row = w > 0 ? xrow[par][w - 1][1][lane] : make_double2(0.0, 0.0);
-  // points valid at this step: column depth and the wavefront's share of the valid rows
+  // points valid at this step: the wavefront's share of the valid rows and
+  // the column depth
   const int vr0 = t.sr_lo ? j + 1 : 0, vr1 = t.LR - (t.sr_hi ? j + 1 : 0);
   const int lo = min(max(vr0 - w * G::RS, 0), G::RS), hi = min(max(vr1 - w * G::RS, 0), G::RS);
-  const unsigned rmask = hi > lo ? (((1u << (2 * hi)) - 1u) & ~((1u << (2 * lo)) - 1u)) : 0u;
-  const unsigned okb = ln.upd & rmask & ((ln.zdepth[0] > j ? G::COL0 : 0u) | (ln.zdepth[1] > j ? G::COL1 : 0u));
-  return sor_half_sweep<G, C, EDGE, AXM, EXACT>(a, ln, okb, up_row, dn_row, w, omega, t.zlo == 0, t.redge,
-                                                axis_wave);
+  return sor_half_sweep<G, C, EDGE, AXM, EXACT>(a, ln, lo, hi, __ballot(ln.zdepth[0] > j), __ballot(ln.zdepth[1] > j),
+                                                up_row, dn_row, omega, t.zlo == 0, t.redge, axis_wave);
+}
+
+// The wavefront's residual sum of step j into wsum[j]: lanes l and l^1 add
+// their values (DPP quad_perm [1,0,3,2]) and the even lane stores the pair's
+// sum, 32 slots per wavefront; sor_block_sums adds the slots once per batch.
+// (A butterfly over the 64 lanes after every step kept each wavefront
+// waiting on six dependent LDS round trips per half-sweep.)
+template <class G>
+__device__ inline void sor_wave_sum(double v, int j, int w, int lane, double (*wsum)[G::SWL]) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0xB1, 0xF, 0xF, false);
+  v += __hiloint2double(hi, lo);
+  if ((lane & 1) == 0) wsum[j][w * 32 + (lane >> 1)] = v;
 }
 
 // Steps j = 0 .. k-1 of an even n0 in colour pairs (0, 1): one copy of the
 // pair loop per tile kind, so interior tiles carry no mirror selects and
-// wavefronts off the axis no axis terms.  lres[j][w] gets the wavefront's
-// owned residual sum of step j.
-// the wavefront's total of v (butterfly over the 64 lanes) into wsum[j][w],
-// right after step j: keeping every lane's sums of the batch for one
-// reduction at its end took 48 KB of LDS per workgroup, which with the
-// register budget allowed one workgroup per CU
-template <class G>
-__device__ inline void sor_wave_sum(double v, int j, int w, int lane, double (*wsum)[G::SW]) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  if (lane == 0) wsum[j][w] = v;
-}
-
+// wavefronts off the axis no axis terms.
 template <class G, bool EDGE, int AXM>
 __device__ inline void sor_pairs(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane, int k,
-                                 const double *om, double2 (*xrow)[G::SW][2][64], double (*lres)[G::SW]) {
+                                 const double *om, double2 (*xrow)[G::SW][2][64], double (*lres)[G::SWL],
+                                 bool axis_wave) {
 #pragma unroll 1
   for (int j = 0; j < k; j += 2) {
-    sor_wave_sum<G>(sor_step<G, 0, EDGE, AXM>(a, t, ln, w, lane, om[j], j, xrow, AXM == 1), j, w, lane, lres);
+    sor_wave_sum<G>(sor_step<G, 0, EDGE, AXM>(a, t, ln, w, lane, om[j], j, xrow, axis_wave), j, w, lane, lres);
     if (j + 1 < k)
-      sor_wave_sum<G>(sor_step<G, 1, EDGE, AXM>(a, t, ln, w, lane, om[j + 1], j + 1, xrow, AXM == 1), j + 1, w, lane,
-                      lres);
+      sor_wave_sum<G>(sor_step<G, 1, EDGE, AXM>(a, t, ln, w, lane, om[j + 1], j + 1, xrow, axis_wave), j + 1, w,
+                      lane, lres);
   }
 }
 
+// steps one by one with run-time colours: an odd n0 (only from
+// spf_poisson_sweeps) and the exact division
+template <class G, bool EXACT>
+__device__ inline void sor_singles(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane,
+                                   int64_t n0, int k, const double *om, double2 (*xrow)[G::SW][2][64],
+                                   double (*lres)[G::SWL], bool axis_wave) {
+#pragma unroll 1
+  for (int j = 0; j < k; j++)
+    sor_wave_sum<G>(((n0 + j) & 1)
+                        ? sor_step<G, 1, true, 2, EXACT>(a, t, ln, w, lane, om[j], j, xrow, axis_wave)
+                        : sor_step<G, 0, true, 2, EXACT>(a, t, ln, w, lane, om[j], j, xrow, axis_wave),
+                    j, w, lane, lres);
+}
+
 // k <= SK half-sweeps n0 .. n0+k-1 with relaxation factors om[0 .. k-1] (LDS,
-// read after each step's barrier); lres[j][w] gets the wavefront's owned
-// residual sum of step j.  An odd n0 (only from spf_poisson_sweeps) and the
-// exact division run the steps one by one with run-time colours.
+// read after each step's barrier); lres[j] gets the workgroup's owned
+// residual slots of step j.  The tile kind is uniform over the workgroup,
+// the axis flag over the wavefront; every copy runs one barrier per step.
 template <class G>
 __device__ inline void sor_sweeps(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane,
                                   int64_t n0, int k, const double *om, double2 (*xrow)[G::SW][2][64],
-                                  double (*lres)[G::SW], bool exact) {
+                                  double (*lres)[G::SWL], bool exact) {
   const bool axis_wave = t.rlo == 0 && w == 0;
   const bool edge = t.zlo == 0 || t.redge;
   if (exact) {
-#pragma unroll 1
-    for (int j = 0; j < k; j++)
-      sor_wave_sum<G>(((n0 + j) & 1) ? sor_step<G, 1, true, 2, true>(a, t, ln, w, lane, om[j], j, xrow, axis_wave)
-                                     : sor_step<G, 0, true, 2, true>(a, t, ln, w, lane, om[j], j, xrow, axis_wave),
-                      j, w, lane, lres);
-  } else if ((n0 & 1) == 0) {
-    // the tile kind is uniform over the workgroup, the axis flag over the wavefront
-    if (edge) {
-      if (axis_wave)
-        sor_pairs<G, true, 1>(a, t, ln, w, lane, k, om, xrow, lres);
-      else
-        sor_pairs<G, true, 0>(a, t, ln, w, lane, k, om, xrow, lres);
-    } else {
-      if (axis_wave)
-        sor_pairs<G, false, 1>(a, t, ln, w, lane, k, om, xrow, lres);
-      else
-        sor_pairs<G, false, 0>(a, t, ln, w, lane, k, om, xrow, lres);
-    }
+    sor_singles<G, true>(a, t, ln, w, lane, n0, k, om, xrow, lres, axis_wave);
+  } else if (n0 & 1) {
+    sor_singles<G, false>(a, t, ln, w, lane, n0, k, om, xrow, lres, axis_wave);
+  } else if (edge) {
+    if (axis_wave)
+      sor_pairs<G, true, 1>(a, t, ln, w, lane, k, om, xrow, lres, axis_wave);
+    else
+      sor_pairs<G, true, 0>(a, t, ln, w, lane, k, om, xrow, lres, axis_wave);
   } else {
-#pragma unroll 1
-    for (int j = 0; j < k; j++)
-      sor_wave_sum<G>(((n0 + j) & 1) ? sor_step<G, 1, true, 2>(a, t, ln, w, lane, om[j], j, xrow, axis_wave)
-                                     : sor_step<G, 0, true, 2>(a, t, ln, w, lane, om[j], j, xrow, axis_wave),
-                      j, w, lane, lres);
+    if (axis_wave)
+      sor_pairs<G, false, 1>(a, t, ln, w, lane, k, om, xrow, lres, axis_wave);
+    else
+      sor_pairs<G, false, 0>(a, t, ln, w, lane, k, om, xrow, lres, axis_wave);
   }
 }
 
@@ -468,13 +488,14 @@ __device__ inline bool sor_lane_exact(const SorLane<G> &ln) {
 // the tile's owned residual sum of each step j < k into out[j * stride]:
 // butterflies within each wavefront, then the wavefronts in order
 template <class G>
-__device__ inline void sor_block_sums(int k, double (*wsum)[G::SW], double *out, int64_t stride) {
+__device__ inline void sor_block_sums(int k, double (*wsum)[G::SWL], double *out, int64_t stride) {
   __syncthreads();
-  if (threadIdx.x < k) {
-    double tot = 0;
-    for (int q = 0; q < G::SW; q++) tot += wsum[threadIdx.x][q];
-    out[threadIdx.x * stride] = tot;
-  }
+  const int j = threadIdx.x >> 5, l = threadIdx.x & 31;
+  double tot = 0;
+  if (j < k)
+    for (int q = 0; q < G::SW; q++) tot += wsum[j][q * 32 + l];
+  for (int off = 16; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 32);
+  if (j < k && l == 0) out[j * stride] = tot;
 }
 
 // owned points to dst: a 16-byte store where both columns are owned
@@ -620,7 +641,7 @@ __device__ inline void sor_batch_block(const SorArgs &a, int bid, const double *
     return;
   }
   __shared__ double2 xrow[2][G::SW][2][64];   // [parity][wave][first,last row][lane]
-  __shared__ double wsum[G::SK][G::SW];
+  __shared__ double wsum[G::SK][G::SWL];
   // the done flag, read once for the workgroup (the check workgroup of the
   // same launch may set it at any moment, and the wavefronts must agree), and
   // the batch's relaxation factors, published through LDS with the tile loads
