@@ -47,7 +47,7 @@ TRAFFIC = {
 TIMER_EVERY = 2
 
 
-def build_run(ctx, workload, n_particles):
+def build_run(ctx, workload, n_particles, cell_sort=0):
     from spiffe_amd import workloads as W
     from spiffe_amd.device import DeviceRun
     if workload == "hires":
@@ -59,7 +59,7 @@ def build_run(ctx, workload, n_particles):
         bunch = W.gun_bunch(g, n_particles)
         pc = W.GUN_POISSON
     f = W.gun_flags(g)
-    run = DeviceRun(ctx, g, f, max_particles=n_particles, screens=W.GUN_SCREENS, compat=False)
+    run = DeviceRun(ctx, g, f, max_particles=n_particles, screens=W.GUN_SCREENS, compat=False, cell_sort=cell_sort)
     for k, v in W.gun_fields(g).items():
         run.upload(k, v)
     run.define_solenoid(**W.GUN_SOLENOID)
@@ -236,6 +236,10 @@ def main():
     ap.add_argument("--scan-runs", type=int, default=64)
     ap.add_argument("--scan-steps", type=int, default=40)
     ap.add_argument("--particles", type=int, default=0, help="0: the workload's own count")
+    ap.add_argument("--cell-sort", type=int, default=-1,
+                    help="re-sort particles by cell every N steps (deposition then sums in a fixed order of its "
+                         "own); 0 keeps the reference's particle order and sums; default: 20 for hires (its "
+                         "configuration), 0 for gun")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device copy (5 times), the reference for HBM byte counters")
@@ -265,8 +269,9 @@ def main():
     from spiffe_amd import workloads as W
     from spiffe_amd.device import Context
     n_particles = args.particles or (W.HIRES_PARTICLES if args.workload == "hires" else 1_000_000)
+    cell_sort = args.cell_sort if args.cell_sort >= 0 else (20 if args.workload == "hires" else 0)
     ctx = Context(0)
-    g, f, bunch, pc, run = build_run(ctx, args.workload, n_particles)
+    g, f, bunch, pc, run = build_run(ctx, args.workload, n_particles, cell_sort)
     cells = (g.nz - 1) * (g.nr - 1)
 
     for _ in range(args.warmup):
@@ -303,11 +308,12 @@ def main():
     t_fdtd, t_push, t_dep, t_sor = avg("fdtd"), avg("push"), avg("deposit"), avg("sor_batch")
     phase = {"fdtd_ms": t_fdtd, "push_ms": t_push, "deposit_ms": t_dep,
              "sor_batch_ms": t_sor, "sor_batches": tm["sor_batch"][2], "sor_batches_timed": tm["sor_batch"][1],
+             "cell_sort_ms": avg("cell_sort"), "cell_sorts": tm["cell_sort"][2],
              "poisson_ms_per_correction": avg("poisson"), "poisson_corrections": corrections,
              "poisson_ms_per_step": per_step("poisson")}
     # device time per step spent in each kernel or phase
     share = {"fdtd_kernel": per_step("fdtd"), "push phase": per_step("push"), "deposit phase": per_step("deposit"),
-             "sor_batch_kernel": per_step("sor_batch")}
+             "sor_batch_kernel": per_step("sor_batch"), "cell sort": per_step("cell_sort")}
     points = (g.nz + 1) * (g.nr + 1)
     npk = 0.5 * (np0 + np1)
     algo = {  # algorithmic bytes per launch and measured ms per launch
@@ -360,7 +366,11 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": wl, "grid_lines": [g.nz, g.nr], "cells": cells, "particles": n_particles,
-                   "dt": g.dt, "poisson": dict(pc), "cell_sort": "none: deposition orders contributions per step"},
+                   "dt": g.dt, "poisson": dict(pc),
+                   "cell_sort": ("particles re-sorted by half-cell every %d steps; deposition sums per cell in a fixed "
+                                 "order of its own (wavefront tree sums folded in storage order)" % cell_sort
+                                 if cell_sort else
+                                 "none: particles in the reference's order, deposition in the reference's order")},
         "cell_updates_per_s": cells * args.steps / wall,
         "phase_ms": phase, "sor_half_sweeps_per_correction": iters_per_correction,
         "poisson_last_converged": converged,

@@ -98,6 +98,10 @@ typedef struct {
   int32_t compat;                 /* 1: reproduce compaction and screen-momentum defects of the reference */
   int64_t screen_row_capacity;    /* device ring size for screen rows (0: default) */
   int64_t lost_row_capacity;      /* device ring size for lost rows (0: default) */
+  int32_t cell_sort;              /* 0: particles stay in the reference's index order and every sum is
+                                     the reference's; N > 0: particles are re-sorted by cell every N steps
+                                     and deposition sums in a fixed order of its own (needs compat = 0) */
+  int32_t reserved;
 } spf_run_desc;
 
 /* Particle SoA on the host, src/spiffe.h:201-221 (status is 32-bit here). */
@@ -175,6 +179,12 @@ int spf_get_step(spf_ctx *ctx, int32_t run, int64_t *time_step, double *time);
  * hands to the push as advance_particles' zForcesStart (src/integrate.c:186-188,
  * src/advance_particles.c:190-200); a run starts with -1 */
 int spf_set_z_forces_start(spf_ctx *ctx, int32_t run, double z_forces_start);
+
+/* No counterpart in the reference (which keeps particles in emission order):
+ * re-sorts a cell-sort run's particles by cell now (spf_step does it every
+ * cell_sort steps).  The particle order is the storage order from then on;
+ * particleID names each particle.  SPF_EINVAL when cell_sort is 0. */
+int spf_particles_sort(spf_ctx *ctx, int32_t run);
 
 /* FDTD leapfrog of TM and TE fields; replaces advance_field_solutions, src/integrate.c:277-541 */
 int spf_fields_advance(spf_ctx *ctx, int32_t run);
@@ -403,7 +413,8 @@ int spf_sdds_parameter(const char *path, int64_t page, const char *name, double 
 #define SPF_T_SOR_BATCH 5  /* SOR batch kernel launches (up to spf_sor_batch_half_sweeps()
                               of the run half-sweeps each), timed per group of back-to-back
                               launches between stopping-rule checks */
-#define SPF_T_N 6
+#define SPF_T_SORT 6       /* re-sorts of cell-sort runs' particles (spf_particles_sort and in steps) */
+#define SPF_T_N 7
 /* every > 0 starts timing one call in `every` of each phase and clears the
  * accumulators; 0 stops it */
 int spf_timing_enable(spf_ctx *ctx, int32_t every);

@@ -28,7 +28,7 @@ FIELDS = ["Ez", "Er", "Bphi", "Bphi2", "Ephi", "Br", "Br2", "Bz", "Bz2", "Jz", "
 FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 
 # phase timers
-TIMERS = ["step", "deposit", "fdtd", "push", "poisson", "sor_batch"]
+TIMERS = ["step", "deposit", "fdtd", "push", "poisson", "sor_batch", "cell_sort"]
 
 PARTICLE_DOUBLES = ["Q", "z", "r", "pz", "pr", "r_pphi", "vz", "vr", "vphi",
                     "r0", "z0", "t0", "rmax", "rmin", "zmax", "zmin"]
@@ -58,6 +58,8 @@ class RunDesc(ctypes.Structure):
         ("compat", ctypes.c_int32),
         ("screen_row_capacity", ctypes.c_int64),
         ("lost_row_capacity", ctypes.c_int64),
+        ("cell_sort", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -171,6 +173,7 @@ SIGNATURES = {
     "spf_particles_count": (ctypes.c_int, [_vp, _i32, _pi64, _pi64]),
     "spf_particles_download": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(Particles), _i64, _pi64]),
     "spf_particles_push": (ctypes.c_int, [_vp, _i32, _d]),
+    "spf_particles_sort": (ctypes.c_int, [_vp, _i32]),
     "spf_events_drain": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(ScreenRow), _i64, _pi64,
                                         ctypes.POINTER(LostRow), _i64, _pi64]),
     "spf_poisson_setup": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(PoissonParams)]),

@@ -41,7 +41,9 @@ void free_run(Run *R) {
   F(R->EzImp); F(R->ErImp); F(R->PsiImp); F(R->BzImp); F(R->BrImp); F(R->flags); F(R->cut0); F(R->cut1);
   F(R->cp); F(R->cm); F(R->bzf); F(R->sor_c); F(R->sor_d); F(R->volume);
   for (int k = 0; k < 16; k++) F(R->P[k]);
-  F(R->pid); F(R->gen); F(R->status); F(R->counters); F(R->krange); F(R->evt); F(R->evflag);
+  F(R->pid); F(R->gen); F(R->status);
+  for (int k = 0; k < 16; k++) F(R->P2[k]);
+  F(R->pid2); F(R->gen2); F(R->status2); F(R->wg_count); F(R->wg_off); F(R->counters); F(R->krange); F(R->evt); F(R->evflag);
   for (int k = 0; k < 7; k++) F(R->ev[k]);
   F(R->last_old); F(R->evlist); F(R->move_src); F(R->move_dst); F(R->srows); F(R->lrows);
   F(R->keys); F(R->skeys); F(R->vals); F(R->cz); F(R->scz); F(R->long_list); F(R->pcount); F(R->gflag); F(R->sort_tmp);
@@ -253,7 +255,9 @@ int64_t spf_run_footprint(const spf_run_desc *d) {
   int64_t event_bytes = n * (4 + 1 + 7 * 8 + 4 + 8);                     // staging, lists
   int64_t sort_bytes = 3 * n * (4 + 4 + 8 + 8 + 8 + 1 + 1);  // keys, contributions (both sorted), flags, sort tables
   int64_t ring = default_srow_cap(d) * (int64_t)sizeof(spf_screen_row) + default_lrow_cap(d) * (int64_t)sizeof(spf_lost_row);
-  return field_bytes + part_bytes + event_bytes + sort_bytes + ring;
+  // cell-sort runs: a second particle set and the reduce deposition's counts
+  int64_t cellsort_bytes = d->cell_sort > 0 ? n * (16 * 8 + 4 + 2 + 4) + (n / 1024 + 2) * 12 : 0;
+  return field_bytes + part_bytes + event_bytes + sort_bytes + ring + cellsort_bytes;
 }
 
 int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
@@ -267,6 +271,9 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
     return fail(c, SPF_EUNSUPPORTED, "area-weighted (smeared) deposition is not implemented on the device");
   if (d->max_particles < 0 || d->max_particles > 0x20000000LL) return fail(c, SPF_EINVAL, "max_particles out of range");
   if ((int64_t)(d->nz + 1) * (d->nr + 1) >= (1LL << 29)) return fail(c, SPF_EINVAL, "grid too large");
+  if (d->cell_sort < 0) return fail(c, SPF_EINVAL, "cell_sort must be >= 0");
+  if (d->cell_sort > 0 && d->compat)
+    return fail(c, SPF_EINVAL, "cell_sort needs compat = 0 (the reference's compaction defects depend on index order)");
   HIPCALL(c, hipSetDevice(c->device));
   Run *R = new (std::nothrow) Run();
   if (!R) return fail(c, SPF_ENOMEM, "host allocation failed");
@@ -307,6 +314,12 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   const int64_t cap = R->cap > 0 ? R->cap : 1;
   for (int k = 0; k < 16; k++) ALLOC(R->P[k], cap);
   ALLOC(R->pid, cap); ALLOC(R->gen, cap); ALLOC(R->status, cap);
+  R->cell_sort = d->cell_sort;
+  if (R->cell_sort > 0) {
+    for (int k = 0; k < 16; k++) ALLOC(R->P2[k], cap);
+    ALLOC(R->pid2, cap); ALLOC(R->gen2, cap); ALLOC(R->status2, cap);
+    ALLOC(R->wg_count, reduce_groups(cap) + 1); ALLOC(R->wg_off, reduce_groups(cap) + 1);
+  }
   ALLOC(R->counters, 16);
   ALLOC(R->krange, 2 * 256 + 2);
   ALLOC(R->evt, cap); ALLOC(R->evflag, cap); ALLOC(R->pcount, cap / 256 + 2);
@@ -826,6 +839,15 @@ void spf::push_args(Run *R, double zForcesStart, PushArgs &a, RowsArgs &w) {
 }
 
 extern "C" {
+
+int spf_particles_sort(spf_ctx *c, int32_t run) {
+  Run *R = get_run(c, run);
+  if (!R) return fail(c, SPF_EINVAL, "bad run id");
+  if (R->cell_sort <= 0) return fail(c, SPF_EINVAL, "the run keeps the reference's particle order (cell_sort = 0)");
+  PhaseTimer pt(c, SPF_T_SORT);
+  HIPCALL(c, sort_particles(*R, c->stream));
+  return SPF_OK;
+}
 
 int spf_particles_push(spf_ctx *c, int32_t run, double zForcesStart) {
   Run *R = get_run(c, run);

@@ -569,6 +569,7 @@ int spf_step(spf_ctx *c, int32_t run, int32_t n_lines, const int32_t *line_dir, 
   R->time = R->time_step * R->d.dt + R->d.start_time;
   for (int32_t k = 0; k < (int32_t)R->cathodes.size(); k++)   // src/integrate.c:162-163
     if ((rc = cathode_emit_step(c, run, k, nullptr))) return rc;
+  if (R->cell_sort > 0 && R->time_step % R->cell_sort == 0 && (rc = spf_particles_sort(c, run))) return rc;
   if (R->d.modes & SPF_SPACE_CHARGE) {
     // reset and deposition (src/integrate.c:169-173) in one pass over the
     // keys: the key kernel clears J, the folds add into it
@@ -611,6 +612,7 @@ int spf_step_batch(spf_ctx *c, int32_t n_runs, const int32_t *run_ids) {
     R->time = R->time_step * R->d.dt + R->d.start_time;
     for (int32_t k = 0; k < (int32_t)R->cathodes.size(); k++)
       if ((rc = cathode_emit_step(c, run_ids[i], k, nullptr))) return rc;
+    if (R->cell_sort > 0 && R->time_step % R->cell_sort == 0 && (rc = spf_particles_sort(c, run_ids[i]))) return rc;
     if (R->d.modes & SPF_SPACE_CHARGE) {
       PhaseTimer dt(c, SPF_T_DEPOSIT);
       HIPCALL(c, deposit_currents(*R, s, true));

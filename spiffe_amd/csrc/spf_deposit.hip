@@ -53,6 +53,9 @@ struct KeyArgs {
   int64_t clear_n;
   int zero_terms;               // COMP_Q: apply the signed-zero effect of the zero-weight terms
   double *zero_target;          // the Q array those terms act on
+  int need_v[3];                // vz, vr, vphi are read (a component uses them)
+  double *pvals;                // reduce path: first component's value per particle (or null)
+  int32_t *wg_count;            // reduce path: partials per workgroup
 };
 
 // NGP value of the Jz/Jphi loops for cell row ir (0 <= ir <= nr-2), :92-100, :254-262
@@ -88,129 +91,357 @@ __global__ __launch_bounds__(kRangeSlots) void key_range_final_kernel(unsigned l
   }
 }
 
-__global__ __launch_bounds__(256) void key_kernel(KeyArgs a) {
-  int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip == 0) a.counters[9] = 0;
-  // the targets' reset (src/integrate.c:169 zeroes J before the deposition),
-  // done here to save a launch; the folds that add into them come later
-  for (int64_t q = ip; q < a.clear_n; q += (int64_t)gridDim.x * blockDim.x)
-    for (int c = 0; c < 3; c++)
-      if (a.clear[c]) a.clear[c][q] = 0.0;
-  // smallest and largest key this lane writes (every lane takes part in the
-  // wavefront's reduction below, so no lane returns early)
-  uint32_t kmin = 0xffffffffu, kmax = 0u;
-  const bool in = ip < a.n_ub;
-  int64_t np = in ? *a.np : 0;
-  bool live = in && ip < np && (a.status[ip] & SPF_PART_ACTIVE);
-  long long izh = -1, izA = -1, irh = -1, irA = -1;
-  double Qo = 0;
-  if (live) {
-    double x = (a.z[ip] - a.zmin) / a.dz;
-    double y = a.r[ip] / a.dr;
+// the columns of one particle slot the deposition reads, loaded together so
+// that no load waits for another (slots past n_ub load nothing)
+struct PartVals {
+  bool live;
+  double Q, z, r, vz, vr, vphi;
+};
+
+__device__ inline PartVals load_part(const KeyArgs &a, int64_t ip, int64_t np) {
+  PartVals p{false, 0, 0, 0, 0, 0, 0};
+  if (ip < a.n_ub) {
+    const uint32_t st = a.status[ip];
+    p.Q = a.Q[ip];
+    p.z = a.z[ip];
+    p.r = a.r[ip];
+    if (a.need_v[0]) p.vz = a.vz[ip];
+    if (a.need_v[1]) p.vr = a.vr[ip];
+    if (a.need_v[2]) p.vphi = a.vphi[ip];
+    p.live = ip < np && (st & SPF_PART_ACTIVE);
+  }
+  return p;
+}
+
+// cell indices of one particle that the components' keys are made from
+struct PartCells {
+  bool live;
+  long long izh, izA, irh, irA;   // x+0.5, x+1.0, y+0.5, y+1.0 truncated (-1: outside)
+  double Qo;
+};
+
+__device__ inline PartCells part_cells(const KeyArgs &a, const PartVals &pv) {
+  PartCells c{pv.live, -1, -1, -1, -1, 0.0};
+  if (pv.live) {
+    double x = (pv.z - a.zmin) / a.dz;
+    double y = pv.r / a.dr;
     double th = x + 0.5, t1 = x + 1.0, yh = y + 0.5, y1 = y + 1.0;
     // every valid cell index lies in [0, nz] x [0, nr]; outside these
     // bounds (and for NaN) the conversions are skipped
-    if (th > -1.0 && th < a.nz + 1.0) izh = (long long)th;
-    if (t1 > -1.0 && t1 < a.nz + 1.0) izA = (long long)t1;
-    if (yh > -1.0 && yh < a.nr + 1.0) irh = (long long)yh;
-    if (y1 > -1.0 && y1 < a.nr + 1.0) irA = (long long)y1;
-    Qo = a.Q[ip] * a.one_over_piDzDr2;
+    if (th > -1.0 && th < a.nz + 1.0) c.izh = (long long)th;
+    if (t1 > -1.0 && t1 < a.nz + 1.0) c.izA = (long long)t1;
+    if (yh > -1.0 && yh < a.nr + 1.0) c.irh = (long long)yh;
+    if (y1 > -1.0 && y1 < a.nr + 1.0) c.irA = (long long)y1;
+    c.Qo = pv.Q * a.one_over_piDzDr2;
   }
+  return c;
+}
+
+// the key (or `discard`) and value of component slot k of particle pv
+__device__ inline uint32_t part_contrib(const KeyArgs &a, const PartCells &pc, const PartVals &pv, int k, double &v) {
+  const int c = a.comp[k];
   const int nz = a.nz, nr = a.nr;
-  for (int k = 0; in && k < a.ncomp; k++) {
-    int c = a.comp[k];
-    long long iz = -1, ir = -1;
-    double v = 0;
-    if (live) {
-      if (c == COMP_JZ) {
-        // iz = x+1.0, ir = y+0.5; valid iz <= nz, ir <= nr; nonzero only for ir <= nr-2
-        if (izA >= 0 && izA <= nz && irh >= 0 && irh <= nr - 2) {
-          iz = izA;
-          ir = irh;
-          v = jm_centered(a.vz[ip] * Qo, irh);
-        }
-      } else if (c == COMP_JR) {
-        // iz = x+0.5, ir = y+1.0; valid ir <= nr-1; nonzero only for 1 <= ir <= nr-2
-        if (izh >= 0 && izh <= nz && irA >= 1 && irA <= nr - 2) {
+  long long iz = -1, ir = -1;
+  v = 0;
+  if (pc.live) {
+    const long long izh = pc.izh, izA = pc.izA, irh = pc.irh, irA = pc.irA;
+    const double Qo = pc.Qo;
+    if (c == COMP_JZ) {
+      // iz = x+1.0, ir = y+0.5; valid iz <= nz, ir <= nr; nonzero only for ir <= nr-2
+      if (izA >= 0 && izA <= nz && irh >= 0 && irh <= nr - 2) {
+        iz = izA;
+        ir = irh;
+        v = jm_centered(pv.vz * Qo, irh);
+      }
+    } else if (c == COMP_JR) {
+      // iz = x+0.5, ir = y+1.0; valid ir <= nr-1; nonzero only for 1 <= ir <= nr-2
+      if (izh >= 0 && izh <= nz && irA >= 1 && irA <= nr - 2) {
+        iz = izh;
+        ir = irA;
+        double QVG = pv.vr * Qo;
+        v = 1.0 * QVG / ((double)irA * (double)irA - (double)(irA - 1) * (double)(irA - 1));
+      }
+    } else if (c == COMP_JPHI) {
+      if (izh >= 0 && izh <= nz && irh >= 0 && irh <= nr - 2) {
+        iz = izh;
+        ir = irh;
+        v = jm_centered(pv.vphi * Qo, irh);
+      }
+    } else {  // COMP_Q: iz = x+0.5, ir = y+0.5 in [0,nz-1] x [0,nr-1], not metal or a Neumann edge
+      if (izh >= 0 && izh < nz && irh >= 0 && irh < nr) {
+        int16_t f = a.flags[fix(a.ldz, (int)izh, (int)irh)];
+        bool blocked = (f & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE)) || (izh == 0 && a.left_neu) ||
+                       (izh >= nz - 1 && a.right_neu) || (irh >= nr - 1 && a.upper_neu);
+        if (!blocked) {
           iz = izh;
-          ir = irA;
-          double QVG = a.vr[ip] * Qo;
-          v = 1.0 * QVG / ((double)irA * (double)irA - (double)(irA - 1) * (double)(irA - 1));
-        }
-      } else if (c == COMP_JPHI) {
-        if (izh >= 0 && izh <= nz && irh >= 0 && irh <= nr - 2) {
-          iz = izh;
           ir = irh;
-          v = jm_centered(a.vphi[ip] * Qo, irh);
+          v = -(pv.Q * 1.0 * 1.0);
         }
-      } else {  // COMP_Q: iz = x+0.5, ir = y+0.5 in [0,nz-1] x [0,nr-1], not metal or a Neumann edge
-        if (izh >= 0 && izh < nz && irh >= 0 && irh < nr) {
-          int16_t f = a.flags[fix(a.ldz, (int)izh, (int)irh)];
-          bool blocked = (f & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE)) || (izh == 0 && a.left_neu) ||
-                         (izh >= nz - 1 && a.right_neu) || (irh >= nr - 1 && a.upper_neu);
-          if (!blocked) {
-            iz = izh;
-            ir = irh;
-            v = -(a.Q[ip] * 1.0 * 1.0);
-          }
-          // The reference also subtracts Q*fb*fr, Q*fa*fl and Q*fa*fr from the
-          // cells (iz+1, ir), (iz, ir+1), (iz+1, ir+1) (src/poisson_correction.c:
-          // 222-236); without smearing these are Q*0, a zero with the sign of Q.
-          // Subtracting +0 changes no value; subtracting -0 (Q < 0) turns a -0
-          // cell into +0 and leaves every other value alone.  A cell can hold -0
-          // only while nothing but -0 has been added to a -0 start, so turning
-          // -0 into +0 before the ordered fold gives the reference's bits
-          // whatever the position of the zero terms in its sequence.
-          if (a.zero_terms && __builtin_signbit(a.Q[ip])) {
-            const long long zc[3][2] = {{izh + 1, irh}, {izh, irh + 1}, {izh + 1, irh + 1}};
-            const bool zb[3] = {(izh == 0 && a.left_neu) || (izh >= nz && a.right_neu) || (irh >= nr - 1 && a.upper_neu),
-                                (izh == 0 && a.left_neu) || (izh >= nz - 1 && a.right_neu) || (irh >= nr && a.upper_neu),
-                                (izh == 0 && a.left_neu) || (izh >= nz && a.right_neu) || (irh >= nr && a.upper_neu)};
+        // The reference also subtracts Q*fb*fr, Q*fa*fl and Q*fa*fr from the
+        // cells (iz+1, ir), (iz, ir+1), (iz+1, ir+1) (src/poisson_correction.c:
+        // 222-236); without smearing these are Q*0, a zero with the sign of Q.
+        // Subtracting +0 changes no value; subtracting -0 (Q < 0) turns a -0
+        // cell into +0 and leaves every other value alone.  A cell can hold -0
+        // only while nothing but -0 has been added to a -0 start, so turning
+        // -0 into +0 before the ordered fold gives the reference's bits
+        // whatever the position of the zero terms in its sequence.
+        if (a.zero_terms && __builtin_signbit(pv.Q)) {
+          const long long zc[3][2] = {{izh + 1, irh}, {izh, irh + 1}, {izh + 1, irh + 1}};
+          const bool zb[3] = {(izh == 0 && a.left_neu) || (izh >= nz && a.right_neu) || (irh >= nr - 1 && a.upper_neu),
+                              (izh == 0 && a.left_neu) || (izh >= nz - 1 && a.right_neu) || (irh >= nr && a.upper_neu),
+                              (izh == 0 && a.left_neu) || (izh >= nz && a.right_neu) || (irh >= nr && a.upper_neu)};
 #pragma unroll
-            for (int t = 0; t < 3; t++) {
-              const int64_t pc = fix(a.ldz, (int)zc[t][0], (int)zc[t][1]);
-              if (zb[t] || (a.flags[pc] & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE))) continue;
-              double *q = a.zero_target + pc;
-              if (__double_as_longlong(*q) == (long long)0x8000000000000000ull) *q = 0.0;
-            }
+          for (int t = 0; t < 3; t++) {
+            const int64_t pcell = fix(a.ldz, (int)zc[t][0], (int)zc[t][1]);
+            if (zb[t] || (a.flags[pcell] & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE))) continue;
+            double *q = a.zero_target + pcell;
+            if (__double_as_longlong(*q) == (long long)0x8000000000000000ull) *q = 0.0;
           }
         }
       }
     }
-    int64_t slot = (int64_t)k * a.n_ub + ip;
-    if (iz >= 0) {
-      const uint32_t cell = (uint32_t)(ir * (nz + 1) + iz);
-      const uint32_t key = (cell << 2) | (uint32_t)k;
-      a.keys[slot] = key;
-      a.vals[slot] = v;
-      kmin = min(kmin, key);
-      kmax = max(kmax, key);
-    } else {
-      a.keys[slot] = a.discard;
-      a.vals[slot] = 0.0;
-    }
   }
-  if (!a.krange) return;   // the sort covers the whole key space
+  if (iz < 0) return a.discard;
+  const uint32_t cell = (uint32_t)(ir * (nz + 1) + iz);
+  return (cell << 2) | (uint32_t)k;
+}
+
+// the targets' reset (src/integrate.c:169 zeroes J before the deposition),
+// done by the key kernels to save a launch; the folds that add into them come later
+__device__ inline void clear_targets(const KeyArgs &a, int64_t ip, int64_t stride) {
+  for (int64_t q = ip; q < a.clear_n; q += stride)
+    for (int c = 0; c < 3; c++)
+      if (a.clear[c]) a.clear[c][q] = 0.0;
+}
+
+// folds a workgroup's smallest and largest key into one of the range slots
+template <int NWAVES>
+__device__ inline void key_range_fold(unsigned long long *krange, uint32_t kmin, uint32_t kmax) {
   for (int off = 32; off > 0; off >>= 1) {
     kmin = min(kmin, (uint32_t)__shfl_xor((int)kmin, off, 64));
     kmax = max(kmax, (uint32_t)__shfl_xor((int)kmax, off, 64));
   }
-  __shared__ uint32_t wmin[4], wmax[4];
+  __shared__ uint32_t wmin[NWAVES], wmax[NWAVES];
   if ((threadIdx.x & 63) == 0) {
     wmin[threadIdx.x >> 6] = kmin;
     wmax[threadIdx.x >> 6] = kmax;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < 4; w++) {
+    for (int w = 1; w < NWAVES; w++) {
       kmin = min(kmin, wmin[w]);
       kmax = max(kmax, wmax[w]);
     }
     if (kmin <= kmax) {
       const int slot = blockIdx.x % kRangeSlots;
-      atomicMin(a.krange + slot, (unsigned long long)kmin);
-      atomicMax(a.krange + kRangeSlots + slot, (unsigned long long)kmax);
+      atomicMin(krange + slot, (unsigned long long)kmin);
+      atomicMax(krange + kRangeSlots + slot, (unsigned long long)kmax);
     }
+  }
+}
+
+__global__ __launch_bounds__(256) void key_kernel(KeyArgs a) {
+  int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip == 0) a.counters[9] = 0;
+  clear_targets(a, ip, (int64_t)gridDim.x * blockDim.x);
+  // smallest and largest key this lane writes (every lane takes part in the
+  // wavefront's reduction below, so no lane returns early)
+  uint32_t kmin = 0xffffffffu, kmax = 0u;
+  const bool in = ip < a.n_ub;
+  int64_t np = in ? *a.np : 0;
+  const PartVals pv = load_part(a, ip, np);
+  const PartCells pc = part_cells(a, pv);
+  for (int k = 0; in && k < a.ncomp; k++) {
+    double v;
+    const uint32_t key = part_contrib(a, pc, pv, k, v);
+    int64_t slot = (int64_t)k * a.n_ub + ip;
+    a.keys[slot] = key;
+    a.vals[slot] = key != a.discard ? v : 0.0;
+    if (key != a.discard) {
+      kmin = min(kmin, key);
+      kmax = max(kmax, key);
+    }
+  }
+  if (!a.krange) return;   // the sort covers the whole key space
+  key_range_fold<4>(a.krange, kmin, kmax);
+}
+
+// ---- deposition in a fixed order of our own (runs with cell-sorted particles)
+//
+// Each wavefront covers 64 * PPL consecutive particle slots, PPL per lane.
+// For each key its particles contribute to, every lane adds its own
+// contributions in slot order, the lanes' sums are added in one fixed tree
+// over the lane positions (lanes without one add +0), and the wavefront
+// emits one (key, partial) pair.  A
+// workgroup's partials are written to its region in wavefront order, the
+// regions are packed in workgroup order (count scan, then a copy), and the
+// packed list goes through the same stable sort and fold as the per-particle
+// pairs, with its length read on the device.  Each cell's sum is thus the
+// fold, in storage order, of its wavefronts' tree sums: an order fixed
+// by the storage order alone, so a run's result does not depend on
+// scheduling, on the runs batched with it or on other contexts.  With
+// particles sorted by cell, a wavefront's 64 particles share one or two
+// cells, so there are far fewer partials than particles.
+constexpr int RW = 4;     // wavefronts per reduce workgroup
+constexpr int PPL = 4;    // particle slots per lane
+constexpr int RG = 64 * RW * PPL;   // particle slots per reduce workgroup (reduce_groups: 1024)
+static_assert(RG == 1024, "reduce_groups() counts workgroups of 1024 slots");
+
+// x of lane (lane - shift within its row, or another row's lane for the
+// broadcasts); lanes the control does not write take +0
+template <int CTRL, int ROWS>
+__device__ inline double dpp_take(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROWS, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROWS, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
+// The 64 lanes' x added in one fixed tree, the total in lane 63: within each
+// row of 16 lanes, steps of 1, 2, 4 and 8 lanes leave the row's sum in its
+// last lane; then row 1 adds row 0's and row 3 adds row 2's (row_bcast:15),
+// and rows 2-3 add lane 31's (row_bcast:31).  Cross-lane moves are DPP, so
+// no LDS is involved.
+__device__ inline double wave_sum_to_last(double x) {
+  x += dpp_take<0x111, 0xF>(x);
+  x += dpp_take<0x112, 0xF>(x);
+  x += dpp_take<0x114, 0xF>(x);
+  x += dpp_take<0x118, 0xF>(x);
+  x += dpp_take<0x142, 0xA>(x);
+  x += dpp_take<0x143, 0xC>(x);
+  return x;
+}
+
+__global__ __launch_bounds__(64 * RW) void key_reduce_kernel(KeyArgs a) {
+  __shared__ uint32_t lk[RW][64 * PPL * 3];
+  __shared__ double lv[RW][64 * PPL * 3];
+  __shared__ int wcnt[RW + 1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t gbase = (int64_t)blockIdx.x * RG;   // first particle slot of the workgroup
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[9] = 0;
+  clear_targets(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, (int64_t)gridDim.x * blockDim.x);
+  uint32_t kmin = 0xffffffffu, kmax = 0u;
+  const int64_t np = gbase < a.n_ub ? *a.np : 0;
+  // lane l holds slots base + j*64 + l of the wavefront's 64*PPL consecutive slots
+  const int64_t base = gbase + (int64_t)w * 64 * PPL;
+  uint32_t key[3][PPL];
+  double val[3][PPL];
+  PartVals pv[PPL];
+#pragma unroll
+  for (int j = 0; j < PPL; j++) pv[j] = load_part(a, base + j * 64 + lane, np);
+#pragma unroll
+  for (int j = 0; j < PPL; j++) {
+    const int64_t ip = base + j * 64 + lane;
+    const bool in = ip < a.n_ub;
+    const PartCells pc = part_cells(a, pv[j]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      key[k][j] = a.discard;
+      val[k][j] = 0.0;
+      if (in && k < a.ncomp) key[k][j] = part_contrib(a, pc, pv[j], k, val[k][j]);
+      if (key[k][j] != a.discard) {
+        kmin = min(kmin, key[k][j]);
+        kmax = max(kmax, key[k][j]);
+      }
+    }
+    if (a.pvals && in) a.pvals[ip] = key[0][j] != a.discard ? val[0][j] : 0.0;   // for the charge sums
+  }
+  int cnt = 0;   // partials of this wavefront (uniform)
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if (k >= a.ncomp) break;
+    // each round takes the first pending key of the lowest lane that has one;
+    // every lane adds its values with that key in slot order, and the lanes'
+    // sums are added in the fixed tree
+    auto any_left = [&]() {
+      bool m = false;
+#pragma unroll
+      for (int j = 0; j < PPL; j++) m |= key[k][j] != a.discard;
+      return m;
+    };
+    unsigned long long pending = __ballot(any_left());
+    while (pending) {
+      const int leader = __ffsll((long long)pending) - 1;
+      uint32_t first = a.discard;
+#pragma unroll
+      for (int j = PPL - 1; j >= 0; j--)
+        if (key[k][j] != a.discard) first = key[k][j];
+      const uint32_t K = (uint32_t)__builtin_amdgcn_readlane((int)first, leader);
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < PPL; j++)
+        if (key[k][j] == K) {
+          s += val[k][j];
+          key[k][j] = a.discard;
+        }
+      const double x = wave_sum_to_last(s);
+      if (lane == 63) {
+        lk[w][cnt] = K;
+        lv[w][cnt] = x;
+      }
+      cnt++;
+      pending = __ballot(any_left());
+    }
+  }
+  if (lane == 0) wcnt[w] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int k = 0; k < RW; k++) {
+      const int c = wcnt[k];
+      wcnt[k] = run;
+      run += c;
+    }
+    wcnt[RW] = run;
+    // workgroups past the particles only clear the targets
+    if (gbase < a.n_ub) a.wg_count[blockIdx.x] = run;
+  }
+  __syncthreads();
+  // the workgroup's region holds at most ncomp partials per particle slot
+  const int64_t region = gbase * a.ncomp + wcnt[w];
+  for (int i = lane; i < cnt; i += 64) {
+    a.keys[region + i] = lk[w][i];
+    a.vals[region + i] = lv[w][i];
+  }
+  if (a.krange) key_range_fold<RW>(a.krange, kmin, kmax);
+}
+
+// exclusive scan of the reduce workgroups' partial counts (one workgroup);
+// the total goes to *total
+__global__ __launch_bounds__(1024) void partial_scan_kernel(const int32_t *cnt, int64_t n, int64_t *off,
+                                                            int64_t *total) {
+  __shared__ int64_t ws[16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t per = (n + 1023) / 1024;
+  const int64_t t0 = std::min<int64_t>((int64_t)threadIdx.x * per, n), t1 = std::min<int64_t>(t0 + per, n);
+  int64_t s = 0;
+  for (int64_t t = t0; t < t1; t++) s += cnt[t];
+  int64_t incl = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int64_t u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) ws[w] = incl;
+  __syncthreads();
+  int64_t before = 0;
+  for (int k = 0; k < w; k++) before += ws[k];
+  int64_t run = before + incl - s;
+  for (int64_t t = t0; t < t1; t++) {
+    off[t] = run;
+    run += cnt[t];
+  }
+  if (threadIdx.x == 1023) *total = run;
+}
+
+// packs each reduce workgroup's partials (region b of stride rstride) to off[b]
+__global__ __launch_bounds__(256) void partial_pack_kernel(const uint32_t *kin, const double *vin, const int32_t *cnt,
+                                                           const int64_t *off, int64_t rstride, uint32_t *kout,
+                                                           double *vout) {
+  const int64_t b = blockIdx.x;
+  const int c = cnt[b];
+  const int64_t src = b * rstride, dst = off[b];
+  for (int i = threadIdx.x; i < c; i += 256) {
+    kout[dst + i] = kin[src + i];
+    vout[dst + i] = vin[src + i];
   }
 }
 
@@ -223,6 +454,7 @@ struct FoldArgs {
   const uint32_t *sk1;
   const double *sv1;
   const uint64_t *range;        // key range of the ranged sort (null: sk/sv hold the result)
+  const int64_t *n_dev; // the number of pairs on the device (null: n)
   int64_t *counters;    // [9]: number of long segments
   int32_t *long_list;   // first positions of segments longer than LONG_SEG
   double *target[4];
@@ -248,15 +480,19 @@ __device__ inline double *target_cell(const FoldArgs &a, uint32_t key) {
 constexpr int FB = 16;
 constexpr int64_t LONG_SEG = 256;
 
-__global__ __launch_bounds__(256) void fold_kernel(FoldArgs a) {
-  // the buffer the ranged sort left its result in
-  const bool in1 = a.range && ranged_sort_in_k1(a.range);
-  const uint32_t *__restrict__ sk = in1 ? a.sk1 : a.sk;
-  const double *__restrict__ sv = in1 ? a.sv1 : a.sv;
+// pairs to fold: the host count, or the device count when there is one
+__device__ inline int64_t fold_pairs(const FoldArgs &a) {
+  if (!a.n_dev) return a.n;
+  const int64_t m = *a.n_dev;
+  return m < a.n ? (m > 0 ? m : 0) : a.n;
+}
+
+// the sorted positions [base, base + 64) of n, one per lane
+__device__ inline void fold_window(const FoldArgs &a, const uint32_t *__restrict__ sk, const double *__restrict__ sv,
+                                   int64_t n, int64_t base) {
   const int lane = threadIdx.x & 63;
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, base = p - lane;
-  if (base >= a.n) return;
-  const bool valid = p < a.n;
+  const int64_t p = base + lane;
+  const bool valid = p < n;
   const uint32_t key = valid ? sk[p] : 0u;
   // a run starts here, or the array has ended
   const bool edge = !valid || p == 0 || sk[p - 1] != key;
@@ -270,14 +506,14 @@ __global__ __launch_bounds__(256) void fold_kernel(FoldArgs a) {
   const uint32_t kL = __shfl(key, L, 64);
   bool longL = false;
   int64_t eL = 0;
-  if (sL < a.n && kL != a.discard) {
-    if (sL + LONG_SEG < a.n && sk[sL + LONG_SEG] == kL) {
+  if (sL < n && kL != a.discard) {
+    if (sL + LONG_SEG < n && sk[sL + LONG_SEG] == kL) {
       longL = true;
     } else {
       // sorted keys: the run ends within LONG_SEG positions of sL
       for (int64_t q = base + 64;; q += 64) {
         const int64_t i = q + lane;
-        const unsigned long long dm = __ballot(i >= a.n || sk[i] != kL);
+        const unsigned long long dm = __ballot(i >= n || sk[i] != kL);
         if (dm) {
           eL = q + (__ffsll((long long)dm) - 1);
           break;
@@ -320,6 +556,18 @@ __global__ __launch_bounds__(256) void fold_kernel(FoldArgs a) {
   }
 }
 
+__global__ __launch_bounds__(256) void fold_kernel(FoldArgs a) {
+  // the buffer the ranged sort left its result in
+  const bool in1 = a.range && ranged_sort_in_k1(a.range);
+  const uint32_t *__restrict__ sk = in1 ? a.sk1 : a.sk;
+  const double *__restrict__ sv = in1 ? a.sv1 : a.sv;
+  const int64_t n = fold_pairs(a), nwin = (n + 63) / 64, wpb = blockDim.x / 64;
+  // one 64-position window per wavefront (with a device count, a capped grid
+  // loops over the windows); the loop condition is uniform over the wavefront
+  for (int64_t win = (int64_t)blockIdx.x * wpb + (threadIdx.x >> 6); win < nwin; win += (int64_t)gridDim.x * wpb)
+    fold_window(a, sk, sv, n, win * 64);
+}
+
 // s + v[0] + v[1] + ... + v[15], left to right, each addition rounded (IEEE
 // v_add_f64, the same operation as `s += v`).  Written as one block of
 // dependent additions so nothing is scheduled between them: the chain then
@@ -355,6 +603,7 @@ __global__ __launch_bounds__(64) void fold_long_kernel(FoldArgs a) {
   const double *__restrict__ sv = in1 ? a.sv1 : a.sv;
   __shared__ __align__(16) double buf[LCH];
   const int lane = threadIdx.x;
+  const int64_t n = fold_pairs(a);
   const int64_t nl = a.counters[9];
   for (int64_t q = blockIdx.x; q < nl; q += gridDim.x) {
     const int64_t s0 = a.long_list[q];
@@ -366,8 +615,8 @@ __global__ __launch_bounds__(64) void fold_long_kernel(FoldArgs a) {
 #pragma unroll
     for (int u = 0; u < LU; u++) {
       const int64_t i = s0 + u * 64 + lane;
-      kk[u] = i < a.n ? sk[i] : ~key;
-      v[u] = i < a.n ? sv[i] : 0.0;
+      kk[u] = i < n ? sk[i] : ~key;
+      v[u] = i < n ? sv[i] : 0.0;
     }
     for (int64_t c0 = s0;; c0 += LCH) {
       // the segment's values in this chunk are a prefix of it, in position
@@ -386,8 +635,8 @@ __global__ __launch_bounds__(64) void fold_long_kernel(FoldArgs a) {
 #pragma unroll
         for (int u = 0; u < LU; u++) {
           const int64_t i = c1 + u * 64 + lane;
-          kk[u] = i < a.n ? sk[i] : ~key;
-          v[u] = i < a.n ? sv[i] : 0.0;
+          kk[u] = i < n ? sk[i] : ~key;
+          v[u] = i < n ? sv[i] : 0.0;
         }
       }
       if (lane == 0) {
@@ -422,6 +671,81 @@ __global__ __launch_bounds__(64) void fold_long_kernel(FoldArgs a) {
   }
 }
 
+// ---- re-sort of a cell-sort run's particles
+//
+// Key: the particle's half-cell, row-major in r (2*(r/dr) and 2*(z-zmin)/dz
+// truncated, clamped to the grid); slots past np get the discard key and sort
+// last.  The stable ranged radix sort orders (key, slot) pairs, and
+// permute_kernel gathers every particle column into the second set in that
+// order.  Any deterministic key would do: it only decides locality.
+struct CellSortArgs {
+  const double *z, *r;
+  const int64_t *np;
+  int64_t n_ub;
+  double zmin, dz, dr;
+  int nz, nr;
+  uint32_t *keys;
+  int64_t *idx;
+  uint32_t discard;
+  unsigned long long *krange;
+};
+
+__global__ __launch_bounds__(256) void cell_key_kernel(CellSortArgs a) {
+  const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = ip < a.n_ub;
+  const int64_t np = in ? *a.np : 0;
+  uint32_t key = a.discard, kmin = 0xffffffffu, kmax = 0u;
+  if (in && ip < np) {
+    const double hz = 2.0 * ((a.z[ip] - a.zmin) / a.dz), hr = 2.0 * (a.r[ip] / a.dr);
+    const long long zmax = 2LL * a.nz + 1, rmax = 2LL * a.nr + 1;
+    // NaN fails every comparison and lands in half-cell 0
+    const long long iz = hz >= 0 ? (hz < (double)zmax ? (long long)hz : zmax) : 0;
+    const long long ir = hr >= 0 ? (hr < (double)rmax ? (long long)hr : rmax) : 0;
+    key = (uint32_t)(ir * (zmax + 1) + iz);
+    kmin = kmax = key;
+  }
+  if (in) {
+    a.keys[ip] = key;
+    a.idx[ip] = ip;
+  }
+  key_range_fold<4>(a.krange, kmin, kmax);
+}
+
+struct PermArgs {
+  const double *src[16];
+  double *dst[16];
+  const int32_t *pid;
+  int32_t *pid2;
+  const int16_t *gen;
+  int16_t *gen2;
+  const uint32_t *st;
+  uint32_t *st2;
+  const int64_t *np;
+  int64_t n_ub;
+  const int64_t *idx0, *idx1;   // the sort's two value buffers (slot numbers)
+  const uint64_t *range;
+};
+
+__global__ __launch_bounds__(256) void permute_kernel(PermArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_ub || i >= *a.np) return;
+  const int64_t *ix = ranged_sort_in_k1(a.range) ? a.idx1 : a.idx0;
+  const int64_t j = ix[i];
+#pragma unroll
+  for (int k = 0; k < 16; k++) a.dst[k][i] = a.src[k][j];
+  a.pid2[i] = a.pid[j];
+  a.gen2[i] = a.gen[j];
+  a.st2[i] = a.st[j];
+}
+
+struct PtrTable {
+  double *p[16];
+};
+
+__global__ void set_ptrs_kernel(PtrTable t, double **dst) {
+  if (threadIdx.x < 16) dst[threadIdx.x] = t.p[threadIdx.x];
+}
+
 }  // namespace
 
 int deposit_cell_bits(int nz, int nr) {
@@ -431,7 +755,55 @@ int deposit_cell_bits(int nz, int nr) {
   return b;
 }
 
-// ordered scatter-add of the listed components into their target arrays
+// The reduce deposition of cell-sort runs: per-wavefront tree-sum partials,
+// packed in workgroup order, then the ranged sort and the fold over the
+// packed list with its length read on the device (counters[10]).
+static hipError_t reduce_scatter(Run &R, KeyArgs k, int ncomp, double *const *targets, hipStream_t s,
+                                 bool clear_first, double *sums, int key_bits, uint32_t discard) {
+  const int64_t n_ub = R.np_ub, n = n_ub * ncomp, nwg = reduce_groups(n_ub);
+  k.keys = R.keys;
+  k.vals = R.cz;
+  k.pvals = sums ? R.scz : nullptr;
+  k.wg_count = R.wg_count;
+  k.krange = reinterpret_cast<unsigned long long *>(R.krange);
+  int64_t kblocks = nwg;
+  if (clear_first) kblocks = std::max<int64_t>(kblocks, std::min<int64_t>(8192, (R.cells + 64 * RW - 1) / (64 * RW)));
+  hipLaunchKernelGGL(key_range_reset_kernel, dim3(1), dim3(kRangeSlots), 0, s, k.krange);
+  hipLaunchKernelGGL(key_reduce_kernel, dim3((unsigned)kblocks), dim3(64 * RW), 0, s, k);
+  hipLaunchKernelGGL(key_range_final_kernel, dim3(1), dim3(kRangeSlots), 0, s, k.krange);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (sums && (e = launch_sum2_vec(R.scz, n_ub, R.red, R.red_n, sums, s)) != hipSuccess) return e;
+  int64_t *total = R.counters + 10;
+  hipLaunchKernelGGL(partial_scan_kernel, dim3(1), dim3(1024), 0, s, R.wg_count, nwg, R.wg_off, total);
+  hipLaunchKernelGGL(partial_pack_kernel, dim3((unsigned)nwg), dim3(256), 0, s, R.keys, R.cz, R.wg_count, R.wg_off,
+                     (int64_t)RG * ncomp, R.skeys, R.scz);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const uint64_t *range = reinterpret_cast<const uint64_t *>(R.krange + 2 * kRangeSlots);
+  e = radix_sort_pairs_ranged(R.skeys, R.scz, R.keys, R.cz, n, key_bits, range, discard, R.sort_tmp,
+                              R.sort_tmp_bytes, s, total);
+  if (e != hipSuccess) return e;
+  FoldArgs f;
+  f.nz = R.d.nz;
+  f.ldz = R.ldz;
+  f.cell_bits = key_bits - 2;
+  f.n = n;
+  f.discard = discard;
+  f.sk = R.skeys;
+  f.sv = R.scz;
+  f.sk1 = R.keys;
+  f.sv1 = R.cz;
+  f.range = range;
+  f.n_dev = total;
+  f.counters = R.counters;
+  f.long_list = R.long_list;
+  for (int i = 0; i < 4; i++) f.target[i] = i < ncomp ? targets[i] : nullptr;
+  hipLaunchKernelGGL(fold_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0, s, f);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(fold_long_kernel, dim3((unsigned)std::min<int64_t>(n / LONG_SEG + 1, 2048)), dim3(64), 0, s, f);
+  return hipGetLastError();
+}
+
 // ordered scatter-add of the listed components into their target arrays; with
 // clear_first (at most three targets) the targets are set to +0 first
 static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *const *targets, hipStream_t s,
@@ -477,6 +849,12 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   k.clear_n = clear_first ? R.cells : 0;
   k.zero_terms = zero_terms ? 1 : 0;
   k.zero_target = zero_terms ? targets[0] : nullptr;
+  k.pvals = nullptr;
+  k.wg_count = nullptr;
+  for (int i = 0; i < 3; i++) k.need_v[i] = 0;
+  for (int i = 0; i < ncomp; i++)
+    if (comps[i] <= COMP_JPHI) k.need_v[comps[i]] = 1;
+  if (R.cell_sort > 0) return reduce_scatter(R, k, ncomp, targets, s, clear_first, sums, key_bits, discard);
   // enough workgroups for the particles and, when it clears the targets, for
   // a wide grid-stride clear (workgroups past n_ub only clear)
   int64_t kblocks = (n_ub + 255) / 256;
@@ -512,6 +890,7 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   f.sk1 = R.skeys;
   f.sv1 = R.scz;
   f.range = range;
+  f.n_dev = nullptr;
   f.counters = R.counters;
   f.long_list = R.long_list;
   for (int i = 0; i < 4; i++) f.target[i] = i < ncomp ? targets[i] : nullptr;
@@ -604,6 +983,67 @@ hipError_t launch_qreal_serial(Run &R, double *out, hipStream_t s) {
   k.upper_neu = R.d.upper_bc == SPF_NEUMANN;
   k.np = R.counters;
   hipLaunchKernelGGL(qreal_serial_kernel, dim3(1), dim3(64), 0, s, k, out);
+  return hipGetLastError();
+}
+
+hipError_t sort_particles(Run &R, hipStream_t s) {
+  const int64_t n_ub = R.np_ub;
+  if (R.cell_sort <= 0 || n_ub == 0) return hipSuccess;
+  CellSortArgs a;
+  a.z = R.P[1];
+  a.r = R.P[2];
+  a.np = R.counters;
+  a.n_ub = n_ub;
+  a.zmin = R.d.zmin;
+  a.dz = R.d.dz;
+  a.dr = R.d.dr;
+  a.nz = R.d.nz;
+  a.nr = R.d.nr;
+  a.keys = R.keys;
+  a.idx = reinterpret_cast<int64_t *>(R.cz);
+  const uint64_t nkeys = (uint64_t)(2 * R.d.nz + 2) * (uint64_t)(2 * R.d.nr + 2);
+  int key_bits = 1;
+  while ((1ull << key_bits) <= nkeys) key_bits++;
+  a.discard = 0xffffffffu;
+  a.krange = reinterpret_cast<unsigned long long *>(R.krange);
+  hipLaunchKernelGGL(key_range_reset_kernel, dim3(1), dim3(kRangeSlots), 0, s, a.krange);
+  hipLaunchKernelGGL(cell_key_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(key_range_final_kernel, dim3(1), dim3(kRangeSlots), 0, s, a.krange);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const uint64_t *range = reinterpret_cast<const uint64_t *>(R.krange + 2 * kRangeSlots);
+  e = radix_sort_pairs_ranged(R.keys, R.cz, R.skeys, R.scz, n_ub, key_bits, range, a.discard, R.sort_tmp,
+                              R.sort_tmp_bytes, s);
+  if (e != hipSuccess) return e;
+  PermArgs p;
+  for (int k = 0; k < 16; k++) {
+    p.src[k] = R.P[k];
+    p.dst[k] = R.P2[k];
+  }
+  p.pid = R.pid;
+  p.pid2 = R.pid2;
+  p.gen = R.gen;
+  p.gen2 = R.gen2;
+  p.st = R.status;
+  p.st2 = R.status2;
+  p.np = R.counters;
+  p.n_ub = n_ub;
+  p.idx0 = reinterpret_cast<const int64_t *>(R.cz);
+  p.idx1 = reinterpret_cast<const int64_t *>(R.scz);
+  p.range = range;
+  hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((n_ub + 255) / 256)), dim3(256), 0, s, p);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  // the gathered set becomes the current one; kernels take the pointers as
+  // arguments, and the device copy used by emission is rewritten in order
+  PtrTable t;
+  for (int k = 0; k < 16; k++) {
+    std::swap(R.P[k], R.P2[k]);
+    t.p[k] = R.P[k];
+  }
+  std::swap(R.pid, R.pid2);
+  std::swap(R.gen, R.gen2);
+  std::swap(R.status, R.status2);
+  hipLaunchKernelGGL(set_ptrs_kernel, dim3(1), dim3(64), 0, s, t, R.Pdev);
   return hipGetLastError();
 }
 

@@ -119,9 +119,18 @@ struct Run {
   int32_t *pid = nullptr;
   int16_t *gen = nullptr;
   uint32_t *status = nullptr;
+  // cell-sort runs: the particle set the re-sort gathers into (swapped with
+  // the current one afterwards), and the reduce deposition's partial counts
+  int64_t cell_sort = 0;    // re-sort interval in steps (0: reference order kept)
+  double *P2[16] = {};
+  int32_t *pid2 = nullptr;
+  int16_t *gen2 = nullptr;
+  uint32_t *status2 = nullptr;
+  int32_t *wg_count = nullptr;
+  int64_t *wg_off = nullptr;
   // [0] np, [1] npActive, [2] n_ev, [3] n_srows, [4] n_lrows, [5] events of a
   // push, [7] newly active particles of a push, [8] event-ring overflow flag,
-  // [9] long segments of a deposit; [6] unused
+  // [9] long segments of a deposit, [10] partials of a reduce deposition; [6] unused
   int64_t *counters = nullptr;
   int64_t *krange = nullptr;    // deposition key range: 256 min and 256 max slots, then the range (2)
   unsigned long long *new_active = nullptr;
@@ -214,6 +223,11 @@ int cathode_current_density(spf_ctx *c, int32_t run, int32_t cathode);
 // deposition
 // Jz, Jr, Jphi += particle currents; with clear_first they are set to +0 first
 hipError_t deposit_currents(Run &R, hipStream_t s, bool clear_first = false);
+// cell-sort runs: stable re-sort of the particles by half-cell (gathered into
+// the second particle set, which then becomes the current one)
+hipError_t sort_particles(Run &R, hipStream_t s);
+// workgroups of the reduce deposition (one per 1024 particle slots)
+inline int64_t reduce_groups(int64_t n_ub) { return (n_ub + 1023) / 1024; }
 int deposit_cell_bits(int nz, int nr);
 // stable radix sort of n (key, value) pairs on the low key_bits bits of the
 // keys; the input (k0, v0) and (k1, v1) alternate as pass input and output,
@@ -227,9 +241,11 @@ hipError_t radix_sort_pairs(uint32_t *k0, double *v0, uint32_t *k1, double *v1, 
 // the passes the range needs do work.  The result is in (k0, v0) when an even
 // number of passes worked (ranged_sort_in_k1 on the device), else in (k1, v1).
 inline int sorted_passes_max(int key_bits) { return (key_bits + 6) / 7; }
+// n_dev (optional): the number of pairs, read on the device (at most n); the
+// tables keep n's layout and only the tiles holding pairs are worked on
 hipError_t radix_sort_pairs_ranged(uint32_t *k0, double *v0, uint32_t *k1, double *v1, int64_t n, int key_bits,
                                    const uint64_t *range, uint32_t discard, void *tmp, size_t tmp_bytes,
-                                   hipStream_t s);
+                                   hipStream_t s, const int64_t *n_dev = nullptr);
 __device__ inline bool ranged_sort_in_k1(const uint64_t *range) {
   const uint64_t lo = range[0], hi = range[1];
   const uint32_t span = hi >= lo ? (uint32_t)(hi - lo) + 1u : 0u;

@@ -387,7 +387,19 @@ __device__ inline void push_block(const PushArgs &a, int64_t n_ub, int bid) {
   }
 }
 
-__global__ __launch_bounds__(256) void push_kernel(PushArgs a, int64_t n_ub) { push_block(a, n_ub, (int)blockIdx.x); }
+// waves per SIMD the push is compiled for (0: the compiler's choice), a build-time constant
+#ifndef SPF_PUSH_WAVES
+#define SPF_PUSH_WAVES 0
+#endif
+#if SPF_PUSH_WAVES > 0
+#define SPF_PUSH_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SPF_PUSH_WAVES, SPF_PUSH_WAVES)))
+#else
+#define SPF_PUSH_OCCUPANCY
+#endif
+
+__global__ __launch_bounds__(256) SPF_PUSH_OCCUPANCY void push_kernel(PushArgs a, int64_t n_ub) {
+  push_block(a, n_ub, (int)blockIdx.x);
+}
 
 __device__ inline int ragged_run_p(const int32_t *first, int n, int b) {
   int lo = 0, hi = n;

@@ -24,6 +24,11 @@
 // rel's top bit returns at once, so a sort over a narrow range of a wide key
 // space costs the passes the range needs.  The passes that ran decide which
 // buffer holds the result (sorted_keys); the host never waits for the range.
+//
+// Device count: the ranged sort can also take the number of pairs from device
+// memory (n_dev, at most the host's n).  The tables keep the layout of n's
+// tiles, the count and scatter kernels loop over the tiles that hold pairs
+// (a grid sized for n, capped), and the scan reads only those tiles.
 #include <algorithm>
 #include <utility>
 
@@ -64,16 +69,21 @@ struct KeyMap {
   __device__ uint32_t rel(uint32_t k) const { return !ranged ? k : (k == discard ? discard_rel : k - kmin); }
 };
 
+// pairs present: the host count, or the device count when there is one
+__device__ inline int64_t pairs_in(int64_t n, const int64_t *n_dev) {
+  if (!n_dev) return n;
+  const int64_t m = *n_dev;
+  return m < n ? (m > 0 ? m : 0) : n;
+}
+
 template <int RB>
-__global__ __launch_bounds__(ST) void sort_count_kernel(const uint32_t *keys, int64_t n, int shift, uint32_t *cnt,
-                                                        int64_t ntiles, const uint64_t *range, uint32_t discard) {
+__device__ inline void sort_count_tile(const uint32_t *keys, int64_t n, int shift, uint32_t *cnt, int64_t ntiles,
+                                       const KeyMap &km, int64_t tile) {
   constexpr int R = 1 << RB;
-  const KeyMap km(range, discard, shift);
-  if (!km.active) return;
   __shared__ uint32_t h[R];
   for (int d = threadIdx.x; d < R; d += ST) h[d] = 0;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * TILE;
+  const int64_t base = tile * TILE;
   const int lane = threadIdx.x & 63;
   uint32_t k[SI];
 #pragma unroll
@@ -97,7 +107,18 @@ __global__ __launch_bounds__(ST) void sort_count_kernel(const uint32_t *keys, in
     if (start) atomicAdd(&h[d], (uint32_t)(next - lane));
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < R; d += ST) cnt[(int64_t)d * ntiles + blockIdx.x] = h[d];
+  for (int d = threadIdx.x; d < R; d += ST) cnt[(int64_t)d * ntiles + tile] = h[d];
+  __syncthreads();   // h is cleared again by the next tile of this workgroup
+}
+
+template <int RB>
+__global__ __launch_bounds__(ST) void sort_count_kernel(const uint32_t *keys, int64_t n, int shift, uint32_t *cnt,
+                                                        int64_t ntiles, const uint64_t *range, uint32_t discard,
+                                                        const int64_t *n_dev) {
+  const KeyMap km(range, discard, shift);
+  if (!km.active) return;
+  const int64_t m = pairs_in(n, n_dev), mt = (m + TILE - 1) / TILE;
+  for (int64_t t = blockIdx.x; t < mt; t += gridDim.x) sort_count_tile<RB>(keys, m, shift, cnt, ntiles, km, t);
 }
 
 // inclusive scans of a and b over the workgroup's threads; returns the
@@ -130,13 +151,14 @@ __device__ inline void block_excl_scan2(uint32_t a, uint32_t b, uint32_t &ea, ui
 // tot[d] = the sum of the row; thread k scans a contiguous chunk of tiles
 __global__ __launch_bounds__(ST) void sort_scan_kernel(const uint32_t *cnt, uint32_t *pre, uint32_t *tot,
                                                        int64_t ntiles, int shift, const uint64_t *range,
-                                                       uint32_t discard) {
+                                                       uint32_t discard, int64_t n, const int64_t *n_dev) {
   if (!KeyMap(range, discard, shift).active) return;
   __shared__ uint32_t ws[2][NW];
   const uint32_t *row = cnt + (int64_t)blockIdx.x * ntiles;
   uint32_t *out = pre + (int64_t)blockIdx.x * ntiles;
-  const int64_t per = (ntiles + ST - 1) / ST;
-  const int64_t t0 = std::min<int64_t>((int64_t)threadIdx.x * per, ntiles), t1 = std::min<int64_t>(t0 + per, ntiles);
+  const int64_t mt = n_dev ? (pairs_in(n, n_dev) + TILE - 1) / TILE : ntiles;   // tiles holding pairs
+  const int64_t per = (mt + ST - 1) / ST;
+  const int64_t t0 = std::min<int64_t>((int64_t)threadIdx.x * per, mt), t1 = std::min<int64_t>(t0 + per, mt);
   uint32_t s = 0;
   for (int64_t t = t0; t < t1; t++) s += row[t];
   uint32_t run, unused;
@@ -153,13 +175,10 @@ __global__ __launch_bounds__(ST) void sort_scan_kernel(const uint32_t *cnt, uint
 // ranking item j before item j+1 and, within an item, lower lanes first,
 // follows the input order.
 template <int RB>
-__global__ __launch_bounds__(ST) void sort_scatter_kernel(const uint32_t *kin, const double *vin, uint32_t *kout,
-                                                          double *vout, int64_t n, int shift, const uint32_t *pre,
-                                                          const uint32_t *tot, int64_t ntiles, const uint64_t *range,
-                                                          uint32_t discard) {
+__device__ inline void sort_scatter_tile(const uint32_t *kin, const double *vin, uint32_t *kout, double *vout,
+                                         int64_t n, int shift, const uint32_t *pre, const uint32_t *tot,
+                                         int64_t ntiles, const KeyMap &km, int64_t tile) {
   constexpr int R = 1 << RB;
-  const KeyMap km(range, discard, shift);
-  if (!km.active) return;
   static_assert(R <= ST, "one thread per digit");
   __shared__ uint32_t off[NW][R];   // per-wavefront digit counts, then each wavefront's start in the tile
   __shared__ uint32_t gbase[R];     // output position of the digit's run minus its tile-local start
@@ -167,7 +186,7 @@ __global__ __launch_bounds__(ST) void sort_scatter_kernel(const uint32_t *kin, c
   __shared__ uint32_t skey[TILE];
   __shared__ double sval[TILE];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t base = (int64_t)blockIdx.x * TILE;
+  const int64_t base = tile * TILE;
   const int nvalid = (int)std::min<int64_t>(TILE, n - base);
   for (int d = threadIdx.x; d < NW * R; d += ST) (&off[0][0])[d] = 0;
   uint32_t key[SI], rank[SI];
@@ -217,7 +236,7 @@ __global__ __launch_bounds__(ST) void sort_scatter_kernel(const uint32_t *kin, c
   uint32_t tstart, gstart;
   block_excl_scan2(tcount, gcount, tstart, gstart, ws);
   if (d < R) {
-    gbase[d] = gstart + pre[(int64_t)d * ntiles + blockIdx.x] - tstart;   // modulo 2^32; n < 2^31
+    gbase[d] = gstart + pre[(int64_t)d * ntiles + tile] - tstart;   // modulo 2^32; n < 2^31
     for (int k = 0; k < NW; k++) off[k][d] += tstart;
   }
   __syncthreads();
@@ -243,16 +262,35 @@ __global__ __launch_bounds__(ST) void sort_scatter_kernel(const uint32_t *kin, c
       }
     }
   }
+  __syncthreads();   // the LDS tables are reused by the next tile of this workgroup
 }
+
+template <int RB>
+__global__ __launch_bounds__(ST) void sort_scatter_kernel(const uint32_t *kin, const double *vin, uint32_t *kout,
+                                                          double *vout, int64_t n, int shift, const uint32_t *pre,
+                                                          const uint32_t *tot, int64_t ntiles, const uint64_t *range,
+                                                          uint32_t discard, const int64_t *n_dev) {
+  const KeyMap km(range, discard, shift);
+  if (!km.active) return;
+  const int64_t m = pairs_in(n, n_dev), mt = (m + TILE - 1) / TILE;
+  for (int64_t t = blockIdx.x; t < mt; t += gridDim.x)
+    sort_scatter_tile<RB>(kin, vin, kout, vout, m, shift, pre, tot, ntiles, km, t);
+}
+
+// workgroups of a count or scatter launch: one per tile, or with a device
+// count a capped grid that loops over the tiles holding pairs
+constexpr int64_t kDynGrid = 2048;
 
 template <int RB>
 hipError_t sort_pass(const uint32_t *ki, const double *vi, uint32_t *ko, double *vo, int64_t n, int shift,
                      uint32_t *cnt, uint32_t *pre, uint32_t *tot, int64_t nt, const uint64_t *range, uint32_t discard,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(sort_count_kernel<RB>, dim3((unsigned)nt), dim3(ST), 0, s, ki, n, shift, cnt, nt, range, discard);
-  hipLaunchKernelGGL(sort_scan_kernel, dim3(1u << RB), dim3(ST), 0, s, cnt, pre, tot, nt, shift, range, discard);
-  hipLaunchKernelGGL(sort_scatter_kernel<RB>, dim3((unsigned)nt), dim3(ST), 0, s, ki, vi, ko, vo, n, shift, pre,
-                     tot, nt, range, discard);
+                     hipStream_t s, const int64_t *n_dev = nullptr) {
+  const unsigned grid = (unsigned)(n_dev ? std::min<int64_t>(nt, kDynGrid) : nt);
+  hipLaunchKernelGGL(sort_count_kernel<RB>, dim3(grid), dim3(ST), 0, s, ki, n, shift, cnt, nt, range, discard, n_dev);
+  hipLaunchKernelGGL(sort_scan_kernel, dim3(1u << RB), dim3(ST), 0, s, cnt, pre, tot, nt, shift, range, discard, n,
+                     n_dev);
+  hipLaunchKernelGGL(sort_scatter_kernel<RB>, dim3(grid), dim3(ST), 0, s, ki, vi, ko, vo, n, shift, pre, tot, nt,
+                     range, discard, n_dev);
   return hipGetLastError();
 }
 
@@ -292,7 +330,7 @@ hipError_t radix_sort_pairs(uint32_t *k0, double *v0, uint32_t *k1, double *v1, 
 
 hipError_t radix_sort_pairs_ranged(uint32_t *k0, double *v0, uint32_t *k1, double *v1, int64_t n, int key_bits,
                                    const uint64_t *range, uint32_t discard, void *tmp, size_t tmp_bytes,
-                                   hipStream_t s) {
+                                   hipStream_t s, const int64_t *n_dev) {
   if (n <= 0) return hipSuccess;
   if (n >= (1ll << 31) || key_bits < 1 || key_bits > 32 || tmp_bytes < radix_sort_tmp_bytes(n))
     return hipErrorInvalidValue;
@@ -301,7 +339,7 @@ hipError_t radix_sort_pairs_ranged(uint32_t *k0, double *v0, uint32_t *k1, doubl
   uint32_t *ki = k0, *ko = k1;
   double *vi = v0, *vo = v1;
   for (int p = 0; p < sorted_passes_max(key_bits); p++) {
-    const hipError_t e = sort_pass<7>(ki, vi, ko, vo, n, 7 * p, cnt, pre, tot, nt, range, discard, s);
+    const hipError_t e = sort_pass<7>(ki, vi, ko, vo, n, 7 * p, cnt, pre, tot, nt, range, discard, s, n_dev);
     if (e != hipSuccess) return e;
     std::swap(ki, ko);
     std::swap(vi, vo);

@@ -123,7 +123,10 @@ LOST_ROW_DTYPE = np.dtype(abi.LostRow)
 
 class DeviceRun:
     def __init__(self, ctx, grid, flags, max_particles=0, screens=(), compat=True, QoMC=K.QoMC_electron,
-                 stiffness=1.0, has_solenoid=False, screen_row_capacity=0, lost_row_capacity=0):
+                 stiffness=1.0, has_solenoid=False, screen_row_capacity=0, lost_row_capacity=0, cell_sort=0):
+        """cell_sort=0 keeps the reference's particle order and sums (bit for bit);
+        cell_sort=N re-sorts the particles by cell every N steps and deposits in a
+        fixed order of the library's own (needs compat=False)."""
         self.ctx, self.lib, self.grid = ctx, ctx.lib, grid
         flags = np.ascontiguousarray(flags, dtype=np.int16)
         if flags.shape != grid.shape:
@@ -151,6 +154,7 @@ class DeviceRun:
         d.compat = 1 if compat else 0
         d.screen_row_capacity = screen_row_capacity
         d.lost_row_capacity = lost_row_capacity
+        d.cell_sort = int(cell_sort)
         self.desc = d
         rid = ctypes.c_int32()
         ctx.check(self.lib.spf_run_create(ctx.h, ctypes.byref(d), ctypes.byref(rid)))
@@ -173,14 +177,19 @@ class DeviceRun:
         self.ctx, self.id = ctx, rid.value
 
     @staticmethod
-    def footprint_for(grid, max_particles, n_screens):
+    def footprint_for(grid, max_particles, n_screens, cell_sort=0):
         """HBM bytes a run of this grid, capacity and screen count would hold
         (spf_run_footprint), before creating it."""
         d = abi.RunDesc()
         d.nz, d.nr = grid.nz, grid.nr
         d.max_particles = int(max_particles)
         d.n_screens = int(n_screens)
+        d.cell_sort = int(cell_sort)
         return abi.load().spf_run_footprint(ctypes.byref(d))
+
+    def sort_particles(self):
+        """Re-sorts a cell-sort run's particles by cell now (spf_particles_sort)."""
+        self.ctx.check(self.lib.spf_particles_sort(self.ctx.h, self.id))
 
     def upload(self, name, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64)
