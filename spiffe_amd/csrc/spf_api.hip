@@ -38,7 +38,7 @@ void free_run(Run *R) {
   for (int k = 0; k < 6; k++)
     for (int j = 0; j < 2; j++) F(R->E2[k][j]);
   F(R->Bphi2); F(R->Br2); F(R->Bz2); F(R->Jz); F(R->Jr); F(R->Jphi); F(R->Psi); F(R->Q); for (int k = 0; k < 5; k++) F(R->PsiSpare[k]);
-  F(R->EzImp); F(R->ErImp); F(R->PsiImp); F(R->BzImp); F(R->BrImp); F(R->flags); F(R->cut0); F(R->cut1);
+  F(R->EzImp); F(R->ErImp); F(R->PsiImp); F(R->BzImp); F(R->BrImp); F(R->flags); F(R->open); F(R->cut0); F(R->cut1);
   F(R->cp); F(R->cm); F(R->bzf); F(R->sor_c); F(R->sor_d); F(R->volume);
   for (int k = 0; k < 16; k++) F(R->P[k]);
   F(R->pid); F(R->gen); F(R->status);
@@ -249,7 +249,7 @@ void *spf_stream(spf_ctx *c) { return c ? (void *)c->stream : nullptr; }
 int64_t spf_run_footprint(const spf_run_desc *d) {
   if (!d || d->nz < 3 || d->nr < 3) return -1;
   int64_t cells = ldz_for(d->nz) * (int64_t)(d->nr + 1);
-  int64_t field_bytes = cells * 8 * 33 + cells * 2;                      // 33 double grids + flags
+  int64_t field_bytes = cells * 8 * 33 + cells * 3;                      // 33 double grids, flags, open cells
   int64_t n = d->max_particles > 0 ? d->max_particles : 1;
   int64_t part_bytes = n * (16 * 8 + 4 + 2 + 4);                         // particle SoA
   int64_t event_bytes = n * (4 + 1 + 7 * 8 + 4 + 8);                     // staging, lists
@@ -307,6 +307,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   ALLOC(R->EzImp, cells); ALLOC(R->ErImp, cells); ALLOC(R->PsiImp, cells);
   ALLOC(R->BzImp, cells); ALLOC(R->BrImp, cells);
   ALLOC(R->flags, cells);
+  ALLOC(R->open, cells);
   ALLOC(R->stage, (int64_t)(nz + 1) * (nr + 1));
   ALLOC(R->cut0, nr + 1); ALLOC(R->cut1, nr + 1);
   ALLOC(R->cp, nr + 1); ALLOC(R->cm, nr + 1); ALLOC(R->bzf, nr + 1);
@@ -352,6 +353,7 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   {
     hipError_t e = hipMemcpyAsync(R->stage, hf.data(), hf.size() * sizeof(int16_t), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = launch_transpose_in_i16((const int16_t *)R->stage, R->flags, nz, nr, R->ldz, s);
+    if (e == hipSuccess) e = launch_open_cells(R->flags, R->ldz, nz, nr, R->open, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return bail(e, "flags upload");
   }
@@ -740,6 +742,7 @@ void spf::push_args(Run *R, double zForcesStart, PushArgs &a, RowsArgs &w) {
   a.longit = (d.modes & SPF_LONGIT_INTERP) != 0;
   a.radial = (d.modes & SPF_RADIAL_INTERP) != 0;
   a.has_sol = d.has_solenoid != 0;
+  a.open = R->open;
   a.zmin = d.zmin;
   a.zmax = d.zmax;
   a.rmax = d.rmax;

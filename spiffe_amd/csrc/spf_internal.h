@@ -41,6 +41,7 @@ struct PushArgs {
   double zmin, zmax, rmax, dz, dr, dt, time, c, QoMC, stiffness, zForcesStart;
   const double *Ez, *Er, *Ephi, *Bphi2, *Br2, *Bz2, *EzImp, *ErImp, *BzImp, *BrImp;
   const int16_t *flags;
+  const uint8_t *open;              // open cells: no loss-test or gather flag within reach
   int16_t turnoff[6];
   double cst[6];
   double *Q, *z, *r, *pz, *pr, *rpphi, *vz, *vr, *vphi, *r0, *z0, *t0, *rmax_p, *rmin_p, *zmax_p, *zmin_p;
@@ -108,6 +109,7 @@ struct Run {
   double *PsiSpare[5] = {};   // spare buffers the SOR batches rotate through
   double *EzImp = nullptr, *ErImp = nullptr, *PsiImp = nullptr, *BzImp = nullptr, *BrImp = nullptr;
   int16_t *flags = nullptr;
+  uint8_t *open = nullptr;      // per cell: the push may skip its flag tests there
   int32_t *cut0 = nullptr, *cut1 = nullptr;
   double *cp = nullptr, *cm = nullptr, *bzf = nullptr;
   double *sor_c = nullptr, *sor_d = nullptr, *volume = nullptr;
@@ -205,6 +207,7 @@ hipError_t launch_scale(double *p, int64_t n, double f, hipStream_t s);
 hipError_t launch_add_line(double *J, int ldz, int direction, int64_t ipos, int64_t istart, int64_t iend, double v,
                            hipStream_t s);
 hipError_t launch_push(const PushArgs &a, int64_t n_ub, hipStream_t s);
+hipError_t launch_open_cells(const int16_t *flags, int ldz, int nz, int nr, uint8_t *open, hipStream_t s);
 hipError_t launch_rows(const RowsArgs &a, hipStream_t s);
 // ragged push of several runs (table from push_multi_table, push_multi_table_bytes
 // long, copied to the device): one push launch, then one rows workgroup per run

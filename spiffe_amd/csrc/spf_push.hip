@@ -31,9 +31,15 @@ __device__ inline long long trunc_l(double x) {
 
 __device__ inline double sqr(double x) { return x * x; }
 
+// OPEN: the particle's cell is an open cell (PushArgs::open), where every flag
+// bit the loss test and the gather read is clear, so fl() is the constant 0
+// and those tests compile away; the arithmetic is the same code.
+template <bool OPEN>
 struct G {
   const PushArgs &a;
-  __device__ int16_t fl(long long iz, long long ir) const { return a.flags[fix(a.ldz, (int)iz, (int)ir)]; }
+  __device__ int16_t fl(long long iz, long long ir) const {
+    return OPEN ? (int16_t)0 : a.flags[fix(a.ldz, (int)iz, (int)ir)];
+  }
   __device__ double at(const double *p, long long iz, long long ir) const { return p[fix(a.ldz, (int)iz, (int)ir)]; }
   // imposed E: while the run has never set one its arrays hold +0.0, so the
   // literal +0.0 gives the same sums without the loads
@@ -245,16 +251,14 @@ struct G {
 // src/advance_particles.c:131-339 for one particle; returns whether the
 // particle has an event (loss or screen crossing) this step
 __device__ inline bool push_one(const PushArgs &a, int64_t ip, int64_t np) {
-  G g{a};
+  G<false> g{a};
   double dt = a.dt, dz = a.dz, dr = a.dr, zmin = a.zmin, zmax = a.zmax, rmax = a.rmax;
   double z = a.z[ip], r = a.r[ip];
   const double zold = z, rold = r;
-  const double zmax0 = a.zmax_p[ip], zmin0 = a.zmin_p[ip], rmax0 = a.rmax_p[ip], rmin0 = a.rmin_p[ip];
-  double zmx = zmax0, zmn = zmin0, rmx = rmax0, rmn = rmin0;
-  if (z > zmx) zmx = z;
-  if (z < zmn) zmn = z;
-  if (r > rmx) rmx = r;
-  if (r < rmn) rmn = r;
+  // The extents are updated with the old position here and with the new one
+  // below when the particle stays active (:151-158, :253-260); both updates
+  // are max/min comparisons, so they are applied together at the end, in
+  // the same order, and the extents need no registers through the gather.
   double vz = a.vz[ip], vr = a.vr[ip];
   z += vz * dt;
   r += vr * dt;
@@ -298,7 +302,10 @@ __device__ inline bool push_one(const PushArgs &a, int64_t ip, int64_t np) {
   long long iz = trunc_l((z - zmin) / dz);
   long long ir = trunc_l(fabs(r) / dr);
   if ((st & ACTIVE) && (iz > a.nz || ir > a.nr || iz < 0 || ir < 0)) lost = true;
-  if (!lost && (st & ACTIVE)) {
+  // the gather's cells lie within [iz-1, iz+3] x [ir-1, ir+3] of this one
+  // (ir is that of |r|, the radius after the reflection below)
+  const bool open = iz >= 1 && iz <= a.nz - 3 && ir >= 1 && ir <= a.nr - 3 && a.open[fix(a.ldz, (int)iz, (int)ir)];
+  if (!lost && (st & ACTIVE) && !open) {   // an open cell's loss code is 0
     if (g.loss_code(iz, ir)) lost = true;
   }
   bool rneg = false;
@@ -307,23 +314,25 @@ __device__ inline bool push_one(const PushArgs &a, int64_t ip, int64_t np) {
     pr *= -1;
     rneg = true;
   }
-  if (!lost && (st & ACTIVE)) {
-    if (z > zmx) zmx = z;
-    if (z < zmn) zmn = z;
-    if (r > rmx) rmx = r;
-    if (r < rmn) rmn = r;
-  }
+  const bool track_new = !lost && (st & ACTIVE);
   double Ez, Er, Ephi, Bz, Br, Bphi;
-  if (st & ACTIVE)
-    g.interp(z, r, Ez, Er, Ephi, Bz, Br, Bphi);
-  else
+  if (st & ACTIVE) {
+    if (open)
+      G<true>{a}.interp(z, r, Ez, Er, Ephi, Bz, Br, Bphi);
+    else
+      g.interp(z, r, Ez, Er, Ephi, Bz, Br, Bphi);
+  } else {
     Ez = Er = Ephi = Bz = Br = Bphi = 0;
+  }
   // momentum kick, :282-287
   double pphi = (r != 0 ? r_pphi / r : 0);
   double gamma = sqrt(sqr(pr) + sqr(pz) + sqr(pphi) + 1);
-  pr += adjustFactor * dt * ((r != 0 ? sqr(pphi) * a.c / gamma / r : 0) + a.QoMC * (Er + vphi * Bz - vz * Bphi)) / a.stiffness;
-  pz += adjustFactor * dt * a.QoMC * (Ez + vr * Bphi - vphi * Br) / a.stiffness;
-  r_pphi += adjustFactor * dt * a.QoMC * r * (Ephi + vz * Br - vr * Bz) / a.stiffness;
+  // x / stiffness is x itself when stiffness is 1 (electrons): the division is skipped then
+  const bool unit_stiffness = a.stiffness == 1.0;
+  auto per_stiffness = [&](double x) { return unit_stiffness ? x : x / a.stiffness; };
+  pr += per_stiffness(adjustFactor * dt * ((r != 0 ? sqr(pphi) * a.c / gamma / r : 0) + a.QoMC * (Er + vphi * Bz - vz * Bphi)));
+  pz += per_stiffness(adjustFactor * dt * a.QoMC * (Ez + vr * Bphi - vphi * Br));
+  r_pphi += per_stiffness(adjustFactor * dt * a.QoMC * r * (Ephi + vz * Br - vr * Bz));
   // velocities, :605-620
   double nvz, nvr, nvphi;
   if (r != 0) {
@@ -344,10 +353,24 @@ __device__ inline bool push_one(const PushArgs &a, int64_t ip, int64_t np) {
   a.vr[ip] = nvr;
   a.vphi[ip] = nvphi;
   // extents and status change only now and then: store them only when they do
-  if (zmx != zmax0) a.zmax_p[ip] = zmx;
-  if (zmn != zmin0) a.zmin_p[ip] = zmn;
-  if (rmx != rmax0) a.rmax_p[ip] = rmx;
-  if (rmn != rmin0) a.rmin_p[ip] = rmn;
+  {
+    const double zmax0 = a.zmax_p[ip], zmin0 = a.zmin_p[ip], rmax0 = a.rmax_p[ip], rmin0 = a.rmin_p[ip];
+    double zmx = zmax0, zmn = zmin0, rmx = rmax0, rmn = rmin0;
+    if (zold > zmx) zmx = zold;
+    if (zold < zmn) zmn = zold;
+    if (rold > rmx) rmx = rold;
+    if (rold < rmn) rmn = rold;
+    if (track_new) {
+      if (z > zmx) zmx = z;
+      if (z < zmn) zmn = z;
+      if (r > rmx) rmx = r;
+      if (r < rmn) rmn = r;
+    }
+    if (zmx != zmax0) a.zmax_p[ip] = zmx;
+    if (zmn != zmin0) a.zmin_p[ip] = zmn;
+    if (rmx != rmax0) a.rmax_p[ip] = rmx;
+    if (rmn != rmin0) a.rmin_p[ip] = rmn;
+  }
   if (lost) st = (st & ~ACTIVE) | LOST;
   if (st != st0) a.status[ip] = st;
   // screen crossings, tested as in :370 and :399
@@ -808,6 +831,35 @@ __global__ void append_commit_kernel(int64_t *counters, int64_t n, int64_t n_act
 }
 
 }  // namespace
+
+namespace {
+
+// open[iz, ir] = 1 when every flag in [iz-1, iz+3] x [ir-1, ir+3] has the
+// loss-test and gather bits (surface, inside, Ez-zero, Er-zero) clear, for
+// 1 <= iz <= nz-3, 1 <= ir <= nr-3; 0 elsewhere
+__global__ __launch_bounds__(256) void open_cells_kernel(const int16_t *flags, int ldz, int nz, int nr,
+                                                          uint8_t *open) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)ldz * (nr + 1)) return;
+  const int ir = (int)(q / ldz), iz = (int)(q % ldz);
+  uint8_t o = 0;
+  if (iz >= 1 && iz <= nz - 3 && ir >= 1 && ir <= nr - 3) {
+    constexpr int16_t mask = SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE | SPF_FL_EZ_ZERO | SPF_FL_ER_ZERO;
+    int16_t any = 0;
+    for (int jr = ir - 1; jr <= ir + 3; jr++)
+      for (int jz = iz - 1; jz <= iz + 3; jz++) any |= flags[fix(ldz, jz, jr)];
+    o = (any & mask) == 0;
+  }
+  open[q] = o;
+}
+
+}  // namespace
+
+hipError_t launch_open_cells(const int16_t *flags, int ldz, int nz, int nr, uint8_t *open, hipStream_t s) {
+  const int64_t n = (int64_t)ldz * (nr + 1);
+  hipLaunchKernelGGL(open_cells_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, flags, ldz, nz, nr, open);
+  return hipGetLastError();
+}
 
 hipError_t launch_push(const PushArgs &a, int64_t n_ub, hipStream_t s) {
   if (n_ub <= 0) return hipSuccess;
