@@ -205,3 +205,94 @@ def test_hires_field_energy_bounded(ctx, hires):
     assert max(energies) < 1.05 * e1, (e1, energies)
     assert min(energies) > 0.2 * e1, (e1, energies)
     d.close()
+
+
+def test_hires_cell_sort_steps_track_the_reference(ctx, hires):
+    """The same ten steps in a cell-sort run (particles re-sorted at steps 0 and
+    5, deposition in the library's own fixed order), against the restatement in
+    the reference's order: screen-crossing and loss counts equal every step,
+    every particle's cell indices equal (matched by particleID), positions and
+    momenta within 1e-9 of their largest magnitude, fields within 1e-9
+    relative L2, the same particles on the screen and lost with the same
+    direction codes."""
+    from test_gpu_cellsort import assert_rows_match, assert_same_cells
+    g, f = hires
+    p = joined(W.gun_bunch_gaussian(g, 1_500_000, charge=W.HIRES_CHARGE * 1_500_000 / W.HIRES_PARTICLES,
+                                    radius=W.HIRES_RADIUS), edge_particles(g, 20_000, 5))
+    n = len(p["z"])
+    o = OracleSim(g, f, max_particles=n, has_solenoid=True)
+    d = DeviceRun(ctx, g, f, max_particles=n, screens=SCREENS, compat=False, cell_sort=5)
+    for k, v in W.gun_fields(g).items():
+        o.set_field(k, v)
+        d.upload(k, v)
+    o.define_solenoid(**W.GUN_SOLENOID)
+    d.define_solenoid(**W.GUN_SOLENOID)
+    o.append(p)
+    d.append(p)
+    pc = dict(W.HIRES_POISSON, maximum_iterations=60)
+    for obj in (o, d):
+        obj.setup_poisson_correction(**pc)
+    rows = [[], [], [], []]
+    for step in range(10):
+        ro, lo = o.step(screens=SCREENS, compat=False)
+        d.step()
+        rd, ld = d.drain()
+        assert len(ro) == len(rd) and len(lo) == len(ld), "event counts differ at step %d" % step
+        for lst, x in zip(rows, (ro, rd, lo, ld)):
+            lst.append(x)
+    ro, rd, lo, ld = (np.concatenate(x) for x in rows)
+    assert len(ro) > 10_000 and len(lo) > 5_000
+    assert_rows_match(ro, rd, lo, ld)
+    assert_same_cells(g, o.particles(), d.particles(), 10)
+    for name in FIELDS:
+        a, b = o.field(name), d.download(name)
+        assert np.linalg.norm(b - a) <= 1e-9 * max(np.linalg.norm(a), 1e-300), name
+    d.close()
+
+
+def test_hires_16M_cell_sort_properties(ctx, hires):
+    """The run's own 16M particles in a cell-sort run (re-sorted every 20 steps,
+    as the run is configured): after the re-sort the particles are ordered by
+    half-cell and are the same particles; the deposit's charge-flux sums hold
+    (sum J x cell volume = sum Q v per component); over six whole steps every
+    particle removed has a lost row and the rest keep their charges."""
+    from test_gpu_cellsort import half_cell_keys
+    g, f = hires
+    p = joined(W.gun_bunch_gaussian(g, W.HIRES_PARTICLES, charge=W.HIRES_CHARGE, radius=W.HIRES_RADIUS),
+               edge_particles(g, 100_000, 6))
+    n = len(p["z"])
+    d = DeviceRun(ctx, g, f, max_particles=n, screens=SCREENS, compat=False, cell_sort=20)
+    for k, v in W.gun_fields(g).items():
+        d.upload(k, v)
+    d.define_solenoid(**W.GUN_SOLENOID)
+    d.append(p)
+    d.sort_particles()
+    q = d.particles()
+    assert np.all(np.diff(half_cell_keys(g, q)) >= 0)
+    assert np.array_equal(np.sort(q["particleID"]), np.sort(p["particleID"]))
+    d.zero_current_arrays()
+    d.add_particle_currents()
+    ir = np.arange(g.nr + 1)
+    vc = np.pi * g.dz * g.dr ** 2 * np.where(ir == 0, 0.25, 2.0 * ir)
+    vo = np.pi * g.dz * g.dr ** 2 * (2.0 * ir - 1)
+    irh = (p["r"] / g.dr + 0.5).astype(np.int64)
+    ira = (p["r"] / g.dr + 1.0).astype(np.int64)
+    qv = {k: p["Q"] * p[k] for k in ("vz", "vr", "vphi")}
+    assert np.sum(d.download("Jz") * vc) == pytest.approx(np.sum(qv["vz"][irh <= g.nr - 2]), rel=1e-9)
+    assert np.sum(d.download("Jphi") * vc) == pytest.approx(np.sum(qv["vphi"][irh <= g.nr - 2]),
+                                                            abs=1e-12 * np.sum(np.abs(qv["vz"])))
+    assert np.sum(d.download("Jr")[:, 1:] * vo[1:]) == pytest.approx(np.sum(qv["vr"][ira <= g.nr - 2]), rel=1e-9)
+    d.setup_poisson_correction(**W.HIRES_POISSON)
+    lost_ids = []
+    for _ in range(6):
+        d.step()
+        rows, lost = d.drain()
+        lost_ids.append(lost["particleID"])
+    lost_ids = np.concatenate(lost_ids)
+    left = d.particles()
+    assert len(left["z"]) + len(lost_ids) == n
+    assert np.array_equal(np.sort(np.concatenate([left["particleID"], lost_ids])), np.sort(p["particleID"]))
+    edge = left["particleID"] >= 100_000_000
+    assert np.all(left["Q"][~edge] == p["Q"][0]) and np.all(left["Q"][edge] == p["Q"][-1])
+    assert len(lost_ids) > 30_000
+    d.close()
