@@ -46,7 +46,7 @@ void free_run(Run *R) {
   F(R->pid2); F(R->gen2); F(R->status2); F(R->wg_count); F(R->wg_off); F(R->counters); F(R->krange); F(R->evt); F(R->evflag);
   for (int k = 0; k < 7; k++) F(R->ev[k]);
   F(R->last_old); F(R->evlist); F(R->move_src); F(R->move_dst); F(R->srows); F(R->lrows);
-  F(R->keys); F(R->skeys); F(R->vals); F(R->cz); F(R->scz); F(R->long_list); F(R->pcount); F(R->gflag); F(R->sort_tmp);
+  F(R->keys); F(R->skeys); F(R->vals); F(R->cz); F(R->scz); F(R->long_list); F(R->pcount); F(R->evoff); F(R->tcnt); F(R->toff); F(R->cbase); F(R->gflag); F(R->sort_tmp);
   F(R->stage); F(R->append_stage); F(R->emit_stage); F(R->Pdev); F(R->omega); F(R->red);
   F(R->red_counter); F(R->sor_state); F(R->sor_sums); F(R->sor_masks);
   R->live = false;
@@ -252,7 +252,8 @@ int64_t spf_run_footprint(const spf_run_desc *d) {
   int64_t field_bytes = cells * 8 * 33 + cells * 3;                      // 33 double grids, flags, open cells
   int64_t n = d->max_particles > 0 ? d->max_particles : 1;
   int64_t part_bytes = n * (16 * 8 + 4 + 2 + 4);                         // particle SoA
-  int64_t event_bytes = n * (4 + 1 + 7 * 8 + 4 + 8);                     // staging, lists
+  int64_t event_bytes = n * (4 + 1 + 7 * 8 + 4 + 8) + (n / 256 + 2) * 4 +
+                        (int64_t)(kMaxScreens + 1) * rows_tiles(n) * 8;   // staging, lists, rows pipeline
   int64_t sort_bytes = 3 * n * (4 + 4 + 8 + 8 + 8 + 1 + 1);  // keys, contributions (both sorted), flags, sort tables
   int64_t ring = default_srow_cap(d) * (int64_t)sizeof(spf_screen_row) + default_lrow_cap(d) * (int64_t)sizeof(spf_lost_row);
   // cell-sort runs: a second particle set and the reduce deposition's counts
@@ -324,6 +325,9 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   ALLOC(R->counters, 16);
   ALLOC(R->krange, 2 * 256 + 2);
   ALLOC(R->evt, cap); ALLOC(R->evflag, cap); ALLOC(R->pcount, cap / 256 + 2);
+  ALLOC(R->evoff, cap / 256 + 2);
+  ALLOC(R->tcnt, (int64_t)(kMaxScreens + 1) * rows_tiles(cap)); ALLOC(R->toff, (int64_t)(kMaxScreens + 1) * rows_tiles(cap));
+  ALLOC(R->cbase, kMaxScreens + 4);
   for (int k = 0; k < 7; k++) ALLOC(R->ev[k], cap);
   ALLOC(R->last_old, 4); ALLOC(R->evlist, cap); ALLOC(R->move_src, cap); ALLOC(R->move_dst, cap);
   R->srow_cap = default_srow_cap(d);
@@ -839,6 +843,11 @@ void spf::push_args(Run *R, double zForcesStart, PushArgs &a, RowsArgs &w) {
   w.move_dst = R->move_dst;
   w.tflag = R->gflag;
   w.overflow = (int32_t *)(R->counters + 8);
+  w.evoff = R->evoff;
+  w.tcnt = R->tcnt;
+  w.toff = R->toff;
+  w.ntmax = rows_tiles(R->cap > 0 ? R->cap : 1);
+  w.cbase = R->cbase;
 }
 
 extern "C" {

@@ -91,6 +91,11 @@ struct RowsArgs {
   int32_t *move_src, *move_dst;     // scratch, capacity = particle capacity
   uint8_t *tflag;                   // scratch bytes, capacity >= particle capacity
   int32_t *overflow;
+  // the multi-workgroup rows pipeline of large runs (rows_pipeline_for)
+  int32_t *evoff;                   // first event of each push workgroup
+  int32_t *tcnt, *toff;             // per row category (screens, then lost) and event tile: count, offset
+  int64_t ntmax;                    // tile stride of tcnt/toff (event tiles of the largest step)
+  int64_t *cbase;                   // per category: first row; [nscreens+1] lost count, [nscreens+2] moves
 };
 
 struct Run {
@@ -152,6 +157,9 @@ struct Run {
   double *cz = nullptr, *scz = nullptr;   // contributions, unsorted and sorted
   int32_t *vals = nullptr;                // list of long segments
   int32_t *pcount = nullptr;              // events per push workgroup
+  int32_t *evoff = nullptr;               // rows pipeline: first event of each push workgroup
+  int32_t *tcnt = nullptr, *toff = nullptr;   // rows pipeline: per category and event tile
+  int64_t *cbase = nullptr;               // rows pipeline: category bases and counts
   bool imposed_e = false;                 // EzImposed/ErImposed may hold nonzero values
   int32_t *long_list = nullptr;           // long segments, folded one per wavefront
   uint8_t *gflag = nullptr;
@@ -209,6 +217,11 @@ hipError_t launch_add_line(double *J, int ldz, int direction, int64_t ipos, int6
 hipError_t launch_push(const PushArgs &a, int64_t n_ub, hipStream_t s);
 hipError_t launch_open_cells(const int16_t *flags, int ldz, int nz, int nr, uint8_t *open, hipStream_t s);
 hipError_t launch_rows(const RowsArgs &a, hipStream_t s);
+// events per tile of the rows pipeline, and the particle slots from which a
+// run's rows go through it (below, one workgroup builds them: fewer launches)
+constexpr int kRowsTile = 1024;
+constexpr int64_t kRowsPipelineMin = 1 << 20;
+inline int64_t rows_tiles(int64_t cap) { return (cap + kRowsTile - 1) / kRowsTile + 1; }
 // ragged push of several runs (table from push_multi_table, push_multi_table_bytes
 // long, copied to the device): one push launch, then one rows workgroup per run
 size_t push_multi_table_bytes(int nruns);

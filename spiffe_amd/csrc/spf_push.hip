@@ -523,6 +523,74 @@ __device__ void find_loss_coordinates(const RowsArgs &a, double *zReturn, double
   *rReturn = fabs(rm);
 }
 
+// screen row of particle ip for screen s, src/advance_particles.c:362-425
+__device__ inline void make_screen_row(const RowsArgs &a, int32_t ip, int s, double dt, spf_screen_row &row) {
+  double zold = a.ev_zold[ip], vzz = a.ev_vz[ip], vra = a.ev_vr[ip];
+  double znew = a.z[ip], rnew = a.r[ip];
+  double r0v = a.ev_rold[ip];
+  for (int js = 0; js <= s; js++) {
+    double zs = a.sz[js];
+    bool fwd = zs <= znew && zs > zold && a.sdir[js] == 1;
+    bool bwd = !fwd && zs >= znew && zs < zold && a.sdir[js] == -1;
+    if (!fwd && !bwd) continue;
+    double dtp = (zs - zold) / vzz;
+    if (fwd) {
+      if (js != s) continue;
+      double spz0 = a.compat ? a.last_old[0] : a.ev_pz0[ip];
+      double spr0 = a.compat ? a.last_old[1] : a.ev_pr0[ip];
+      double srp0 = a.compat ? a.last_old[2] : a.ev_rp0[ip];
+      double pphi_a = rnew != 0 ? a.rpphi[ip] / rnew : 0;
+      double pphi0 = r0v != 0 ? srp0 / r0v : 0;
+      row.r = r0v + vra * dtp;
+      row.pz = spz0 + (a.pz[ip] - spz0) / dt * dtp;
+      row.pr = spr0 + (a.pr[ip] - spr0) / dt * dtp;
+      row.pphi = pphi0 + (pphi_a - pphi0) / dt * dtp;
+    } else {
+      r0v += vra * dtp;
+      if (js != s) continue;
+      row.r = r0v;
+      row.pz = a.pz[ip];
+      row.pr = a.pr[ip];
+      row.pphi = r0v != 0 ? a.rpphi[ip] / r0v : 0;
+    }
+    row.t = (a.time - dt + dtp);
+  }
+  row.q = a.Q[ip];
+  row.r0 = a.r0[ip];
+  row.t0 = a.t0[ip];
+  row.rmin = a.rmin_p[ip];
+  row.rmax = a.rmax_p[ip];
+  row.zmin = a.zmin_p[ip];
+  row.zmax = a.zmax_p[ip];
+  row.ip = ip;
+  row.step = a.step;
+  row.particleID = a.pid[ip];
+  row.generation = a.gen[ip];
+  row.screen = (int16_t)s;
+}
+
+// lost row of particle ip, src/advance_particles.c:440-480 (columns :581-599)
+__device__ inline void make_lost_row(const RowsArgs &a, int32_t ip, spf_lost_row &row) {
+  double zLost, rLost;
+  short code;
+  double rr = a.r[ip] * ((a.evt[ip] & EV_RNEG) ? -1 : 1);
+  find_loss_coordinates(a, &zLost, &rLost, &code, a.ev_zold[ip], a.ev_rold[ip], a.z[ip], rr);
+  row.t = a.time;
+  row.rLost = (float)rLost;
+  row.zLost = (float)zLost;
+  row.pz = (float)a.pz[ip];
+  row.pr = (float)a.pr[ip];
+  row.r0 = (float)a.r0[ip];
+  row.z0 = (float)a.z0[ip];
+  row.q = (float)a.Q[ip];
+  row.particleID = a.pid[ip];
+  row.ip = ip;
+  row.step = a.step;
+  row.generation = a.gen[ip];
+  row.directionCode = code;
+  row.pad = 0;
+}
+
 constexpr int ROWS_THREADS = 1024;
 
 // exclusive prefix of a 0/1 flag over the workgroup; returns the total
@@ -619,49 +687,8 @@ __device__ inline void rows_body(const RowsArgs &a) {
       int excl;
       int tot = block_flag_scan(has, wsum, excl);
       if (has) {
-        double zold = a.ev_zold[ip], vzz = a.ev_vz[ip], vra = a.ev_vr[ip];
-        double znew = a.z[ip], rnew = a.r[ip];
-        double r0v = a.ev_rold[ip];
         spf_screen_row row;
-        for (int js = 0; js <= s; js++) {
-          double zs = a.sz[js];
-          bool fwd = zs <= znew && zs > zold && a.sdir[js] == 1;
-          bool bwd = !fwd && zs >= znew && zs < zold && a.sdir[js] == -1;
-          if (!fwd && !bwd) continue;
-          double dtp = (zs - zold) / vzz;
-          if (fwd) {
-            if (js != s) continue;
-            double spz0 = a.compat ? a.last_old[0] : a.ev_pz0[ip];
-            double spr0 = a.compat ? a.last_old[1] : a.ev_pr0[ip];
-            double srp0 = a.compat ? a.last_old[2] : a.ev_rp0[ip];
-            double pphi_a = rnew != 0 ? a.rpphi[ip] / rnew : 0;
-            double pphi0 = r0v != 0 ? srp0 / r0v : 0;
-            row.r = r0v + vra * dtp;
-            row.pz = spz0 + (a.pz[ip] - spz0) / dt * dtp;
-            row.pr = spr0 + (a.pr[ip] - spr0) / dt * dtp;
-            row.pphi = pphi0 + (pphi_a - pphi0) / dt * dtp;
-          } else {
-            r0v += vra * dtp;
-            if (js != s) continue;
-            row.r = r0v;
-            row.pz = a.pz[ip];
-            row.pr = a.pr[ip];
-            row.pphi = r0v != 0 ? a.rpphi[ip] / r0v : 0;
-          }
-          row.t = (a.time - dt + dtp);
-        }
-        row.q = a.Q[ip];
-        row.r0 = a.r0[ip];
-        row.t0 = a.t0[ip];
-        row.rmin = a.rmin_p[ip];
-        row.rmax = a.rmax_p[ip];
-        row.zmin = a.zmin_p[ip];
-        row.zmax = a.zmax_p[ip];
-        row.ip = ip;
-        row.step = a.step;
-        row.particleID = a.pid[ip];
-        row.generation = a.gen[ip];
-        row.screen = (int16_t)s;
+        make_screen_row(a, ip, s, dt, row);
         int64_t pos = base + running + excl;
         if (pos < a.srow_cap)
           a.srows[pos] = row;
@@ -683,25 +710,8 @@ __device__ inline void rows_body(const RowsArgs &a) {
     int excl;
     int tot = block_flag_scan(has, wsum, excl);
     if (has) {
-      double zLost, rLost;
-      short code;
-      double rr = a.r[ip] * ((a.evt[ip] & EV_RNEG) ? -1 : 1);
-      find_loss_coordinates(a, &zLost, &rLost, &code, a.ev_zold[ip], a.ev_rold[ip], a.z[ip], rr);
       spf_lost_row row;
-      row.t = a.time;
-      row.rLost = (float)rLost;
-      row.zLost = (float)zLost;
-      row.pz = (float)a.pz[ip];
-      row.pr = (float)a.pr[ip];
-      row.r0 = (float)a.r0[ip];
-      row.z0 = (float)a.z0[ip];
-      row.q = (float)a.Q[ip];
-      row.particleID = a.pid[ip];
-      row.ip = ip;
-      row.step = a.step;
-      row.generation = a.gen[ip];
-      row.directionCode = code;
-      row.pad = 0;
+      make_lost_row(a, ip, row);
       int64_t pos = lbase + nlost + excl;
       if (pos < a.lrow_cap)
         a.lrows[pos] = row;
@@ -905,8 +915,269 @@ hipError_t launch_push_multi(const void *dev_table, int nruns, int64_t total_blo
   return hipGetLastError();
 }
 
+namespace {
+
+// ---- the rows of a large run in several kernels (same rows, same order)
+//
+// A single workgroup walks a step's events in chunks; with ~10^5 screen
+// crossings in a step that takes milliseconds.  Large runs use a pipeline
+// with no workgroup waiting for another:
+//  1. rows_scan_kernel: exclusive scan of the push workgroups' event counts
+//     (or, with no event, only this step's activations into the counters),
+//     and the tile counts cleared;
+//  2. rows_list_kernel: one wavefront per push workgroup writes its flagged
+//     particles, in order, at its offset: the ordered event list;
+//     it also counts, per tile of kRowsTile events and row category
+//     (screen s, then lost), the events in it;
+//  3. rows_tile_scan_kernel: offsets of the tiles per category, and the
+//     categories' first rows (screens grouped in screen order);
+//  4. rows_write_kernel: every event's rows at (category base + tile offset
+//     + rank in the tile), and the ordered lost list;
+//  5. rows_compact_kernel: the compaction's sources and destinations
+//     (as in rows_body), np and the counters;
+//  6. rows_move_kernel: the moves.
+
+__global__ __launch_bounds__(1024) void rows_scan_kernel(RowsArgs a) {
+  __shared__ long long ws[16];
+  if (*a.ev_total == 0) {
+    if (threadIdx.x == 0) {
+      *a.n_ev = 0;
+      a.np_active[0] += (int64_t)*a.new_active;
+      *a.new_active = 0;
+    }
+    return;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t per = (a.npb + 1023) / 1024;
+  const int64_t t0 = std::min<int64_t>((int64_t)threadIdx.x * per, a.npb), t1 = std::min<int64_t>(t0 + per, a.npb);
+  long long sum = 0;
+  for (int64_t t = t0; t < t1; t++) sum += a.pcount[t];
+  long long incl = sum;
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) ws[w] = incl;
+  __syncthreads();
+  long long run = incl - sum;
+  for (int k = 0; k < w; k++) run += ws[k];
+  for (int64_t t = t0; t < t1; t++) {
+    a.evoff[t] = (int32_t)run;
+    run += a.pcount[t];
+  }
+  __shared__ long long sh_nev;
+  if (threadIdx.x == 1023) {
+    *a.n_ev = run;
+    sh_nev = run;
+  }
+  __syncthreads();
+  // the tile counts the list kernel adds to start at 0
+  const int64_t ntiles = (sh_nev + kRowsTile - 1) / kRowsTile;
+  for (int c = 0; c <= a.nscreens; c++)
+    for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) a.tcnt[(int64_t)c * a.ntmax + t] = 0;
+}
+
+// category c of an event word: screen c (c < nscreens) or lost (c == nscreens)
+__device__ inline bool rows_has(const RowsArgs &a, uint32_t e, int c) {
+  return c < a.nscreens ? ((e >> (2 + c)) & 1u) != 0 : (e & EV_LOST) != 0;
+}
+
+// also counts each tile's events per category (integer additions: the same
+// counts in any order), one addition per wavefront chunk, category and tile
+__global__ __launch_bounds__(256) void rows_list_kernel(RowsArgs a) {
+  if (*a.n_ev == 0) return;
+  const int lane = threadIdx.x & 63;
+  for (int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); b < a.npb; b += (int64_t)gridDim.x * 4) {
+    if (a.pcount[b] == 0) continue;
+    int64_t o = a.evoff[b];
+    for (int j = 0; j < 4; j++) {
+      const int64_t i = b * 256 + 64 * j + lane;
+      const bool f = i < a.n_ub && a.evflag[i];
+      const unsigned long long m = __ballot(f);
+      const int64_t pos = o + __popcll(m & ((1ull << lane) - 1ull));
+      const uint32_t e = f ? a.evt[i] : 0u;
+      if (f) a.evlist[pos] = (int32_t)i;
+      if (m) {
+        // the chunk's events fill positions [o, o + popc(m)): at most two tiles;
+        // one addition per category and tile from lane 0
+        const int64_t t0 = o / kRowsTile, t1 = (o + __popcll(m) - 1) / kRowsTile;
+        const unsigned long long in0 = __ballot(f && pos / kRowsTile == t0);
+        for (int c = 0; c <= a.nscreens; c++) {
+          const unsigned long long hc = __ballot(f && rows_has(a, e, c));
+          const int n0 = __popcll(hc & in0), n1 = __popcll(hc) - n0;
+          if (lane == 0) {
+            if (n0) atomicAdd(a.tcnt + (int64_t)c * a.ntmax + t0, n0);
+            if (n1) atomicAdd(a.tcnt + (int64_t)c * a.ntmax + t1, n1);
+          }
+        }
+      }
+      o += __popcll(m);
+    }
+  }
+}
+
+__global__ __launch_bounds__(1024) void rows_tile_scan_kernel(RowsArgs a) {
+  __shared__ long long ws[16];
+  const int64_t nev = *a.n_ev;
+  if (nev == 0) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t ntiles = (nev + kRowsTile - 1) / kRowsTile;
+  const int64_t per = (ntiles + 1023) / 1024;
+  const int64_t t0 = std::min<int64_t>((int64_t)threadIdx.x * per, ntiles), t1 = std::min<int64_t>(t0 + per, ntiles);
+  int64_t sbase = *a.n_srows;
+  for (int c = 0; c <= a.nscreens; c++) {
+    const int32_t *cnt = a.tcnt + (int64_t)c * a.ntmax;
+    int32_t *off = a.toff + (int64_t)c * a.ntmax;
+    long long sum = 0;
+    for (int64_t t = t0; t < t1; t++) sum += cnt[t];
+    long long incl = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
+    }
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    long long run = incl - sum, total = 0;
+    for (int k = 0; k < w; k++) run += ws[k];
+    for (int k = 0; k < 16; k++) total += ws[k];
+    for (int64_t t = t0; t < t1; t++) {
+      off[t] = (int32_t)run;
+      run += cnt[t];
+    }
+    if (threadIdx.x == 0) {
+      if (c < a.nscreens) {
+        a.cbase[c] = sbase;
+      } else {
+        a.cbase[c] = *a.n_lrows;
+        a.cbase[c + 1] = total;   // lost this step
+        *a.n_lrows += total;
+        *a.n_srows = sbase;
+      }
+    }
+    sbase += total;
+    __syncthreads();   // ws is reused by the next category
+  }
+}
+
+__global__ __launch_bounds__(kRowsTile) void rows_write_kernel(RowsArgs a) {
+  __shared__ int wsum[kRowsTile / 64];
+  const int64_t nev = *a.n_ev, ntiles = (nev + kRowsTile - 1) / kRowsTile;
+  const double dt = a.dt;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t k = tile * kRowsTile + threadIdx.x;
+    const int32_t ip = k < nev ? a.evlist[k] : 0;
+    const uint32_t e = k < nev ? a.evt[ip] : 0u;
+    for (int c = 0; c <= a.nscreens; c++) {
+      const bool has = k < nev && rows_has(a, e, c);
+      int excl;
+      block_flag_scan(has, wsum, excl);
+      if (has) {
+        const int64_t rank = a.toff[(int64_t)c * a.ntmax + tile] + excl;
+        const int64_t pos = a.cbase[c] + rank;
+        if (c < a.nscreens) {
+          spf_screen_row row;
+          make_screen_row(a, ip, c, dt, row);
+          if (pos < a.srow_cap)
+            a.srows[pos] = row;
+          else
+            atomicOr(a.overflow, 1);
+        } else {
+          spf_lost_row row;
+          make_lost_row(a, ip, row);
+          if (pos < a.lrow_cap)
+            a.lrows[pos] = row;
+          else
+            atomicOr(a.overflow, 1);
+          a.move_dst[rank] = ip;
+        }
+      }
+    }
+  }
+}
+
+// the compaction of the fixed (non-compat) behaviour, as in rows_body
+__global__ __launch_bounds__(1024) void rows_compact_kernel(RowsArgs a) {
+  __shared__ int wsum[1024 / 64];
+  __shared__ long long sh_m0;
+  if (*a.n_ev == 0) return;
+  const int64_t np = a.np[0], nlost = a.cbase[a.nscreens + 1], new_np = np - nlost;
+  int cnt = 0;
+  for (int64_t k = threadIdx.x; k < nlost; k += blockDim.x) cnt += a.move_dst[k] < new_np;
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int k = 0; k < 16; k++) t += wsum[k];
+    sh_m0 = t;
+  }
+  __syncthreads();
+  const int64_t m0 = sh_m0;
+  for (int64_t t = threadIdx.x; t < nlost; t += blockDim.x) a.tflag[t] = 0;
+  __syncthreads();
+  for (int64_t k = m0 + threadIdx.x; k < nlost; k += blockDim.x) a.tflag[a.move_dst[k] - new_np] = 1;
+  __syncthreads();
+  int64_t running = 0;
+  for (int64_t c = 0; c < nlost; c += blockDim.x) {
+    const int64_t t = c + threadIdx.x;
+    const bool surv = t < nlost && !a.tflag[t];
+    int excl;
+    const int tot = block_flag_scan(surv, wsum, excl);
+    if (surv) a.move_src[m0 - 1 - (running + excl)] = (int32_t)(new_np + t);
+    running += tot;
+  }
+  if (threadIdx.x == 0) {
+    a.cbase[a.nscreens + 2] = m0;
+    const int64_t new_active = (int64_t)*a.new_active;
+    *a.new_active = 0;
+    a.np[0] = new_np;
+    a.np_active[0] += new_active - nlost;
+    *a.ev_total = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void rows_move_kernel(RowsArgs a) {
+  if (*a.n_ev == 0) return;
+  const int64_t m = a.cbase[a.nscreens + 2];
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t d = a.move_dst[k], s = a.move_src[k];
+    a.Q[d] = a.Q[s];
+    a.z[d] = a.z[s];
+    a.r[d] = a.r[s];
+    a.pz[d] = a.pz[s];
+    a.pr[d] = a.pr[s];
+    a.vz[d] = a.vz[s];
+    a.vr[d] = a.vr[s];
+    a.rpphi[d] = a.rpphi[s];
+    a.vphi[d] = a.vphi[s];
+    a.t0[d] = a.t0[s];
+    a.r0[d] = a.r0[s];
+    a.z0[d] = a.z0[s];
+    a.rmin_p[d] = a.rmin_p[s];
+    a.zmin_p[d] = a.zmin_p[s];
+    a.rmax_p[d] = a.rmax_p[s];
+    a.zmax_p[d] = a.zmax_p[s];
+    a.gen[d] = a.gen[s];
+    a.pid[d] = a.pid[s];
+    a.status[d] = a.status[s];
+  }
+}
+
+}  // namespace
+
 hipError_t launch_rows(const RowsArgs &a, hipStream_t s) {
-  hipLaunchKernelGGL(rows_kernel, dim3(1), dim3(ROWS_THREADS), 0, s, a);
+  if (a.compat || a.n_ub < kRowsPipelineMin) {
+    hipLaunchKernelGGL(rows_kernel, dim3(1), dim3(ROWS_THREADS), 0, s, a);
+    return hipGetLastError();
+  }
+  const unsigned tiles = (unsigned)std::min<int64_t>((a.n_ub + kRowsTile - 1) / kRowsTile, 1024);
+  hipLaunchKernelGGL(rows_scan_kernel, dim3(1), dim3(1024), 0, s, a);
+  hipLaunchKernelGGL(rows_list_kernel, dim3((unsigned)std::min<int64_t>((a.npb + 3) / 4, 4096)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(rows_tile_scan_kernel, dim3(1), dim3(1024), 0, s, a);
+  hipLaunchKernelGGL(rows_write_kernel, dim3(tiles), dim3(kRowsTile), 0, s, a);
+  hipLaunchKernelGGL(rows_compact_kernel, dim3(1), dim3(1024), 0, s, a);
+  hipLaunchKernelGGL(rows_move_kernel, dim3((unsigned)std::min<int64_t>((a.n_ub + 255) / 256, 2048)), dim3(256), 0, s,
+                     a);
   return hipGetLastError();
 }
 

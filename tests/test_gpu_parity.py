@@ -237,14 +237,17 @@ def test_push_single_step_rows_and_compaction(ctx, compat):
     d.close()
 
 
-@pytest.mark.parametrize("compat", [True, False])
-def test_push_mass_loss_compaction(ctx, compat):
-    """One step where ~40% of 60 000 particles leave the box: lost slots on both sides of
-    the new np and many 1024-particle chunks of the ordered compaction."""
+@pytest.mark.parametrize("compat,n", [(True, 60000), (False, 60000), (False, 1_200_000)])
+def test_push_mass_loss_compaction(ctx, compat, n):
+    """One step where ~40% of the particles leave the box: lost slots on both sides of
+    the new np and many 1024-particle chunks of the ordered compaction.  From 2^20
+    particle slots up the rows and the compaction go through the multi-workgroup
+    pipeline, with screens crossed by many particles as well."""
     g = gun_grid(modes=TMTE)
     f = gun_flags(g)
-    n = 60000
-    o, d = pair(ctx, g, f, cap=n, compat=compat)
+    screens = [(g.zmin + 0.5 * (g.zmax - g.zmin), 1), (g.zmin + 0.3 * (g.zmax - g.zmin), -1)] if n > 100000 else []
+    o = OracleSim(g, f, max_particles=n)
+    d = DeviceRun(ctx, g, f, max_particles=n, screens=screens, compat=compat)
     fl = random_fields(g, 61, scale=1e6)
     load_fields(o, d, fl)
     p = random_particles(g, n, 62, flags=f, pz_scale=0.5)
@@ -261,11 +264,12 @@ def test_push_mass_loss_compaction(ctx, compat):
         o.f.time_step = step
         o.f.time = step * g.dt + g.start_time
         d.set_step(step)
-        ro, lo = o.advance_particles((), compat)
+        ro, lo = o.advance_particles(screens, compat)
         d.advance_particles()
         rd, ld = d.drain()
         if step == 0:
             assert len(lo) > 0.3 * n
+            assert len(ro) > (500 if screens else -1)
         assert_rows(ro, rd, lo, ld)
         assert_particles(o, d)
     d.close()
