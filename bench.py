@@ -47,19 +47,24 @@ TRAFFIC = {
 TIMER_EVERY = 2
 
 
-def build_run(ctx, workload, n_particles, cell_sort=0):
+def build_run(ctx, workload, n_particles, cell_sort=0, events=False):
     from spiffe_amd import workloads as W
     from spiffe_amd.device import DeviceRun
+    screens = W.GUN_SCREENS
     if workload == "hires":
         g = W.gun_grid(dz=W.HIRES_DZ)
         bunch = W.gun_bunch_gaussian(g, n_particles, charge=W.HIRES_CHARGE, radius=W.HIRES_RADIUS)
         pc = W.HIRES_POISSON
+        if events:   # a screen inside the bunch, and particles leaving through the cathode plane and the wall
+            screens = W.HIRES_EVENT_SCREENS
+            lv = W.gun_leavers(g, 100_000)
+            bunch = {k: np.concatenate([bunch[k], lv[k]]) for k in bunch}
     else:
         g = W.gun_grid()
         bunch = W.gun_bunch(g, n_particles)
         pc = W.GUN_POISSON
     f = W.gun_flags(g)
-    run = DeviceRun(ctx, g, f, max_particles=n_particles, screens=W.GUN_SCREENS, compat=False, cell_sort=cell_sort)
+    run = DeviceRun(ctx, g, f, max_particles=len(bunch["z"]), screens=screens, compat=False, cell_sort=cell_sort)
     for k, v in W.gun_fields(g).items():
         run.upload(k, v)
     run.define_solenoid(**W.GUN_SOLENOID)
@@ -241,6 +246,10 @@ def main():
                          "own); 0 keeps the reference's particle order and sums; default: 20 for hires (its "
                          "configuration), 0 for gun")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--events", action="store_true",
+                    help="hires only: add a screen inside the bunch (about 110k crossings a step) and 100k particles "
+                         "that leave through the cathode plane and the wall, so the event rows and the compaction are "
+                         "timed in every step")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device copy (5 times), the reference for HBM byte counters")
     args = ap.parse_args()
@@ -271,12 +280,15 @@ def main():
     n_particles = args.particles or (W.HIRES_PARTICLES if args.workload == "hires" else 1_000_000)
     cell_sort = args.cell_sort if args.cell_sort >= 0 else (20 if args.workload == "hires" else 0)
     ctx = Context(0)
-    g, f, bunch, pc, run = build_run(ctx, args.workload, n_particles, cell_sort)
+    g, f, bunch, pc, run = build_run(ctx, args.workload, n_particles, cell_sort, args.events and args.workload == "hires")
     cells = (g.nz - 1) * (g.nr - 1)
 
     for _ in range(args.warmup):
         run.step()
     ctx.synchronize()
+    ring = 300_000 * (args.steps + args.warmup) if args.events else 1 << 20
+    if args.events:
+        run.drain(max_rows=ring, max_lost=ring)   # the warm-up steps' rows
     np0, _ = run.count()
     _, solves0, hs0 = run.poisson_stats()
     # timed region: K whole steps; the library's HIP-event timers on its stream sample one
@@ -290,6 +302,10 @@ def main():
     tm = ctx.timings()
     ctx.timing(0)
     np1, _ = run.count()
+    if args.events:   # the timed steps' rows, drained after the clock stopped
+        rows, lost = run.drain(max_rows=ring, max_lost=ring)
+        srows0, lost0, srows1, lost1 = 0, 0, len(rows), len(lost)
+        del rows, lost
     last_it, solves1, hs1 = run.poisson_stats()
     pushes = 0.5 * (np0 + np1) * args.steps
     ms_per_step = 1e3 * wall / args.steps
@@ -372,6 +388,8 @@ def main():
                                  if cell_sort else
                                  "none: particles in the reference's order, deposition in the reference's order")},
         "cell_updates_per_s": cells * args.steps / wall,
+        "events": ({"screen_rows_per_step": (srows1 - srows0) / args.steps, "lost_per_step": (lost1 - lost0) / args.steps}
+                   if args.events else None),
         "phase_ms": phase, "sor_half_sweeps_per_correction": iters_per_correction,
         "poisson_last_converged": converged,
         "kernel_ms_per_step": share,

@@ -112,7 +112,28 @@ def gun_bunch_gaussian(grid, n, charge=100e-12, radius=0.6e-3, sigma_t=3e-12, se
             "generation": np.zeros(n, dtype=np.int16), "status": np.full(n, abi.PART_ACTIVE, dtype=np.uint32)}
 
 
+def gun_leavers(grid, n, seed=6):
+    """Particles that leave during the first steps: half drift back through the
+    cathode plane z = zmin, half out through the wall at the top of the half
+    cell (ids from 100 000 000 up)."""
+    rng = np.random.default_rng(seed)
+    h = n // 2
+    z = np.concatenate([grid.zmin + rng.uniform(2e-6, 60e-6, h), grid.zmin + rng.uniform(5e-3, 30e-3, n - h)])
+    r = np.concatenate([rng.uniform(0, 1e-3, h), rng.uniform(41.80e-3, 41.99e-3, n - h)])
+    pz = np.concatenate([-0.5 * np.ones(h), 0.1 * rng.standard_normal(n - h)])
+    pr = np.concatenate([0.01 * rng.standard_normal(h), 0.8 * np.ones(n - h)])
+    gam = np.sqrt(pz * pz + pr * pr + 1)
+    return {"Q": np.full(n, -1e-17), "z": z, "r": r, "pz": pz, "pr": pr, "r_pphi": np.zeros(n),
+            "vz": K.c_mks * pz / gam, "vr": K.c_mks * pr / gam, "vphi": np.zeros(n), "r0": r.copy(),
+            "z0": z.copy(), "t0": np.zeros(n), "rmax": r.copy(), "rmin": r.copy(), "zmax": z.copy(),
+            "zmin": z.copy(), "particleID": np.arange(n, dtype=np.int32) + 100_000_000,
+            "generation": np.zeros(n, dtype=np.int16), "status": np.full(n, abi.PART_ACTIVE, dtype=np.uint32)}
+
+
 GUN_SCREENS = [(0.10, 1)]
+# a screen inside the high-resolution bunch (centred 2 mm from the cathode):
+# about 0.7% of its particles cross it every step
+HIRES_EVENT_SCREENS = [(0.002, 1), (0.10, 1)]
 # external solenoid: Gaussian on-axis Bz, 0.25 T peak at z = 20 cm, sigma 5 cm
 GUN_SOLENOID = dict(gaussian=1, Bz_peak=0.25, gaussian_center=0.20, gaussian_sigma=0.05)
 GUN_POISSON = dict(step_interval=10, accuracy=1e-8, maximum_iterations=3000)
