@@ -144,23 +144,26 @@ def scan_bench(n_runs, steps):
            "config": {"workload": "64-run 1.6-cell gun scan: dz=dr U(0.05,0.1) mm, particles logU(1e5,4e6), "
                                   "phase U(0,60) deg, gradient U(80,140) MV/m, solenoid U(0.15,0.35) T, "
                                   "charge logU(10 pC,2 nC), cathode radius U(0.5,2) mm; Poisson every 10 steps "
-                                  "to 1e-8 capped at 3000 half-sweeps", "seed": S.SCAN_SEED}}
+                                  "to 1e-8 capped at 3000 half-sweeps; particles re-sorted by cell at the "
+                                  "interval the autotuner picks (the same for both ways of running)",
+                      "seed": S.SCAN_SEED}}
     with Context(0) as ctx:
         grids = {}
         out["total_particles"] = int(sum(c["particles"] for c in cfgs))
-        seq_step, seq_all = S.one_at_a_time(ctx, cfgs, steps, grids)
-        # the autotuner picks the sub-batch size on the scan's own resident runs
-        # (steps 1-8 of every run, between corrections), then the scan runs batched
+        # the autotuner, on the scan's own resident runs (steps between corrections):
+        # the sub-batch size, then the re-sort interval
         runs = [S.build_run(ctx, c, grids) for c in cfgs]
         ctx.step_batch(runs)
         best, timings = S.tune_batch(ctx, runs, steps=2)
+        cadence, cad_probe = S.tune_cadence(ctx, runs)
         for r in runs:
             r.close()
-        bat_step, bat_all, kept = S.batched(ctx, cfgs, steps, best, grids, keep=True)
+        seq_step, seq_all = S.one_at_a_time(ctx, cfgs, steps, grids, cell_sort=cadence)
+        bat_step, bat_all, kept = S.batched(ctx, cfgs, steps, best, grids, keep=True, cell_sort=cadence)
         # bit-for-bit check of four runs: batched state against a run stepped alone
         checked = []
         for i in (0, n_runs // 3, 2 * n_runs // 3, n_runs - 1):
-            alone = S.build_run(ctx, cfgs[i], grids)
+            alone = S.build_run(ctx, cfgs[i], grids, cell_sort=cadence)
             for _ in range(steps):
                 alone.step()
             same = all(np.array_equal(alone.download(k).view(np.uint64), kept[i].download(k).view(np.uint64))
@@ -178,7 +181,10 @@ def scan_bench(n_runs, steps):
                 "end_to_end": {"batched_runs_per_hr": n_runs / bat_all * 3600,
                                "one_at_a_time_runs_per_hr": n_runs / seq_all * 3600,
                                "batched_s": bat_all, "one_at_a_time_s": seq_all},
-                "autotune": {"sub_batch": best, "probe_s": timings}, "bitwise_checks": checked})
+                "autotune": {"sub_batch": best, "probe_s": timings, "cell_sort_interval": cadence,
+                             "cell_sort_probe": {"sort_s": cad_probe["sort_s"],
+                                                 "mean_step_s": cad_probe["mean_step_s"]}},
+                "bitwise_checks": checked})
     return out
 
 

@@ -185,6 +185,9 @@ int spf_set_z_forces_start(spf_ctx *ctx, int32_t run, double z_forces_start);
  * cell_sort steps).  The particle order is the storage order from then on;
  * particleID names each particle.  SPF_EINVAL when cell_sort is 0. */
 int spf_particles_sort(spf_ctx *ctx, int32_t run);
+/* No counterpart in the reference: changes a cell-sort run's re-sort interval
+ * (> 0; a run created with cell_sort = 0 keeps the reference's order, SPF_EINVAL). */
+int spf_set_cell_sort(spf_ctx *ctx, int32_t run, int64_t interval);
 
 /* FDTD leapfrog of TM and TE fields; replaces advance_field_solutions, src/integrate.c:277-541 */
 int spf_fields_advance(spf_ctx *ctx, int32_t run);

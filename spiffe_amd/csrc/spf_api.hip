@@ -139,6 +139,35 @@ int spf::batch_table(spf_ctx *c, size_t bytes, void **host, void **dev) {
   return SPF_OK;
 }
 
+int spf::deposit_scratch(spf_ctx *c, int64_t pairs, int64_t runs) {
+  if (pairs <= c->dep_cap && runs <= c->dep_runs && c->dep_misc) return SPF_OK;
+  // grown only between uses: the stream's queued work may still read the old buffers
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  for (void *p : {(void *)c->dep_k0, (void *)c->dep_k1, (void *)c->dep_v0, (void *)c->dep_v1, c->dep_tmp,
+                  (void *)c->dep_long, (void *)c->dep_krange, (void *)c->dep_misc, (void *)c->dep_run_off})
+    if (p) HIPCALL(c, hipFree(p));
+  c->dep_k0 = c->dep_k1 = nullptr;
+  c->dep_v0 = c->dep_v1 = nullptr;
+  c->dep_tmp = nullptr;
+  c->dep_long = nullptr;
+  c->dep_krange = c->dep_misc = c->dep_run_off = nullptr;
+  c->dep_cap = c->dep_runs = 0;
+  const int64_t n = std::max<int64_t>(pairs, 1), r = std::max<int64_t>(runs, 1);
+  c->dep_tmp_bytes = radix_sort_tmp_bytes(n);
+  HIPCALL(c, hipMalloc((void **)&c->dep_k0, n * sizeof(uint32_t)));
+  HIPCALL(c, hipMalloc((void **)&c->dep_k1, n * sizeof(uint32_t)));
+  HIPCALL(c, hipMalloc((void **)&c->dep_v0, n * sizeof(double)));
+  HIPCALL(c, hipMalloc((void **)&c->dep_v1, n * sizeof(double)));
+  HIPCALL(c, hipMalloc(&c->dep_tmp, c->dep_tmp_bytes));
+  HIPCALL(c, hipMalloc((void **)&c->dep_long, (n / 256 + 1) * sizeof(int32_t)));
+  HIPCALL(c, hipMalloc((void **)&c->dep_krange, (2 * 256 + 2) * sizeof(int64_t)));
+  HIPCALL(c, hipMalloc((void **)&c->dep_misc, 16 * sizeof(int64_t)));
+  HIPCALL(c, hipMalloc((void **)&c->dep_run_off, (r + 1) * sizeof(int64_t)));
+  c->dep_cap = n;
+  c->dep_runs = r;
+  return SPF_OK;
+}
+
 hipError_t spf::batch_table_upload(spf_ctx *c, const void *host, void *dev, size_t bytes) {
   hipError_t e = hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) e = hipEventRecord(c->bt_ev[c->bt_cur], c->stream);
@@ -222,6 +251,9 @@ int spf_ctx_destroy(spf_ctx *c) {
   for (hipEvent_t e : c->bt_ev)
     if (e) (void)hipEventDestroy(e);
   if (c->bw_flags) (void)hipHostFree(c->bw_flags);
+  for (void *p : {(void *)c->dep_k0, (void *)c->dep_k1, (void *)c->dep_v0, (void *)c->dep_v1, c->dep_tmp,
+                  (void *)c->dep_long, (void *)c->dep_krange, (void *)c->dep_misc, (void *)c->dep_run_off})
+    if (p) (void)hipFree(p);
   if (c->hstage) (void)hipHostFree(c->hstage);
   if (c->stage_done) (void)hipEventDestroy(c->stage_done);
   for (hipEvent_t e : c->sor_ev)
@@ -858,6 +890,15 @@ int spf_particles_sort(spf_ctx *c, int32_t run) {
   if (R->cell_sort <= 0) return fail(c, SPF_EINVAL, "the run keeps the reference's particle order (cell_sort = 0)");
   PhaseTimer pt(c, SPF_T_SORT);
   HIPCALL(c, sort_particles(*R, c->stream));
+  return SPF_OK;
+}
+
+int spf_set_cell_sort(spf_ctx *c, int32_t run, int64_t interval) {
+  Run *R = get_run(c, run);
+  if (!R) return fail(c, SPF_EINVAL, "bad run id");
+  if (R->cell_sort <= 0) return fail(c, SPF_EINVAL, "the run was created with cell_sort = 0");
+  if (interval <= 0) return fail(c, SPF_EINVAL, "the re-sort interval must be > 0");
+  R->cell_sort = interval;
   return SPF_OK;
 }
 

@@ -605,7 +605,9 @@ int spf_step_batch(spf_ctx *c, int32_t n_runs, const int32_t *run_ids) {
   hipStream_t s = c->stream;
   PhaseTimer pt(c, SPF_T_STEP);
   // per run, in the order of src/integrate.c:134-183: the ramped imposed field,
-  // the time, emission, J reset and deposition
+  // the time, emission, J reset and deposition; the cell-sort runs with
+  // particles deposit together in one set of launches
+  std::vector<Run *> together;
   for (int32_t i = 0; i < n_runs; i++) {
     Run *R = runs[i];
     if (R->ramp_time > 0 && (rc = spf_imposed_efield(c, run_ids[i], 0, R->ramp_time, R->flat_top_time))) return rc;
@@ -614,10 +616,26 @@ int spf_step_batch(spf_ctx *c, int32_t n_runs, const int32_t *run_ids) {
       if ((rc = cathode_emit_step(c, run_ids[i], k, nullptr))) return rc;
     if (R->cell_sort > 0 && R->time_step % R->cell_sort == 0 && (rc = spf_particles_sort(c, run_ids[i]))) return rc;
     if (R->d.modes & SPF_SPACE_CHARGE) {
+      if (R->cell_sort > 0 && R->np_ub > 0 && (R->d.modes & (SPF_TM_FIELDS | SPF_TE_FIELDS))) {
+        together.push_back(R);
+        continue;
+      }
       PhaseTimer dt(c, SPF_T_DEPOSIT);
       HIPCALL(c, deposit_currents(*R, s, true));
     } else if ((rc = spf_currents_reset(c, run_ids[i]))) {
       return rc;
+    }
+  }
+  if (together.size() == 1) {
+    PhaseTimer dt(c, SPF_T_DEPOSIT);
+    HIPCALL(c, deposit_currents(*together[0], s, true));
+  } else if (!together.empty()) {
+    PhaseTimer dt(c, SPF_T_DEPOSIT);
+    hipError_t e = deposit_currents_multi(c, together.data(), (int)together.size());
+    if (e == hipErrorInvalidValue) {   // the key spaces do not fit one 32-bit space: one run at a time
+      for (Run *R : together) HIPCALL(c, deposit_currents(*R, s, true));
+    } else {
+      HIPCALL(c, e);
     }
   }
   // Poisson corrections: the runs due now solve together, grouped by accuracy and limit

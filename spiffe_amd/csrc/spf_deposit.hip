@@ -311,14 +311,16 @@ __device__ inline double wave_sum_to_last(double x) {
   return x;
 }
 
-__global__ __launch_bounds__(64 * RW) void key_reduce_kernel(KeyArgs a) {
+// Workgroup b of nb of one run's reduce deposition; keys are emitted plus
+// key_base (0 for a run alone; the run's offset in a batch's key space).
+__device__ inline void key_reduce_body(const KeyArgs &a, int64_t b, int64_t nb, uint32_t key_base) {
   __shared__ uint32_t lk[RW][64 * PPL * 3];
   __shared__ double lv[RW][64 * PPL * 3];
   __shared__ int wcnt[RW + 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t gbase = (int64_t)blockIdx.x * RG;   // first particle slot of the workgroup
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[9] = 0;
-  clear_targets(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, (int64_t)gridDim.x * blockDim.x);
+  const int64_t gbase = b * RG;   // first particle slot of the workgroup
+  if (b == 0 && threadIdx.x == 0) a.counters[9] = 0;
+  clear_targets(a, b * blockDim.x + threadIdx.x, nb * blockDim.x);
   uint32_t kmin = 0xffffffffu, kmax = 0u;
   const int64_t np = gbase < a.n_ub ? *a.np : 0;
   // lane l holds slots base + j*64 + l of the wavefront's 64*PPL consecutive slots
@@ -375,7 +377,7 @@ __global__ __launch_bounds__(64 * RW) void key_reduce_kernel(KeyArgs a) {
         }
       const double x = wave_sum_to_last(s);
       if (lane == 63) {
-        lk[w][cnt] = K;
+        lk[w][cnt] = K + key_base;
         lv[w][cnt] = x;
       }
       cnt++;
@@ -393,7 +395,7 @@ __global__ __launch_bounds__(64 * RW) void key_reduce_kernel(KeyArgs a) {
     }
     wcnt[RW] = run;
     // workgroups past the particles only clear the targets
-    if (gbase < a.n_ub) a.wg_count[blockIdx.x] = run;
+    if (gbase < a.n_ub) a.wg_count[b] = run;
   }
   __syncthreads();
   // the workgroup's region holds at most ncomp partials per particle slot
@@ -402,13 +404,18 @@ __global__ __launch_bounds__(64 * RW) void key_reduce_kernel(KeyArgs a) {
     a.keys[region + i] = lk[w][i];
     a.vals[region + i] = lv[w][i];
   }
+  if (a.krange && kmin <= kmax) {
+    kmin += key_base;
+    kmax += key_base;
+  }
   if (a.krange) key_range_fold<RW>(a.krange, kmin, kmax);
 }
 
+__global__ __launch_bounds__(64 * RW) void key_reduce_kernel(KeyArgs a) { key_reduce_body(a, blockIdx.x, gridDim.x, 0u); }
+
 // exclusive scan of the reduce workgroups' partial counts (one workgroup);
 // the total goes to *total
-__global__ __launch_bounds__(1024) void partial_scan_kernel(const int32_t *cnt, int64_t n, int64_t *off,
-                                                            int64_t *total) {
+__device__ inline void partial_scan_body(const int32_t *cnt, int64_t n, int64_t *off, int64_t *total) {
   __shared__ int64_t ws[16];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t per = (n + 1023) / 1024;
@@ -430,6 +437,11 @@ __global__ __launch_bounds__(1024) void partial_scan_kernel(const int32_t *cnt, 
     run += cnt[t];
   }
   if (threadIdx.x == 1023) *total = run;
+}
+
+__global__ __launch_bounds__(1024) void partial_scan_kernel(const int32_t *cnt, int64_t n, int64_t *off,
+                                                            int64_t *total) {
+  partial_scan_body(cnt, n, off, total);
 }
 
 // packs each reduce workgroup's partials (region b of stride rstride) to off[b]
@@ -458,10 +470,34 @@ struct FoldArgs {
   int64_t *counters;    // [9]: number of long segments
   int32_t *long_list;   // first positions of segments longer than LONG_SEG
   double *target[4];
+  const struct FoldTarget *tt;   // several runs' keys in one key space (null: one run, target[])
+  int ntt;
+};
+
+// a run's place in a batch's key space: keys [key_base, next run's key_base)
+struct FoldTarget {
+  uint32_t key_base;
+  int nz, ldz;
+  double *target[4];
 };
 
 
 __device__ inline double *target_cell(const FoldArgs &a, uint32_t key) {
+  if (a.tt) {
+    int lo = 0, hi = a.ntt;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (a.tt[mid].key_base <= key)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const FoldTarget &t = a.tt[lo];
+    const uint32_t kl = key - t.key_base;
+    const uint32_t cell = kl >> 2;
+    const int iz = (int)(cell % (uint32_t)(t.nz + 1)), ir = (int)(cell / (uint32_t)(t.nz + 1));
+    return t.target[kl & 3u] + fix(t.ldz, iz, ir);
+  }
   uint32_t k = key & 3u;
   uint32_t cell = key >> 2;
   int iz = (int)(cell % (uint32_t)(a.nz + 1)), ir = (int)(cell / (uint32_t)(a.nz + 1));
@@ -746,6 +782,74 @@ __global__ void set_ptrs_kernel(PtrTable t, double **dst) {
   if (threadIdx.x < 16) dst[threadIdx.x] = t.p[threadIdx.x];
 }
 
+// ---- the reduce deposition of several cell-sort runs in one set of launches
+//
+// Each run's keys are offset by the run's base in a batch key space (the
+// sum of the runs' key spaces), its workgroups' partials are packed run by
+// run into one list, and one ranged sort and one fold over that list find
+// each key's run by its base.  A run's partials, their order within each key
+// and the fold's additions are those of the run's own reduce deposition, so
+// every run gets the bits it gets alone.
+struct ReduceRun {
+  KeyArgs k;
+  uint32_t key_base;
+  int64_t nb;          // workgroups of the run's reduce launch share (particles, or the clear when wider)
+  int64_t nwg;         // workgroups holding particle slots
+  int64_t rstride;     // region stride of the run's partials (RG * ncomp)
+  int64_t *wg_off;
+  int64_t *total;      // the run's partial count
+};
+
+__device__ inline int ragged_run_d(const int32_t *first, int n, int b) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (first[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+__global__ __launch_bounds__(64 * RW) void key_reduce_multi_kernel(const ReduceRun *__restrict__ T,
+                                                                  const int32_t *__restrict__ first, int nruns) {
+  const int r = ragged_run_d(first, nruns, (int)blockIdx.x);
+  key_reduce_body(T[r].k, (int64_t)blockIdx.x - first[r], T[r].nb, T[r].key_base);
+}
+
+__global__ __launch_bounds__(1024) void partial_scan_multi_kernel(const ReduceRun *__restrict__ T) {
+  const ReduceRun &R = T[blockIdx.x];
+  partial_scan_body(R.k.wg_count, R.nwg, R.wg_off, R.total);
+}
+
+// run_off[r]: the first packed position of run r; *m: all partials; counters[9] cleared for the fold
+__global__ void run_offsets_kernel(const ReduceRun *__restrict__ T, int nruns, int64_t *run_off, int64_t *m,
+                                   int64_t *counters) {
+  if (threadIdx.x != 0) return;
+  int64_t run = 0;
+  for (int r = 0; r < nruns; r++) {
+    run_off[r] = run;
+    run += *T[r].total;
+  }
+  *m = run;
+  counters[9] = 0;
+}
+
+__global__ __launch_bounds__(256) void partial_pack_multi_kernel(const ReduceRun *__restrict__ T,
+                                                                 const int32_t *__restrict__ first, int nruns,
+                                                                 const int64_t *run_off, uint32_t *kout, double *vout) {
+  const int r = ragged_run_d(first, nruns, (int)blockIdx.x);
+  const int64_t b = (int64_t)blockIdx.x - first[r];
+  const ReduceRun &R = T[r];
+  const int c = R.k.wg_count[b];
+  const int64_t src = b * R.rstride, dst = run_off[r] + R.wg_off[b];
+  for (int i = threadIdx.x; i < c; i += 256) {
+    kout[dst + i] = R.k.keys[src + i];
+    vout[dst + i] = R.k.vals[src + i];
+  }
+}
+
 }  // namespace
 
 int deposit_cell_bits(int nz, int nr) {
@@ -753,6 +857,50 @@ int deposit_cell_bits(int nz, int nr) {
   int b = 1;
   while ((1ull << b) < cells) b++;
   return b;
+}
+
+// the key kernels' arguments for the listed components of run R
+static KeyArgs make_key_args(Run &R, const int *comps, int ncomp, double *const *targets, bool clear_first,
+                             bool zero_terms, uint32_t discard, int cell_bits) {
+  const int64_t n_ub = R.np_ub;
+  KeyArgs k;
+  k.nz = R.d.nz;
+  k.nr = R.d.nr;
+  k.ncomp = ncomp;
+  k.ldz = R.ldz;
+  for (int i = 0; i < 4; i++) k.comp[i] = i < ncomp ? comps[i] : 0;
+  k.zmin = R.d.zmin;
+  k.dz = R.d.dz;
+  k.dr = R.d.dr;
+  k.one_over_piDzDr2 = 1 / (3.141592653589793238462643 * R.d.dz * (R.d.dr * R.d.dr));
+  k.Q = R.P[0];
+  k.z = R.P[1];
+  k.r = R.P[2];
+  k.vz = R.P[6];
+  k.vr = R.P[7];
+  k.vphi = R.P[8];
+  k.status = R.status;
+  k.flags = R.flags;
+  k.left_neu = R.d.left_bc == SPF_NEUMANN;
+  k.right_neu = R.d.right_bc == SPF_NEUMANN;
+  k.upper_neu = R.d.upper_bc == SPF_NEUMANN;
+  k.np = R.counters;
+  k.n_ub = n_ub;
+  k.keys = R.keys;
+  k.vals = R.cz;
+  k.discard = discard;
+  k.cell_bits = cell_bits;
+  k.counters = R.counters;
+  for (int i = 0; i < 3; i++) k.clear[i] = clear_first && i < ncomp ? targets[i] : nullptr;
+  k.clear_n = clear_first ? R.cells : 0;
+  k.zero_terms = zero_terms ? 1 : 0;
+  k.zero_target = zero_terms ? targets[0] : nullptr;
+  k.pvals = nullptr;
+  k.wg_count = nullptr;
+  for (int i = 0; i < 3; i++) k.need_v[i] = 0;
+  for (int i = 0; i < ncomp; i++)
+    if (comps[i] <= COMP_JPHI) k.need_v[comps[i]] = 1;
+  return k;
 }
 
 // The reduce deposition of cell-sort runs: per-wavefront tree-sum partials,
@@ -797,6 +945,8 @@ static hipError_t reduce_scatter(Run &R, KeyArgs k, int ncomp, double *const *ta
   f.n_dev = total;
   f.counters = R.counters;
   f.long_list = R.long_list;
+  f.tt = nullptr;
+  f.ntt = 0;
   for (int i = 0; i < 4; i++) f.target[i] = i < ncomp ? targets[i] : nullptr;
   hipLaunchKernelGGL(fold_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0, s, f);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -817,43 +967,7 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   const int cell_bits = deposit_cell_bits(R.d.nz, R.d.nr);
   const int key_bits = cell_bits + 2;
   const uint32_t discard = (1u << key_bits) - 1u;   // all-ones cell with slot 3: never a real key
-  KeyArgs k;
-  k.nz = R.d.nz;
-  k.nr = R.d.nr;
-  k.ncomp = ncomp;
-  k.ldz = R.ldz;
-  for (int i = 0; i < 4; i++) k.comp[i] = i < ncomp ? comps[i] : 0;
-  k.zmin = R.d.zmin;
-  k.dz = R.d.dz;
-  k.dr = R.d.dr;
-  k.one_over_piDzDr2 = 1 / (3.141592653589793238462643 * R.d.dz * (R.d.dr * R.d.dr));
-  k.Q = R.P[0];
-  k.z = R.P[1];
-  k.r = R.P[2];
-  k.vz = R.P[6];
-  k.vr = R.P[7];
-  k.vphi = R.P[8];
-  k.status = R.status;
-  k.flags = R.flags;
-  k.left_neu = R.d.left_bc == SPF_NEUMANN;
-  k.right_neu = R.d.right_bc == SPF_NEUMANN;
-  k.upper_neu = R.d.upper_bc == SPF_NEUMANN;
-  k.np = R.counters;
-  k.n_ub = n_ub;
-  k.keys = R.keys;
-  k.vals = R.cz;
-  k.discard = discard;
-  k.cell_bits = cell_bits;
-  k.counters = R.counters;
-  for (int i = 0; i < 3; i++) k.clear[i] = clear_first && i < ncomp ? targets[i] : nullptr;
-  k.clear_n = clear_first ? R.cells : 0;
-  k.zero_terms = zero_terms ? 1 : 0;
-  k.zero_target = zero_terms ? targets[0] : nullptr;
-  k.pvals = nullptr;
-  k.wg_count = nullptr;
-  for (int i = 0; i < 3; i++) k.need_v[i] = 0;
-  for (int i = 0; i < ncomp; i++)
-    if (comps[i] <= COMP_JPHI) k.need_v[comps[i]] = 1;
+  KeyArgs k = make_key_args(R, comps, ncomp, targets, clear_first, zero_terms, discard, cell_bits);
   if (R.cell_sort > 0) return reduce_scatter(R, k, ncomp, targets, s, clear_first, sums, key_bits, discard);
   // enough workgroups for the particles and, when it clears the targets, for
   // a wide grid-stride clear (workgroups past n_ub only clear)
@@ -893,12 +1007,123 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   f.n_dev = nullptr;
   f.counters = R.counters;
   f.long_list = R.long_list;
+  f.tt = nullptr;
+  f.ntt = 0;
   for (int i = 0; i < 4; i++) f.target[i] = i < ncomp ? targets[i] : nullptr;
   hipLaunchKernelGGL(fold_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // at most n / LONG_SEG long segments
   int64_t nlong = n / LONG_SEG + 1;
   hipLaunchKernelGGL(fold_long_kernel, dim3((unsigned)std::min<int64_t>(nlong, 8192)), dim3(64), 0, s, f);
+  return hipGetLastError();
+}
+
+
+// The J reset and deposition of several cell-sort runs (space charge, at
+// least one component, particles present) in one set of launches.
+hipError_t deposit_currents_multi(spf_ctx *c, Run *const *runs, int n) {
+  hipStream_t s = c->stream;
+  std::vector<ReduceRun> rr(n);
+  std::vector<FoldTarget> ft(n);
+  std::vector<int32_t> first(n + 1), first2(n + 1);
+  uint64_t key_base = 0;
+  int64_t total_pairs = 0, blocks = 0, pblocks = 0;
+  for (int i = 0; i < n; i++) {
+    Run &R = *runs[i];
+    int comps[3];
+    double *targets[3];
+    int nc = 0;
+    if (R.d.modes & SPF_TM_FIELDS) {
+      comps[nc] = COMP_JZ;
+      targets[nc++] = R.Jz;
+      comps[nc] = COMP_JR;
+      targets[nc++] = R.Jr;
+    }
+    if (R.d.modes & SPF_TE_FIELDS) {
+      comps[nc] = COMP_JPHI;
+      targets[nc++] = R.Jphi;
+    }
+    const int cell_bits = deposit_cell_bits(R.d.nz, R.d.nr);
+    ReduceRun &q = rr[i];
+    q.k = make_key_args(R, comps, nc, targets, true, false, 0xffffffffu, cell_bits);
+    q.k.wg_count = R.wg_count;
+    q.key_base = (uint32_t)key_base;
+    q.nwg = reduce_groups(R.np_ub);
+    q.nb = std::max<int64_t>(q.nwg, std::min<int64_t>(2048, (R.cells + 64 * RW - 1) / (64 * RW)));
+    q.rstride = (int64_t)RG * nc;
+    q.wg_off = R.wg_off;
+    q.total = R.counters + 10;
+    ft[i].key_base = (uint32_t)key_base;
+    ft[i].nz = R.d.nz;
+    ft[i].ldz = R.ldz;
+    for (int k = 0; k < 4; k++) ft[i].target[k] = k < nc ? targets[k] : nullptr;
+    first[i] = (int32_t)blocks;
+    first2[i] = (int32_t)pblocks;
+    blocks += q.nb;
+    pblocks += q.nwg;
+    total_pairs += R.np_ub * nc;
+    key_base += 4ull * (uint64_t)(R.d.nz + 1) * (uint64_t)(R.d.nr + 1);
+  }
+  first[n] = (int32_t)blocks;
+  first2[n] = (int32_t)pblocks;
+  if (key_base >= 0xffffffffull || blocks >= (1ll << 31) || total_pairs >= (1ll << 31)) return hipErrorInvalidValue;
+  if (deposit_scratch(c, total_pairs, n) != SPF_OK) return hipErrorOutOfMemory;
+  unsigned long long *krange = reinterpret_cast<unsigned long long *>(c->dep_krange);
+  for (int i = 0; i < n; i++) rr[i].k.krange = krange;
+  // one table: the runs, both block maps, the fold targets
+  const size_t o1 = (sizeof(ReduceRun) * (size_t)n + 15) / 16 * 16;
+  const size_t o2 = o1 + (sizeof(int32_t) * (size_t)(n + 1) + 15) / 16 * 16;
+  const size_t o3 = o2 + (sizeof(int32_t) * (size_t)(n + 1) + 15) / 16 * 16;
+  const size_t bytes = o3 + sizeof(FoldTarget) * (size_t)n;
+  void *host, *dev;
+  if (batch_table(c, bytes, &host, &dev) != SPF_OK) return hipErrorOutOfMemory;
+  memcpy(host, rr.data(), sizeof(ReduceRun) * (size_t)n);
+  memcpy((char *)host + o1, first.data(), sizeof(int32_t) * (size_t)(n + 1));
+  memcpy((char *)host + o2, first2.data(), sizeof(int32_t) * (size_t)(n + 1));
+  memcpy((char *)host + o3, ft.data(), sizeof(FoldTarget) * (size_t)n);
+  hipError_t e = batch_table_upload(c, host, dev, bytes);
+  if (e != hipSuccess) return e;
+  const ReduceRun *T = (const ReduceRun *)dev;
+  const int32_t *F1 = (const int32_t *)((const char *)dev + o1), *F2 = (const int32_t *)((const char *)dev + o2);
+  const FoldTarget *TT = (const FoldTarget *)((const char *)dev + o3);
+  int64_t *misc = c->dep_misc;   // [0] packed partials, [9] long segments
+  hipLaunchKernelGGL(key_range_reset_kernel, dim3(1), dim3(kRangeSlots), 0, s, krange);
+  hipLaunchKernelGGL(key_reduce_multi_kernel, dim3((unsigned)blocks), dim3(64 * RW), 0, s, T, F1, n);
+  hipLaunchKernelGGL(key_range_final_kernel, dim3(1), dim3(kRangeSlots), 0, s, krange);
+  hipLaunchKernelGGL(partial_scan_multi_kernel, dim3((unsigned)n), dim3(1024), 0, s, T);
+  hipLaunchKernelGGL(run_offsets_kernel, dim3(1), dim3(64), 0, s, T, n, c->dep_run_off, misc, misc);
+  if (pblocks > 0)
+    hipLaunchKernelGGL(partial_pack_multi_kernel, dim3((unsigned)pblocks), dim3(256), 0, s, T, F2, n, c->dep_run_off,
+                       c->dep_k0, c->dep_v0);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  int key_bits = 1;
+  while (key_bits < 32 && (1ull << key_bits) <= key_base) key_bits++;
+  const uint64_t *range = reinterpret_cast<const uint64_t *>(c->dep_krange + 2 * kRangeSlots);
+  e = radix_sort_pairs_ranged(c->dep_k0, c->dep_v0, c->dep_k1, c->dep_v1, total_pairs, key_bits, range, 0xffffffffu,
+                              c->dep_tmp, c->dep_tmp_bytes, s, misc);
+  if (e != hipSuccess) return e;
+  FoldArgs f;
+  f.nz = 0;
+  f.ldz = 0;
+  f.cell_bits = 0;
+  f.n = total_pairs;
+  f.discard = 0xffffffffu;
+  f.sk = c->dep_k0;
+  f.sv = c->dep_v0;
+  f.sk1 = c->dep_k1;
+  f.sv1 = c->dep_v1;
+  f.range = range;
+  f.n_dev = misc;
+  f.counters = misc;
+  f.long_list = c->dep_long;
+  for (int i = 0; i < 4; i++) f.target[i] = nullptr;
+  f.tt = TT;
+  f.ntt = n;
+  hipLaunchKernelGGL(fold_kernel, dim3((unsigned)std::min<int64_t>((total_pairs + 255) / 256, 4096)), dim3(256), 0, s,
+                     f);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(fold_long_kernel, dim3((unsigned)std::min<int64_t>(total_pairs / LONG_SEG + 1, 2048)), dim3(64),
+                     0, s, f);
   return hipGetLastError();
 }
 

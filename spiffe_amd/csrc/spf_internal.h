@@ -239,6 +239,10 @@ int cathode_current_density(spf_ctx *c, int32_t run, int32_t cathode);
 // deposition
 // Jz, Jr, Jphi += particle currents; with clear_first they are set to +0 first
 hipError_t deposit_currents(Run &R, hipStream_t s, bool clear_first = false);
+// J reset and deposition of n cell-sort runs (space charge, TM or TE fields,
+// particles present) in one set of launches, each run the bits of its own
+// reduce deposition; hipErrorInvalidValue when their key spaces do not fit 32 bits
+hipError_t deposit_currents_multi(spf_ctx *c, Run *const *runs, int n);
 // cell-sort runs: stable re-sort of the particles by half-cell (gathered into
 // the second particle set, which then becomes the current one)
 hipError_t sort_particles(Run &R, hipStream_t s);
@@ -358,6 +362,14 @@ struct spf_ctx {
   // ragged (multi-run) launches: argument tables staged in pinned slots and
   // copied to device slots, round-robin; a slot's host side is rewritten only
   // after its copy has completed (event per slot)
+  // batched reduce deposition: packed partials and their sort (grown on demand)
+  int64_t dep_cap = 0, dep_runs = 0;
+  uint32_t *dep_k0 = nullptr, *dep_k1 = nullptr;
+  double *dep_v0 = nullptr, *dep_v1 = nullptr;
+  void *dep_tmp = nullptr;
+  size_t dep_tmp_bytes = 0;
+  int32_t *dep_long = nullptr;
+  int64_t *dep_krange = nullptr, *dep_misc = nullptr, *dep_run_off = nullptr;
   static constexpr int kTableSlots = 16;
   char *bt_host = nullptr, *bt_dev = nullptr;
   size_t bt_slot = 0;
@@ -401,6 +413,8 @@ void push_args(Run *R, double zForcesStart, PushArgs &a, RowsArgs &w);
 // device addresses; fill the host side, then batch_table_upload it
 int batch_table(spf_ctx *c, size_t bytes, void **host, void **dev);
 hipError_t batch_table_upload(spf_ctx *c, const void *host, void *dev, size_t bytes);
+// the batched deposition's scratch for `pairs` packed pairs and `runs` runs
+int deposit_scratch(spf_ctx *c, int64_t pairs, int64_t runs);
 // c->bw_flags holds at least n int64
 int batch_flags(spf_ctx *c, size_t n);
 

@@ -191,6 +191,10 @@ class DeviceRun:
         """Re-sorts a cell-sort run's particles by cell now (spf_particles_sort)."""
         self.ctx.check(self.lib.spf_particles_sort(self.ctx.h, self.id))
 
+    def set_cell_sort(self, interval):
+        """Changes a cell-sort run's re-sort interval (spf_set_cell_sort)."""
+        self.ctx.check(self.lib.spf_set_cell_sort(self.ctx.h, self.id, int(interval)))
+
     def upload(self, name, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64)
         if a.shape != self.grid.shape:

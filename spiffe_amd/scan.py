@@ -16,7 +16,11 @@ Two ways to run a scan on one GPU:
   launches over the runs), in sub-batches whose size tune_batch() picks by
   timing the candidates on the scan's own runs.
 Both give every run the same bits (spf_step_batch keeps each run's
-arithmetic), which verify_batched() checks run by run.
+arithmetic), which the bench checks run by run.
+
+Scan runs are cell-sort runs (particles re-sorted by cell, deposition in the
+library's own fixed order, results within the design tolerances of the
+reference's); tune_cadence() picks the re-sort interval from one timed probe.
 """
 import math
 import time
@@ -28,6 +32,7 @@ from .device import DeviceRun
 
 SCAN_SEED = 20260
 SCAN_POISSON = dict(step_interval=10, accuracy=1e-8, maximum_iterations=3000)
+SCAN_CELL_SORT = 10   # re-sort interval until tune_cadence has chosen one
 
 
 def scan_configs(n=64, seed=SCAN_SEED):
@@ -48,7 +53,7 @@ def scan_configs(n=64, seed=SCAN_SEED):
     return out
 
 
-def build_run(ctx, cfg, grids=None):
+def build_run(ctx, cfg, grids=None, cell_sort=SCAN_CELL_SORT):
     """A resident run of one scan configuration (grids caches geometry per mesh)."""
     key = cfg["dz"]
     if grids is not None and key in grids:
@@ -58,7 +63,8 @@ def build_run(ctx, cfg, grids=None):
         f = W.gun_flags(g)
         if grids is not None:
             grids[key] = (g, f)
-    run = DeviceRun(ctx, g, f, max_particles=cfg["particles"], screens=W.GUN_SCREENS, compat=False)
+    run = DeviceRun(ctx, g, f, max_particles=cfg["particles"], screens=W.GUN_SCREENS, compat=False,
+                    cell_sort=cell_sort)
     ez = W.gun_fields(g, E0=cfg["gradient"])["Ez"] * math.cos(math.radians(cfg["phase_deg"]))
     run.upload("Ez", ez)
     run.define_solenoid(**dict(W.GUN_SOLENOID, Bz_peak=cfg["solenoid"]))
@@ -75,12 +81,12 @@ def footprint_bytes(ctx, cfg):
     return b
 
 
-def one_at_a_time(ctx, cfgs, steps, grids=None):
+def one_at_a_time(ctx, cfgs, steps, grids=None, cell_sort=SCAN_CELL_SORT):
     """Each run created, stepped `steps` times and destroyed before the next.
     Returns (device-resident stepping seconds, end-to-end seconds)."""
     step_s, t0 = 0.0, time.perf_counter()
     for cfg in cfgs:
-        run = build_run(ctx, cfg, grids)
+        run = build_run(ctx, cfg, grids, cell_sort)
         ctx.synchronize()
         t1 = time.perf_counter()
         for _ in range(steps):
@@ -91,12 +97,12 @@ def one_at_a_time(ctx, cfgs, steps, grids=None):
     return step_s, time.perf_counter() - t0
 
 
-def batched(ctx, cfgs, steps, sub_batch, grids=None, keep=False):
+def batched(ctx, cfgs, steps, sub_batch, grids=None, keep=False, cell_sort=SCAN_CELL_SORT):
     """All runs resident, each sub-batch of `sub_batch` runs stepped together
     `steps` times by spf_step_batch.  Returns (stepping seconds, end-to-end
     seconds, runs if keep else None)."""
     t0 = time.perf_counter()
-    runs = [build_run(ctx, cfg, grids) for cfg in cfgs]
+    runs = [build_run(ctx, cfg, grids, cell_sort) for cfg in cfgs]
     ctx.synchronize()
     t1 = time.perf_counter()
     for i in range(0, len(runs), sub_batch):
@@ -132,3 +138,43 @@ def tune_batch(ctx, runs, candidates=(8, 16, 32, 64), steps=3):
         timings[s] = time.perf_counter() - t0
     best = min(timings, key=timings.get)
     return best, timings
+
+
+def tune_cadence(ctx, runs, probe_steps=20, candidates=(2, 4, 5, 8, 10, 16, 20)):
+    """The re-sort interval for a batch of resident cell-sort runs: one re-sort
+    of every run is timed, then `probe_steps` batched steps with no re-sort in
+    between, each timed (a step with a Poisson correction, which costs the same
+    at any interval, counts as the median of the others); the interval N
+    chosen minimises the mean step time (re-sort + steps 1..N after it) / N
+    over the candidates (the per-step time grows as particles drift from the
+    cell order).  Returns (N, probe timings).  The probe steps advance the
+    runs."""
+    for r in runs:
+        r.set_cell_sort(1 << 40)   # no re-sort inside the probe
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for r in runs:
+        r.sort_particles()
+    ctx.synchronize()
+    t_sort = time.perf_counter() - t0
+    t_steps, corrected = [], []
+    for _ in range(probe_steps):
+        solves = runs[0].poisson_stats()[1]
+        t0 = time.perf_counter()
+        ctx.step_batch(runs)
+        ctx.synchronize()
+        t_steps.append(time.perf_counter() - t0)
+        corrected.append(runs[0].poisson_stats()[1] != solves)
+    # a step with a Poisson correction costs the same whatever the interval:
+    # it stands in with the median of the other steps
+    plain = sorted(t for t, c in zip(t_steps, corrected) if not c)
+    if plain:
+        t_steps = [plain[len(plain) // 2] if c else t for t, c in zip(t_steps, corrected)]
+    cost = {}
+    for n in candidates:
+        if n <= probe_steps:
+            cost[n] = (t_sort + sum(t_steps[:n])) / n
+    best = min(cost, key=cost.get)
+    for r in runs:
+        r.set_cell_sort(best)
+    return best, {"sort_s": t_sort, "step_s": t_steps, "mean_step_s": cost}

@@ -40,6 +40,9 @@ from .device import DeviceRun
 
 DC_POISSON = dict(step_interval=10, accuracy=1e-6, maximum_iterations=2000)
 DC_IMPOSED_ITERATIONS = 2000
+# every run keeps its particles sorted by cell, re-sorted at the interval the
+# design scan's autotuner picks for these runs
+SERVE_CELL_SORT = 20
 
 
 def request_stream(n, rate_per_s, seed=7, steps=40, dc_fraction=0.2):
@@ -89,7 +92,7 @@ def prepare(req, grids):
 def build(ctx, prep):
     """The resident run of a prepared request (uploads; the DC gun's imposed field solve)."""
     d = DeviceRun(ctx, prep["grid"], prep["flags"], max_particles=prep["cap"], screens=prep["screens"],
-                  compat=False)
+                  compat=False, cell_sort=SERVE_CELL_SORT)
     for k, v in prep["fields"].items():
         d.upload(k, v)
     if "solenoid" in prep:
@@ -165,7 +168,7 @@ class _Pipeline:
                 if wait > 0:
                     time.sleep(wait)
                 p = prepare(r, self.grids)
-                fp = DeviceRun.footprint_for(p["grid"], p["cap"], len(p["screens"]))
+                fp = DeviceRun.footprint_for(p["grid"], p["cap"], len(p["screens"]), SERVE_CELL_SORT)
                 with self.lock:
                     while not self.stop and self.reserved > 0 and self.reserved + fp > self.budget:
                         self.lock.wait(0.01)

@@ -288,3 +288,37 @@ def test_cell_sort_run_is_deterministic_alone_and_batched(ctx):
         assert np.array_equal(pa[k].view(np.uint8), pc_[k].view(np.uint8)), k
     for r in (a, b, c, other):
         r.close()
+
+
+def test_cell_sort_runs_deposit_together_with_their_own_bits(ctx):
+    """Three cell-sort runs on different grids (and an exact-order run) stepped
+    together: the cell-sort runs' deposition goes through one set of launches
+    over a shared key space; every run must keep the bits it has alone."""
+    from _support import gun_grid as gg
+    specs = [(141, 61, 30_000, 41), (101, 45, 12_000, 42), (181, 77, 50_000, 43)]
+
+    def make(nz, nr, n, seed, cs=4):
+        g = gg(nz=nz, nr=nr, modes=TMTE | abi.SPACE_CHARGE)
+        f = gun_flags(g)
+        d = DeviceRun(ctx, g, f, max_particles=n, compat=False, cell_sort=cs, screens=[(0.07, 1)])
+        for k, v in random_fields(g, seed, scale=3e5).items():
+            d.upload(k, v)
+        d.append(random_particles(g, n, seed + 1, flags=f, pz_scale=1.0))
+        return d
+
+    alone = [make(*sp) for sp in specs]
+    batch = [make(*sp) for sp in specs]
+    exact = make(121, 51, 8_000, 44, cs=0)
+    for _ in range(12):
+        for d in alone:
+            d.step()
+        ctx.step_batch(batch + [exact])
+    for a, b in zip(alone, batch):
+        for name in ("Jz", "Jr", "Jphi", "Ez", "Er", "Bphi"):
+            assert np.array_equal(bits(a.download(name)), bits(b.download(name))), name
+        pa, pb = a.particles(), b.particles()
+        for k in ("z", "r", "pz", "pr", "r_pphi"):
+            assert np.array_equal(bits(pa[k]), bits(pb[k])), k
+        assert np.array_equal(pa["particleID"], pb["particleID"])
+    for d in alone + batch + [exact]:
+        d.close()
