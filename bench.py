@@ -156,9 +156,13 @@ def scan_bench(n_runs, steps):
         ctx.step_batch(runs)
         best, timings = S.tune_batch(ctx, runs, steps=2)
         cadence, cad_probe = S.tune_cadence(ctx, runs)
+        shape, shape_probe = S.tune_sor_shape(ctx, runs)
         for r in runs:
             r.close()
+        # one at a time: each run solves with the shape its own grid size picks
+        ctx.set_sor_shape(-1)
         seq_step, seq_all = S.one_at_a_time(ctx, cfgs, steps, grids, cell_sort=cadence)
+        ctx.set_sor_shape(shape)
         bat_step, bat_all, kept = S.batched(ctx, cfgs, steps, best, grids, keep=True, cell_sort=cadence)
         # bit-for-bit check of four runs: batched state against a run stepped alone
         checked = []
@@ -182,6 +186,7 @@ def scan_bench(n_runs, steps):
                                "one_at_a_time_runs_per_hr": n_runs / seq_all * 3600,
                                "batched_s": bat_all, "one_at_a_time_s": seq_all},
                 "autotune": {"sub_batch": best, "probe_s": timings, "cell_sort_interval": cadence,
+                             "sor_shape": shape, "sor_shape_probe_s": shape_probe,
                              "cell_sort_probe": {"sort_s": cad_probe["sort_s"],
                                                  "mean_step_s": cad_probe["mean_step_s"]}},
                 "bitwise_checks": checked})

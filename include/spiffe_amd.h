@@ -188,6 +188,11 @@ int spf_particles_sort(spf_ctx *ctx, int32_t run);
 /* No counterpart in the reference: changes a cell-sort run's re-sort interval
  * (> 0; a run created with cell_sort = 0 keeps the reference's order, SPF_EINVAL). */
 int spf_set_cell_sort(spf_ctx *ctx, int32_t run, int64_t interval);
+/* No counterpart in the reference: the SOR tile shape of this context's solves,
+ * 0 (12 half-sweeps per launch on 104x24-point tiles), 1 (8 on 112x32), or -1
+ * (the default: 1 from 1.2M grid points solved at once, else 0).  Psi, the
+ * iteration counts and every other result are the same for either shape. */
+int spf_set_sor_shape(spf_ctx *ctx, int32_t shape);
 
 /* FDTD leapfrog of TM and TE fields; replaces advance_field_solutions, src/integrate.c:277-541 */
 int spf_fields_advance(spf_ctx *ctx, int32_t run);

@@ -175,6 +175,7 @@ SIGNATURES = {
     "spf_particles_push": (ctypes.c_int, [_vp, _i32, _d]),
     "spf_particles_sort": (ctypes.c_int, [_vp, _i32]),
     "spf_set_cell_sort": (ctypes.c_int, [_vp, _i32, ctypes.c_int64]),
+    "spf_set_sor_shape": (ctypes.c_int, [_vp, _i32]),
     "spf_events_drain": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(ScreenRow), _i64, _pi64,
                                         ctypes.POINTER(LostRow), _i64, _pi64]),
     "spf_poisson_setup": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(PoissonParams)]),

@@ -893,6 +893,13 @@ int spf_particles_sort(spf_ctx *c, int32_t run) {
   return SPF_OK;
 }
 
+int spf_set_sor_shape(spf_ctx *c, int32_t shape) {
+  if (!c) return SPF_EINVAL;
+  if (shape < -1 || shape > 1) return fail(c, SPF_EINVAL, "shape must be -1, 0 or 1");
+  c->sor_shape = shape;
+  return SPF_OK;
+}
+
 int spf_set_cell_sort(spf_ctx *c, int32_t run, int64_t interval) {
   Run *R = get_run(c, run);
   if (!R) return fail(c, SPF_EINVAL, "bad run id");

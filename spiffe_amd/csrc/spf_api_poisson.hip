@@ -295,7 +295,8 @@ int sor_finish(spf_ctx *c, SorPass &p) {
 
 int sor_solve(spf_ctx *c, Run *R, double **Psi, const double *rhs, double accuracy, int64_t limit, int64_t *iters) {
   SorPass p;
-  int rc = sor_pass_init(c, p, R, *Psi, rhs, accuracy, limit, sor_shape_for((int64_t)(R->d.nz + 1) * (R->d.nr + 1)));
+  const int shape = c->sor_shape >= 0 ? c->sor_shape : sor_shape_for((int64_t)(R->d.nz + 1) * (R->d.nr + 1));
+  int rc = sor_pass_init(c, p, R, *Psi, rhs, accuracy, limit, shape);
   if (rc || (rc = sor_finish(c, p))) return rc;
   *iters = p.iters;
   adopt_result(R, Psi, p.ring, kRing, p.out);
@@ -315,7 +316,7 @@ int sor_solve_multi(spf_ctx *c, Run *const *runs, const double *const *rhs, doub
   // together the runs fill the GPU as one large grid would
   int64_t points = 0;
   for (int i = 0; i < np; i++) points += (int64_t)(runs[i]->d.nz + 1) * (runs[i]->d.nr + 1);
-  const int shape = sor_shape_for(points);
+  const int shape = c->sor_shape >= 0 ? c->sor_shape : sor_shape_for(points);
   for (int i = 0; i < np; i++) {
     if ((rc = sor_pass_init(c, ps[i], runs[i], runs[i]->Psi, rhs[i], accuracy, limit, shape))) return rc;
     pp[i] = &ps[i];

@@ -362,6 +362,7 @@ struct spf_ctx {
   // ragged (multi-run) launches: argument tables staged in pinned slots and
   // copied to device slots, round-robin; a slot's host side is rewritten only
   // after its copy has completed (event per slot)
+  int sor_shape = -1;           // SOR tile shape of this context's solves (-1: by the points solved)
   // batched reduce deposition: packed partials and their sort (grown on demand)
   int64_t dep_cap = 0, dep_runs = 0;
   uint32_t *dep_k0 = nullptr, *dep_k1 = nullptr;

@@ -95,6 +95,10 @@ class Context:
         self.check(self.lib.spf_timing_read(self.h, ms, cnt, tot))
         return {name: (ms[i], cnt[i], tot[i]) for i, name in enumerate(abi.TIMERS)}
 
+    def set_sor_shape(self, shape):
+        """SOR tile shape of this context's solves: 0, 1, or -1 (by the points solved)."""
+        self.check(self.lib.spf_set_sor_shape(self.h, int(shape)))
+
     def step_batch(self, runs):
         """One whole step of each run (DeviceRun objects of this context), their
         phases launched together over all of them."""

@@ -178,3 +178,28 @@ def tune_cadence(ctx, runs, probe_steps=20, candidates=(2, 4, 5, 8, 10, 16, 20))
     for r in runs:
         r.set_cell_sort(best)
     return best, {"sort_s": t_sort, "step_s": t_steps, "mean_step_s": cost}
+
+
+def tune_sor_shape(ctx, runs, shapes=(0, 1)):
+    """The SOR tile shape for a batch of resident runs: for each shape, the
+    batch is stepped up to and through its next Poisson correction and the
+    correction step is timed (every scan correction runs to the same
+    half-sweep cap, so the steps compare).  The context keeps the fastest
+    shape.  Psi and every other result are the same for either shape; the
+    probe steps advance the runs.  Returns (shape, correction step seconds)."""
+    times = {}
+    for shape in shapes:
+        ctx.set_sor_shape(shape)
+        for _ in range(SCAN_POISSON["step_interval"] + 1):
+            solves = runs[0].poisson_stats()[1]
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ctx.step_batch(runs)
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
+            if runs[0].poisson_stats()[1] != solves:
+                times[shape] = dt
+                break
+    best = min(times, key=times.get)
+    ctx.set_sor_shape(best)
+    return best, times

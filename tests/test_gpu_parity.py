@@ -398,6 +398,34 @@ def test_sor_solve_counts_and_bits(ctx, limit, acc):
     d.close()
 
 
+@pytest.mark.parametrize("shape", [0, 1])
+@pytest.mark.parametrize("limit,acc", [(13, 1e-12), (3000, 1e-9)])
+def test_sor_either_tile_shape_gives_the_same_solve(ctx, shape, limit, acc):
+    """The context's SOR tile shape forced to either build-time shape (12 half-sweeps
+    per launch on 104x24-point tiles, or 8 on 112x32): the restatement's iteration
+    count and Psi bits, as with the shape chosen by size."""
+    g = gun_grid(modes=abi.TM_FIELDS | abi.RADIAL_INTERP | abi.LONGIT_INTERP)
+    g.left_bc = abi.DIRICHLET
+    f = gun_flags(g)
+    o, d = pair(ctx, g, f)
+    rng = np.random.default_rng(limit + 1)
+    psi0 = rng.standard_normal(g.shape)
+    rhs = rng.standard_normal(g.shape) * 1e3
+    o.set_field("Psi", psi0)
+    o.set_field("Q", rhs)
+    d.upload("Psi", psi0)
+    d.upload("Q", rhs)
+    ctx.set_sor_shape(shape)
+    try:
+        it_d = d.solve_poisson_cyl(0, acc, limit)
+    finally:
+        ctx.set_sor_shape(-1)
+    it_o, _ = o.solve_poisson_cyl("Psi", "Q", acc, limit)
+    assert it_o == it_d
+    assert_fields(o, d, ["Psi"])
+    d.close()
+
+
 @pytest.mark.parametrize("case", ["tiny", "huge", "patch", "zero"])
 def test_sor_division_outside_fast_set(ctx, case):
     """Residuals whose magnitudes leave the fast division's range (below 1e-280 or above
