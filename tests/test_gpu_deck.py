@@ -64,32 +64,37 @@ def compare_lost(path, lost_rows):
     return sum(len(l) for l in want)
 
 
-def test_dc_gun_deck(ctx_free, tmp_path):
-    # the DC-gun deck on a 0.2 mm mesh, 20 k macroparticles, 700 steps, with field saving
+@pytest.mark.parametrize("nz,nr,N,dt,steps,interval", [
+    (151, 61, 20000, 4e-13, 700, 10),        # 0.2 mm mesh, 20 k macroparticles, 700 steps
+    (601, 241, 200000, 1e-13, 1600, 100),    # the serving stream's DC gun: 0.05 mm, 200 k, its own dt
+])
+def test_dc_gun_deck(ctx_free, tmp_path, nz, nr, N, dt, steps, interval):
+    """The DC-gun deck with field saving; on the full 0.05 mm mesh with all 200 k
+    macroparticles over the first 1600 steps (the bunch reaches the first screen),
+    with a Poisson correction every 100 steps so the restatement keeps up."""
     for f in ("dc_gun.geo", "flat_top_10ps.sdds"):
         shutil.copy(os.path.join(DECKS, f), tmp_path / f)
     text = open(os.path.join(DECKS, "dc_gun.spiffe")).read()
-    N = 20000
     epm = 50e-12 / K.e_mks / N
-    dt = 4e-13
-    steps = 700
-    text = text.replace("nz = 601, nr = 241", "nz = 151, nr = 61")
+    text = text.replace("nz = 601, nr = 241", "nz = %d, nr = %d" % (nz, nr))
     text = text.replace("electrons_per_macroparticle = 1560.5454", "electrons_per_macroparticle = %r" % epm)
+    text = text.replace("step_interval = 10,", "step_interval = %d," % interval)
     text = text.replace("dt_integration = 1e-13, finish_time = 400e-12",
                         "dt_integration = %r, finish_time = %r" % (dt, (steps + 0.5) * dt))
     text = text.replace("&integrate", '&define_field_saving filename = "%%s.fsave", time_interval = %r, '
                                       'save_before_exiting = 1 &end\n\n&integrate' % (100 * dt))
     (tmp_path / "dc.spiffe").write_text(text)
-    stats = deck.run_deck(str(tmp_path / "dc.spiffe"), max_particles=30000, compat=False)
+    cap_particles = int(1.5 * N)
+    stats = deck.run_deck(str(tmp_path / "dc.spiffe"), max_particles=cap_particles, compat=False)
     assert stats["steps"] == steps
 
     # the same run from the oracle
-    g = Grid(nz=151, nr=61, zmin=0.0, zmax=0.030, rmax=0.012, dt=dt,
+    g = Grid(nz=nz, nr=nr, zmin=0.0, zmax=0.030, rmax=0.012, dt=dt,
              modes=abi.TM_FIELDS | abi.RADIAL_INTERP | abi.LONGIT_INTERP | abi.SPACE_CHARGE)
-    f, psi = oracle_flags(151, 61, 0.0, 0.030, 0.012, "dc_gun.geo")
-    o = O.OracleSim(g, f, max_particles=30000)
+    f, psi = oracle_flags(nz, nr, 0.0, 0.030, 0.012, "dc_gun.geo")
+    o = O.OracleSim(g, f, max_particles=cap_particles)
     o.set_field("PsiImposed", psi)
-    o.setup_poisson_correction(step_interval=10, accuracy=1e-6, maximum_iterations=2000)
+    o.setup_poisson_correction(step_interval=interval, accuracy=1e-6, maximum_iterations=2000)
     o.define_cathode(profile=profile("flat_top_10ps.sdds"), z_position=0.0, outer_radius=3e-3,
                      current_density=1.768388e5, electrons_per_macroparticle=epm, initial_pz=1e-3,
                      zoned_emission=0, random_number_seed=987654321)
@@ -111,7 +116,10 @@ def test_dc_gun_deck(ctx_free, tmp_path):
     n1 = compare_screen(str(tmp_path / "dc.sc1"), rows_by_screen[0])
     n2 = compare_screen(str(tmp_path / "dc.sc2"), rows_by_screen[1])
     nl = compare_lost(str(tmp_path / "dc.lost"), lost_pages)
-    assert n1 > 0.9 * N and n2 > 0 and nl > 0
+    if N == 20000:
+        assert n1 > 0.9 * N and n2 > 0 and nl > 0
+    else:
+        assert n1 > 0
     assert stats["screen_rows"] == n1 + n2 and stats["lost_rows"] == nl
     fs = sdds_ref.read(str(tmp_path / "dc.fsave"))["pages"]
     assert len(fs) == len(saves) == stats["field_saves"]
