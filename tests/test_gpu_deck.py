@@ -131,12 +131,13 @@ def test_dc_gun_deck(ctx_free, tmp_path, nz, nr, N, dt, steps, interval):
 
 
 def test_pillbox_deck_antenna_and_emission(ctx_free, tmp_path):
-    # the plain-method deck, cut to 2400 steps: the antenna rings the cavity up,
-    # the cathode starts at step 2200
+    # the plain-method deck whole, 5200 steps: the antenna rings the cavity up
+    # (200-step ramp, 2000 field-only steps), the cathode emits from step 2200,
+    # 3000 steps with the beam; every screen page and lost row bit for bit
     for fn in ("pillbox.geo", "ramp_200dt.wf", "gauss_2ps.sdds"):
         shutil.copy(os.path.join(DECKS, fn), tmp_path / fn)
     shutil.copy(os.path.join(DECKS, "pillbox.spiffe"), tmp_path / "pb.spiffe")
-    steps = 2400
+    steps = 5200
     stats = deck.run_deck(str(tmp_path / "pb.spiffe"), max_particles=16000, max_steps=steps)
     assert stats["steps"] == steps and stats["emitted"] > 1000
     dt = 1.061394451537358e-12
@@ -159,13 +160,14 @@ def test_pillbox_deck_antenna_and_emission(ctx_free, tmp_path):
         t = step * dt + dt / 2
         J = lib.oracle_antenna_current(t_wf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                        W.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(t_wf),
-                                       ctypes.byref(last), 1.0, 2.8685e9, 0.0, t, ctypes.byref(errc))
+                                       ctypes.byref(last), 1e9, 2.8685e9, 90.0, t, ctypes.byref(errc))
         rows, lost = o.step(lines=[(0, 0, 0, iend, J)], screens=[(0.012, 1)], compat=True)
         rows_by_step.append((step, rows))
         lost_pages.append(lost)
-    compare_screen(str(tmp_path / "pb.sc1"), rows_by_step)
-    compare_lost(str(tmp_path / "pb.lost"), lost_pages)
+    n_screen = compare_screen(str(tmp_path / "pb.sc1"), rows_by_step)
+    n_lost = compare_lost(str(tmp_path / "pb.lost"), lost_pages)
     assert stats["emitted"] == lib.oracle_cathodes_next_id(o.cathodes) - 1
+    assert n_screen > 9000, (n_screen, n_lost, stats)   # the bunch crosses the screen
 
 
 @pytest.fixture
