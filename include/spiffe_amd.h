@@ -144,7 +144,7 @@ typedef struct {
   double accuracy;
   double error_charge_threshold;
   int64_t maximum_iterations;
-  int64_t guess_code;   /* 0 none (warm start), 3 zero; 1 and 2 unsupported on device */
+  int64_t guess_code;   /* 0 none (warm start), 1 line charge, 2 point charge, 3 zero */
 } spf_poisson_params;
 
 /* ---- context ---- */

@@ -332,7 +332,10 @@ double oracle_antenna_current(const double *t, const double *W, int64_t n_pts, i
   return J;
 }
 
-/* Current deposition, src/space_charge.c:34-295 */
+/* Current deposition, src/space_charge.c:34-295.  With longitudinal smearing
+ * a particle whose iz is nz adds fp*Jm and fp*Jp to row nz+1, one past the
+ * reference's arrays (zero terms for Jz and Jr, whose fp is 0 there; for
+ * Jphi a particle beyond zmax); those writes are left out here. */
 void oracle_add_particle_currents(ofields *f, const obeam *b) {
   int64_t ip, iz, ir, nz = f->nz, nr = f->nr;
   double fp, fm, Jm, Jp, dz = f->dz, dr = f->dr;
@@ -390,11 +393,11 @@ void oracle_add_particle_currents(ofields *f, const obeam *b) {
         fp = 0;
       }
       A(f, f->Jz, iz, ir) += fm * Jm;
-      if (modes & OFL_LONGIT_SMEAR)
+      if ((modes & OFL_LONGIT_SMEAR) && iz + 1 <= nz)
         A(f, f->Jz, iz + 1, ir) += fp * Jm;
       if (modes & OFL_RADIAL_SMEAR) {
         A(f, f->Jz, iz, ir + 1) += fm * Jp;
-        if (modes & OFL_LONGIT_SMEAR)
+        if ((modes & OFL_LONGIT_SMEAR) && iz + 1 <= nz)
           A(f, f->Jz, iz + 1, ir + 1) += fp * Jp;
       }
     }
@@ -444,11 +447,11 @@ void oracle_add_particle_currents(ofields *f, const obeam *b) {
         fp = 0;
       }
       A(f, f->Jr, iz, ir) += fm * Jm;
-      if (modes & OFL_LONGIT_SMEAR)
+      if ((modes & OFL_LONGIT_SMEAR) && iz + 1 <= nz)
         A(f, f->Jr, iz + 1, ir) += fp * Jm;
       if (modes & OFL_RADIAL_SMEAR) {
         A(f, f->Jr, iz, ir + 1) += fm * Jp;
-        if (modes & OFL_LONGIT_SMEAR)
+        if ((modes & OFL_LONGIT_SMEAR) && iz + 1 <= nz)
           A(f, f->Jr, iz + 1, ir + 1) += fp * Jp;
       }
     }
@@ -499,11 +502,11 @@ void oracle_add_particle_currents(ofields *f, const obeam *b) {
         fp = 0;
       }
       A(f, f->Jphi, iz, ir) += fm * Jm;
-      if (modes & OFL_LONGIT_SMEAR)
+      if ((modes & OFL_LONGIT_SMEAR) && iz + 1 <= nz)
         A(f, f->Jphi, iz + 1, ir) += fp * Jm;
       if (modes & OFL_RADIAL_SMEAR) {
         A(f, f->Jphi, iz, ir + 1) += fm * Jp;
-        if (modes & OFL_LONGIT_SMEAR)
+        if ((modes & OFL_LONGIT_SMEAR) && iz + 1 <= nz)
           A(f, f->Jphi, iz + 1, ir + 1) += fp * Jp;
       }
     }

@@ -286,7 +286,8 @@ int64_t spf_run_footprint(const spf_run_desc *d) {
   int64_t part_bytes = n * (16 * 8 + 4 + 2 + 4);                         // particle SoA
   int64_t event_bytes = n * (4 + 1 + 7 * 8 + 4 + 8) + (n / 256 + 2) * 4 +
                         (int64_t)(kMaxScreens + 1) * rows_tiles(n) * 8;   // staging, lists, rows pipeline
-  int64_t sort_bytes = 3 * n * (4 + 4 + 8 + 8 + 8 + 1 + 1);  // keys, contributions (both sorted), flags, sort tables
+  // keys, contributions (both sorted), flags, sort tables: three components, four slots each when smeared
+  int64_t sort_bytes = 3 * spf::deposit_slots_per_particle(d->modes) * n * (4 + 4 + 8 + 8 + 8 + 1 + 1);
   int64_t ring = default_srow_cap(d) * (int64_t)sizeof(spf_screen_row) + default_lrow_cap(d) * (int64_t)sizeof(spf_lost_row);
   // cell-sort runs: a second particle set and the reduce deposition's counts
   int64_t cellsort_bytes = d->cell_sort > 0 ? n * (16 * 8 + 4 + 2 + 4) + (n / 1024 + 2) * 12 : 0;
@@ -300,8 +301,6 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   if (!d->metal_flags) return fail(c, SPF_EINVAL, "metal_flags is required");
   if (d->n_screens < 0 || d->n_screens > kMaxScreens) return fail(c, SPF_EINVAL, "n_screens must be in [0, 24]");
   if (d->n_screens > 0 && (!d->screen_z || !d->screen_direction)) return fail(c, SPF_EINVAL, "screen arrays missing");
-  if (d->modes & (SPF_RADIAL_SMEAR | SPF_LONGIT_SMEAR))
-    return fail(c, SPF_EUNSUPPORTED, "area-weighted (smeared) deposition is not implemented on the device");
   if (d->max_particles < 0 || d->max_particles > 0x20000000LL) return fail(c, SPF_EINVAL, "max_particles out of range");
   if ((int64_t)(d->nz + 1) * (d->nr + 1) >= (1LL << 29)) return fail(c, SPF_EINVAL, "grid too large");
   if (d->cell_sort < 0) return fail(c, SPF_EINVAL, "cell_sort must be >= 0");
@@ -365,8 +364,9 @@ int spf_run_create(spf_ctx *c, const spf_run_desc *d, int32_t *run_out) {
   R->srow_cap = default_srow_cap(d);
   R->lrow_cap = default_lrow_cap(d);
   ALLOC(R->srows, R->srow_cap); ALLOC(R->lrows, R->lrow_cap);
-  // ordered scatter-add: three components (Jz, Jr, Jphi) per particle
-  const int64_t cap3 = 3 * cap;
+  // ordered scatter-add: three components (Jz, Jr, Jphi) per particle, four
+  // key slots each with area-weighted deposition
+  const int64_t cap3 = 3 * deposit_slots_per_particle(d->modes) * cap;
   ALLOC(R->keys, cap3); ALLOC(R->skeys, cap3); ALLOC(R->cz, cap3); ALLOC(R->scz, cap3);
   ALLOC(R->vals, cap3); ALLOC(R->long_list, cap3 / 256 + 1); ALLOC(R->gflag, cap3);
   R->sort_tmp_bytes = radix_sort_tmp_bytes(cap3);

@@ -349,7 +349,8 @@ int error_charge_skips(spf_ctx *c, Run *R, bool space_charge, double thr, bool *
   memcpy(h, c->pinned + 52, sizeof(h));
   const double Te = h[0], Ae = h[1];
   const double Tr = space_charge ? -h[2] : 0.0, Ar = space_charge ? h[3] : 0.0;
-  const double de = order_margin((int64_t)R->d.nz * R->d.nr, Ae), dr = order_margin(R->np_ub, Ar);
+  const double de = order_margin((int64_t)R->d.nz * R->d.nr, Ae),
+               dr = order_margin(R->np_ub * deposit_slots_per_particle(R->d.modes), Ar);
   const double big = 0x1p1000, slack = 1e-12;
   // 1: skip, 0: correct, -1: undecided
   int dec = -1;
@@ -405,7 +406,13 @@ int poisson_pre(spf_ctx *c, Run *R, int space_charge, bool *solve) {
   if (!poisson_due(R)) return SPF_OK;
   const spf_poisson_params &p = R->pc;
   hipStream_t s = c->stream;
+  const int nz = R->d.nz, nr = R->d.nr;
   HIPCALL(c, launch_gauss_q(*R, s));
+  // the point-charge guess's "centroid": the sum of the Gauss-law charges
+  // times their z, ir outer and iz inner as it is formed at :176 (metal and
+  // Neumann points hold +0 and add nothing)
+  if (p.guess_code == 2)
+    HIPCALL(c, launch_serial_moment_ir_iz(R->Q, R->ldz, nz, nr, R->d.dz, R->d.zmin, R->sor_sums + 13, s));
   const double thr = p.error_charge_threshold;
   if (space_charge) HIPCALL(c, launch_q_subtract(*R, s, thr < 0 ? R->sor_sums + 4 : nullptr));
   if (thr != 0) {
@@ -414,8 +421,28 @@ int poisson_pre(spf_ctx *c, Run *R, int space_charge, bool *solve) {
     if (rc) return rc;
     if (skip) return SPF_OK;
   }
+  // the guesses take Qerror, the serial sum of :246-250 (iz outer)
+  if (p.guess_code == 1 || p.guess_code == 2)
+    HIPCALL(c, launch_serial_sum_iz_ir(R->Q, R->ldz, nz, nr, 0, R->sor_sums + 12, s));
   HIPCALL(c, launch_q_to_rhs(*R, s));
-  if (p.guess_code == 3) HIPCALL(c, launch_zero_psi(*R, s));
+  if (p.guess_code == 3) {
+    HIPCALL(c, launch_zero_psi(*R, s));
+  } else if (p.guess_code == 2) {
+    HIPCALL(c, launch_point_guess(*R, R->sor_sums + 12, R->sor_sums + 13, s));
+  } else if (p.guess_code == 1) {
+    // line charge, :575-582: Qerror/(2 pi eps0) * log(r) per row, with the
+    // host's log (the reference's libm), r = 0.5*dr on the axis row
+    HIPCALL(c, hipMemcpyAsync(c->pinned + 54, R->sor_sums + 12, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCALL(c, hipStreamSynchronize(s));
+    double charge;
+    memcpy(&charge, c->pinned + 54, sizeof(double));
+    std::vector<double> row(nr);
+    for (int ir = 0; ir < nr; ir++)
+      row[ir] = charge / (2 * 3.141592653589793238462643 * R->d.epsilon_o) * std::log((ir ? (double)ir : 0.5) * R->d.dr);
+    HIPCALL(c, hipMemcpyAsync(R->stage, row.data(), nr * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCALL(c, launch_line_guess(*R, R->stage, s));
+    HIPCALL(c, hipStreamSynchronize(s));
+  }
   *solve = true;
   return SPF_OK;
 }
@@ -453,8 +480,6 @@ int spf_poisson_setup(spf_ctx *c, int32_t run, const spf_poisson_params *p) {
   if (p->accuracy <= 0) return fail(c, SPF_EINVAL, "accuracy <= 0 for poisson_correction");
   if (p->maximum_iterations <= 0) return fail(c, SPF_EINVAL, "maximum iterations <= 0 for poisson_correction");
   if (p->guess_code < 0 || p->guess_code > 3) return fail(c, SPF_EINVAL, "unknown guess type for poisson_correction");
-  if (p->guess_code == 1 || p->guess_code == 2)
-    return fail(c, SPF_EUNSUPPORTED, "line-charge and point-charge guesses are not implemented on the device");
   if (!(R->d.modes & SPF_TM_FIELDS)) return fail(c, SPF_EINVAL, "TM fields must be enabled in order to do poisson correction");
   R->pc = *p;
   R->pc_set = true;
@@ -607,7 +632,8 @@ int spf_step_batch(spf_ctx *c, int32_t n_runs, const int32_t *run_ids) {
   PhaseTimer pt(c, SPF_T_STEP);
   // per run, in the order of src/integrate.c:134-183: the ramped imposed field,
   // the time, emission, J reset and deposition; the cell-sort runs with
-  // particles deposit together in one set of launches
+  // particles (and without smearing, which takes the ordered deposition)
+  // deposit together in one set of launches
   std::vector<Run *> together;
   for (int32_t i = 0; i < n_runs; i++) {
     Run *R = runs[i];
@@ -617,7 +643,8 @@ int spf_step_batch(spf_ctx *c, int32_t n_runs, const int32_t *run_ids) {
       if ((rc = cathode_emit_step(c, run_ids[i], k, nullptr))) return rc;
     if (R->cell_sort > 0 && R->time_step % R->cell_sort == 0 && (rc = spf_particles_sort(c, run_ids[i]))) return rc;
     if (R->d.modes & SPF_SPACE_CHARGE) {
-      if (R->cell_sort > 0 && R->np_ub > 0 && (R->d.modes & (SPF_TM_FIELDS | SPF_TE_FIELDS))) {
+      if (R->cell_sort > 0 && R->np_ub > 0 && (R->d.modes & (SPF_TM_FIELDS | SPF_TE_FIELDS)) &&
+          deposit_slots_per_particle(R->d.modes) == 1) {
         together.push_back(R);
         continue;
       }

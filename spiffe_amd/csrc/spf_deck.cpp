@@ -185,7 +185,6 @@ struct Deck {
     if (up == SPF_DIRICHLET) bomb("Dirichlet boundary conditions are not implemented for the upper boundary");
     if (ri < 0) bomb("unrecognized boundary condition for right boundary");
     if (le < 0) bomb("unrecognized boundary condition for left boundary");
-    if (rs || ls) bomb("area-weighted (smeared) deposition is not implemented on the device", SPF_EUNSUPPORTED);
     d.nz = (int32_t)nz;
     d.nr = (int32_t)nr;
     d.zmin = zmin;
@@ -199,8 +198,9 @@ struct Deck {
     d.left_bc = le;
     tm = !xtm;
     this->te = te != 0;
+    // src/geometry.c:79
     d.modes = (rad ? SPF_RADIAL_INTERP : 0) | (lon ? SPF_LONGIT_INTERP : 0) | (tm ? SPF_TM_FIELDS : 0) |
-              (te ? SPF_TE_FIELDS : 0);
+              (te ? SPF_TE_FIELDS : 0) | (rs ? SPF_RADIAL_SMEAR : 0) | (ls ? SPF_LONGIT_SMEAR : 0);
     for (int k = 0; k < 6; k++) d.turn_off[k] = (int16_t)off[k];
     // the boundary file
     std::string path = compose(boundary, root, dir);

@@ -56,6 +56,8 @@ struct KeyArgs {
   int need_v[3];                // vz, vr, vphi are read (a component uses them)
   double *pvals;                // reduce path: first component's value per particle (or null)
   int32_t *wg_count;            // reduce path: partials per workgroup
+  int smear_r, smear_z;         // FL_RADIAL_SMEAR, FL_LONGIT_SMEAR
+  int tpp;                      // key slots per particle and component: 4 when smeared, else 1
 };
 
 // NGP value of the Jz/Jphi loops for cell row ir (0 <= ir <= nr-2), :92-100, :254-262
@@ -206,6 +208,195 @@ __device__ inline uint32_t part_contrib(const KeyArgs &a, const PartCells &pc, c
   return (cell << 2) | (uint32_t)k;
 }
 
+// ---- area-weighted ("smeared") deposition, src/space_charge.c:61-293 and
+// src/poisson_correction.c:188-239 with FL_RADIAL_SMEAR / FL_LONGIT_SMEAR.
+// A particle adds to up to four points per component, in the reference's
+// order (iz, ir), (iz+1, ir), (iz, ir+1), (iz+1, ir+1).  Its key slots are
+// laid out particle-major (slot 4*ip + t of the component's block), so the
+// stable sort keeps each point's terms in particle index order.  Every term
+// the reference adds inside the arrays is kept, zeros included, so signed
+// zeros come out as the reference's.  The one term the reference writes past
+// its arrays (row nz+1, from a particle with iz == nz) is left out.
+
+// C's conversion to long of v when -1 < v < lim (the conversion truncates
+// toward zero), else -1: every index the callers accept lies below lim
+__device__ inline long long trunc_in(double v, double lim) { return (v > -1.0 && v < lim) ? (long long)v : -1; }
+
+struct Smear4 {
+  long long iz[4], ir[4];   // iz[t] = -1: no term
+  double v[4];
+};
+
+// component c (COMP_JZ, COMP_JR, COMP_JPHI) of particle pv, src/space_charge.c:61-139, :141-215, :222-293
+__device__ inline void smear_current(const KeyArgs &a, const PartVals &pv, int c, Smear4 &s) {
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    s.iz[t] = -1;
+    s.ir[t] = 0;
+    s.v[t] = 0.0;
+  }
+  if (!pv.live) return;
+  const bool RS = a.smear_r, LS = a.smear_z;
+  const long long nz = a.nz, nr = a.nr;
+  const double x = (pv.z - a.zmin) / a.dz, y = pv.r / a.dr;
+  long long iz, ir;
+  double vel;
+  if (c == COMP_JZ) {
+    ir = trunc_in(RS ? y : y + 0.5, nr + 1.0);
+    iz = trunc_in(LS ? x + 0.5 : x + 1.0, nz + 1.0);
+    vel = pv.vz;
+  } else if (c == COMP_JR) {
+    ir = trunc_in(RS ? y + 0.5 : y + 1.0, nr + 1.0);
+    iz = trunc_in(LS ? x : x + 0.5, nz + 1.0);
+    vel = pv.vr;
+  } else {
+    ir = trunc_in(RS ? y : y + 0.5, nr + 1.0);
+    iz = trunc_in(LS ? x : x + 0.5, nz + 1.0);
+    vel = pv.vphi;
+  }
+  if (iz < 0 || iz > nz || ir < 0 || ir > (c == COMP_JR ? nr - 1 : nr)) return;
+  const double QVG = vel * (pv.Q * a.one_over_piDzDr2);
+  // radial weights and the current densities of the rows below and above
+  double fp, fm, Jm, Jp;
+  if (c == COMP_JR) {
+    if (RS) {
+      fp = (y + 0.5) - (double)ir;
+      fm = 1 - fp;
+    } else {
+      fm = 1;
+      fp = 0;
+    }
+    if (ir == 0) {
+      Jm = 0;
+      Jp = QVG;
+    } else if (ir < nr - 1) {
+      Jm = fm * QVG / ((double)ir * (double)ir - (double)(ir - 1) * (double)(ir - 1));
+      Jp = fp * QVG / ((double)(ir + 1) * (double)(ir + 1) - (double)ir * (double)ir);
+    } else {
+      Jm = 0;
+      ir--;
+      Jp = QVG / ((double)(ir + 1) * (double)(ir + 1) - (double)ir * (double)ir);
+    }
+  } else {
+    if (RS) {
+      fp = y - (double)ir;
+      fm = 1 - fp;
+    } else {
+      fm = 1;
+      fp = 0;
+    }
+    if (ir == 0) {
+      Jm = 4 * fm * QVG;
+      Jp = fp * QVG / (1.5 * 1.5 - 0.5 * 0.5);
+    } else if (ir < nr - 1) {
+      Jm = fm * QVG / ((ir + 0.5) * (ir + 0.5) - (ir - 0.5) * (ir - 0.5));
+      Jp = fp * QVG / ((ir + 1.5) * (ir + 1.5) - (ir + 0.5) * (ir + 0.5));
+    } else {
+      Jm = 0;
+      Jp = fp * QVG / ((ir + 0.5) * (ir + 0.5) - (ir - 0.5) * (ir - 0.5));
+      ir--;
+    }
+  }
+  // longitudinal weights of the points to the left and right
+  double gm = 1, gp = 0;
+  if (LS) {
+    if (c == COMP_JZ) {
+      if (iz == 0) {
+        gp = 1;
+        gm = 0;
+      } else if (iz == nz) {
+        gm = 1;
+        gp = 0;
+      } else {
+        gp = (x + 0.5) - (double)iz;
+        gm = 1 - gp;
+      }
+    } else if (c == COMP_JR) {
+      if (iz == nz) {
+        gm = 1;
+        gp = 0;
+      } else {
+        gp = x - (double)iz;
+        gm = 1 - gp;
+      }
+    } else {
+      if (iz == nz - 1) {
+        gp = 0;
+        gm = 1 - gp;
+      } else {
+        gp = x - (double)iz;
+        gm = 1 - gp;
+      }
+    }
+  }
+  s.iz[0] = iz;
+  s.ir[0] = ir;
+  s.v[0] = gm * Jm;
+  if (LS && iz + 1 <= nz) {
+    s.iz[1] = iz + 1;
+    s.ir[1] = ir;
+    s.v[1] = gp * Jm;
+  }
+  if (RS) {
+    s.iz[2] = iz;
+    s.ir[2] = ir + 1;
+    s.v[2] = gm * Jp;
+    if (LS && iz + 1 <= nz) {
+      s.iz[3] = iz + 1;
+      s.ir[3] = ir + 1;
+      s.v[3] = gp * Jp;
+    }
+  }
+}
+
+// the charge a particle takes off each of its four points, src/poisson_correction.c:188-239:
+// v[t] = Q*weight (the reference subtracts it and adds it to Qreal); iz[t] = -1
+// where the point is metal, on a Neumann edge, or the particle outside
+__device__ inline void smear_charge(const KeyArgs &a, const PartVals &pv, Smear4 &s) {
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    s.iz[t] = -1;
+    s.ir[t] = 0;
+    s.v[t] = 0.0;
+  }
+  if (!pv.live) return;
+  const bool RS = a.smear_r, LS = a.smear_z;
+  const long long nz = a.nz, nr = a.nr;
+  const double x = (pv.z - a.zmin) / a.dz, y = pv.r / a.dr;
+  const long long iz = trunc_in(LS ? x : x + 0.5, nz + 1.0), ir = trunc_in(RS ? y : y + 0.5, nr + 1.0);
+  if (iz < 0 || iz >= nz || ir < 0 || ir >= nr) return;
+  double fa, fb, fl, fr;
+  if (RS && ir != nr - 1) {
+    fa = y - (double)ir;
+    fb = 1 - fa;
+  } else {
+    fb = 1;
+    fa = 0;
+  }
+  if (LS && iz != nz - 1) {
+    fr = x - (double)iz;
+    fl = 1 - fr;
+  } else {
+    fl = 1;
+    fr = 0;
+  }
+  const bool L = a.left_neu, R = a.right_neu, U = a.upper_neu;
+  const long long pz[4] = {iz, iz + 1, iz, iz + 1}, pr[4] = {ir, ir, ir + 1, ir + 1};
+  const bool edge[4] = {(iz == 0 && L) || (iz >= nz - 1 && R) || (ir >= nr - 1 && U),
+                        (iz == 0 && L) || (iz >= nz && R) || (ir >= nr - 1 && U),
+                        (iz == 0 && L) || (iz >= nz - 1 && R) || (ir >= nr && U),
+                        (iz == 0 && L) || (iz >= nz && R) || (ir >= nr && U)};
+  const double w[4] = {pv.Q * fb * fl, pv.Q * fb * fr, pv.Q * fa * fl, pv.Q * fa * fr};
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int16_t f = a.flags[fix(a.ldz, (int)pz[t], (int)pr[t])];
+    if (edge[t] || (f & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE))) continue;
+    s.iz[t] = pz[t];
+    s.ir[t] = pr[t];
+    s.v[t] = w[t];
+  }
+}
+
 // the targets' reset (src/integrate.c:169 zeroes J before the deposition),
 // done by the key kernels to save a launch; the folds that add into them come later
 __device__ inline void clear_targets(const KeyArgs &a, int64_t ip, int64_t stride) {
@@ -263,6 +454,42 @@ __global__ __launch_bounds__(256) void key_kernel(KeyArgs a) {
     }
   }
   if (!a.krange) return;   // the sort covers the whole key space
+  key_range_fold<4>(a.krange, kmin, kmax);
+}
+
+// the key kernel of area-weighted deposition: four slots per particle and
+// component, particle-major (slot 4*ip + t of component k's block)
+__global__ __launch_bounds__(256) void key_smear_kernel(KeyArgs a) {
+  int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip == 0) a.counters[9] = 0;
+  clear_targets(a, ip, (int64_t)gridDim.x * blockDim.x);
+  uint32_t kmin = 0xffffffffu, kmax = 0u;
+  const bool in = ip < a.n_ub;
+  int64_t np = in ? *a.np : 0;
+  const PartVals pv = load_part(a, ip, np);
+  for (int k = 0; in && k < a.ncomp; k++) {
+    const int c = a.comp[k];
+    Smear4 sm;
+    if (c == COMP_Q)
+      smear_charge(a, pv, sm);
+    else
+      smear_current(a, pv, c, sm);
+    const int64_t base = (int64_t)k * 4 * a.n_ub + 4 * ip;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      uint32_t key = a.discard;
+      double v = 0.0;
+      if (sm.iz[t] >= 0) {
+        key = ((uint32_t)(sm.ir[t] * (a.nz + 1) + sm.iz[t]) << 2) | (uint32_t)k;
+        v = c == COMP_Q ? -sm.v[t] : sm.v[t];
+        kmin = min(kmin, key);
+        kmax = max(kmax, key);
+      }
+      a.keys[base + t] = key;
+      a.vals[base + t] = v;
+    }
+  }
+  if (!a.krange) return;
   key_range_fold<4>(a.krange, kmin, kmax);
 }
 
@@ -852,6 +1079,8 @@ __global__ __launch_bounds__(256) void partial_pack_multi_kernel(const ReduceRun
 
 }  // namespace
 
+int deposit_slots_per_particle(uint32_t modes) { return (modes & (SPF_RADIAL_SMEAR | SPF_LONGIT_SMEAR)) ? 4 : 1; }
+
 int deposit_cell_bits(int nz, int nr) {
   uint64_t cells = (uint64_t)(nz + 1) * (uint64_t)(nr + 1);
   int b = 1;
@@ -900,6 +1129,14 @@ static KeyArgs make_key_args(Run &R, const int *comps, int ncomp, double *const 
   for (int i = 0; i < 3; i++) k.need_v[i] = 0;
   for (int i = 0; i < ncomp; i++)
     if (comps[i] <= COMP_JPHI) k.need_v[comps[i]] = 1;
+  k.smear_r = (R.d.modes & SPF_RADIAL_SMEAR) != 0;
+  k.smear_z = (R.d.modes & SPF_LONGIT_SMEAR) != 0;
+  k.tpp = deposit_slots_per_particle(R.d.modes);
+  // smeared charge removal keeps its zero-weight terms in the ordered fold
+  if (k.tpp != 1) {
+    k.zero_terms = 0;
+    k.zero_target = nullptr;
+  }
   return k;
 }
 
@@ -963,12 +1200,16 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
     return launch_fill3(ncomp > 0 ? targets[0] : nullptr, ncomp > 1 ? targets[1] : nullptr,
                         ncomp > 2 ? targets[2] : nullptr, R.cells, 0.0, s);
   if (n_ub == 0 || ncomp == 0) return hipSuccess;
-  const int64_t n = n_ub * ncomp;
+  const int tpp = deposit_slots_per_particle(R.d.modes);
+  const int64_t n = n_ub * ncomp * tpp;
   const int cell_bits = deposit_cell_bits(R.d.nz, R.d.nr);
   const int key_bits = cell_bits + 2;
   const uint32_t discard = (1u << key_bits) - 1u;   // all-ones cell with slot 3: never a real key
   KeyArgs k = make_key_args(R, comps, ncomp, targets, clear_first, zero_terms, discard, cell_bits);
-  if (R.cell_sort > 0) return reduce_scatter(R, k, ncomp, targets, s, clear_first, sums, key_bits, discard);
+  // cell-sort runs use the reduce deposition, except when smeared: a particle
+  // then has several targets per component, and the ordered path serves
+  if (R.cell_sort > 0 && tpp == 1)
+    return reduce_scatter(R, k, ncomp, targets, s, clear_first, sums, key_bits, discard);
   // enough workgroups for the particles and, when it clears the targets, for
   // a wide grid-stride clear (workgroups past n_ub only clear)
   int64_t kblocks = (n_ub + 255) / 256;
@@ -980,12 +1221,16 @@ static hipError_t ordered_scatter(Run &R, const int *comps, int ncomp, double *c
   const bool ranged = key_bits > 21;
   k.krange = ranged ? reinterpret_cast<unsigned long long *>(R.krange) : nullptr;
   if (ranged) hipLaunchKernelGGL(key_range_reset_kernel, dim3(1), dim3(kRangeSlots), 0, s, k.krange);
-  hipLaunchKernelGGL(key_kernel, dim3((unsigned)kblocks), dim3(256), 0, s, k);
+  if (tpp == 4)
+    hipLaunchKernelGGL(key_smear_kernel, dim3((unsigned)kblocks), dim3(256), 0, s, k);
+  else
+    hipLaunchKernelGGL(key_kernel, dim3((unsigned)kblocks), dim3(256), 0, s, k);
   if (ranged) hipLaunchKernelGGL(key_range_final_kernel, dim3(1), dim3(kRangeSlots), 0, s, k.krange);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  // the first component's values in particle order, before the sort reuses their buffer
-  if (sums && (e = launch_sum2_vec(R.cz, n_ub, R.red, R.red_n, sums, s)) != hipSuccess) return e;
+  // the first component's values in particle order (tpp per particle), before
+  // the sort reuses their buffer
+  if (sums && (e = launch_sum2_vec(R.cz, n_ub * tpp, R.red, R.red_n, sums, s)) != hipSuccess) return e;
   const uint64_t *range = ranged ? reinterpret_cast<const uint64_t *>(R.krange + 2 * kRangeSlots) : nullptr;
   uint32_t *sk = R.keys;
   double *sv = R.cz;
@@ -1153,36 +1398,58 @@ hipError_t launch_q_subtract(Run &R, hipStream_t s, double *qreal_sums) {
 
 namespace {
 
-// Qreal of src/poisson_correction.c:221-224 (no smearing: fb = fl = 1) summed in
-// particle order from +0, one rounded addition at a time; the zero-weight
-// terms of :226-239 add +-0, which never changes a sum that starts at +0.
+// Qreal of src/poisson_correction.c:220-239 summed in particle order from +0,
+// one rounded addition at a time.  Without smearing fb = fl = 1 and the other
+// three terms weigh zero: they add +-0, which never changes a sum that
+// starts at +0.  With smearing every unblocked term is added, in the order
+// (iz, ir), (iz+1, ir), (iz, ir+1), (iz+1, ir+1).
 __global__ __launch_bounds__(64) void qreal_serial_kernel(KeyArgs a, double *out) {
-  __shared__ double buf[64];
+  __shared__ double buf[64 * 4];
+  __shared__ unsigned char tk[64 * 4];
   const int lane = threadIdx.x;
   const int64_t np = *a.np;
+  const bool smear = a.smear_r || a.smear_z;
   double s = 0.0;
   for (int64_t c = 0; c < np; c += 64) {
     const int64_t ip = c + lane;
-    double t = 0.0;
-    bool take = false;
-    if (ip < np && (a.status[ip] & SPF_PART_ACTIVE)) {
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    bool take[4] = {false, false, false, false};
+    if (smear) {
+      PartVals pv{false, 0, 0, 0, 0, 0, 0};
+      if (ip < np) {
+        pv.live = (a.status[ip] & SPF_PART_ACTIVE) != 0;
+        pv.Q = a.Q[ip];
+        pv.z = a.z[ip];
+        pv.r = a.r[ip];
+      }
+      Smear4 sm;
+      smear_charge(a, pv, sm);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        take[k] = sm.iz[k] >= 0;
+        t[k] = sm.v[k];
+      }
+    } else if (ip < np && (a.status[ip] & SPF_PART_ACTIVE)) {
       const double th = (a.z[ip] - a.zmin) / a.dz + 0.5, yh = a.r[ip] / a.dr + 0.5;
       if (th > -1.0 && th < a.nz + 1.0 && yh > -1.0 && yh < a.nr + 1.0) {
         const long long iz = (long long)th, ir = (long long)yh;
         if (iz >= 0 && iz < a.nz && ir >= 0 && ir < a.nr) {
           const int16_t f = a.flags[fix(a.ldz, (int)iz, (int)ir)];
-          take = !((f & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE)) || (iz == 0 && a.left_neu) ||
-                   (iz >= a.nz - 1 && a.right_neu) || (ir >= a.nr - 1 && a.upper_neu));
-          t = a.Q[ip] * 1.0 * 1.0;
+          take[0] = !((f & (SPF_FL_IS_SURFACE | SPF_FL_IS_INSIDE)) || (iz == 0 && a.left_neu) ||
+                      (iz >= a.nz - 1 && a.right_neu) || (ir >= a.nr - 1 && a.upper_neu));
+          t[0] = a.Q[ip] * 1.0 * 1.0;
         }
       }
     }
-    buf[lane] = take ? t : 0.0;
-    const unsigned long long m = __ballot(take);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      buf[4 * lane + k] = t[k];
+      tk[4 * lane + k] = take[k];
+    }
     __syncthreads();
     if (lane == 0)
-      for (int k = 0; k < 64; k++)
-        if ((m >> k) & 1ull) s += buf[k];
+      for (int k = 0; k < 64 * 4; k++)
+        if (tk[k]) s += buf[k];
     __syncthreads();
   }
   if (lane == 0) *out = s;
@@ -1207,6 +1474,9 @@ hipError_t launch_qreal_serial(Run &R, double *out, hipStream_t s) {
   k.right_neu = R.d.right_bc == SPF_NEUMANN;
   k.upper_neu = R.d.upper_bc == SPF_NEUMANN;
   k.np = R.counters;
+  k.smear_r = (R.d.modes & SPF_RADIAL_SMEAR) != 0;
+  k.smear_z = (R.d.modes & SPF_LONGIT_SMEAR) != 0;
+  k.tpp = deposit_slots_per_particle(R.d.modes);
   hipLaunchKernelGGL(qreal_serial_kernel, dim3(1), dim3(64), 0, s, k, out);
   return hipGetLastError();
 }

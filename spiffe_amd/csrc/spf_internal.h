@@ -249,6 +249,9 @@ hipError_t sort_particles(Run &R, hipStream_t s);
 // workgroups of the reduce deposition (one per 1024 particle slots)
 inline int64_t reduce_groups(int64_t n_ub) { return (n_ub + 1023) / 1024; }
 int deposit_cell_bits(int nz, int nr);
+// key slots per particle and component of the ordered deposition: 4 with
+// area-weighted (smeared) deposition, else 1
+int deposit_slots_per_particle(uint32_t modes);
 // stable radix sort of n (key, value) pairs on the low key_bits bits of the
 // keys; the input (k0, v0) and (k1, v1) alternate as pass input and output,
 // *kres/*vres name the pair holding the result
@@ -289,6 +292,9 @@ hipError_t launch_sum2_vec(const double *a, int64_t n, double *partials, int64_t
 hipError_t launch_serial_sum_iz_ir(const double *a, int ldz, int iz_n, int ir_n, int absval, double *out,
                                    hipStream_t s);
 hipError_t launch_serial_sum_ir_iz(const double *a, int ldz, int iz_n, int ir_n, double *out, hipStream_t s);
+// sum over ir < ir_n (outer), iz < iz_n of a[iz][ir] * (iz*dz + zmin), serially in that order
+hipError_t launch_serial_moment_ir_iz(const double *a, int ldz, int iz_n, int ir_n, double dz, double zmin, double *out,
+                                      hipStream_t s);
 // bound on how far sums of n terms with magnitude sum `mag` can differ between two summation orders
 double order_margin(int64_t n, double mag);
 hipError_t launch_q_to_rhs(Run &R, hipStream_t s);
@@ -329,6 +335,11 @@ hipError_t launch_sor_exact_sweep(Run &R, double *P, const double *rhs, int64_t 
 hipError_t launch_sor_state_clear(Run &R, hipStream_t s);
 hipError_t launch_field_from_psi(Run &R, int imposed, double field_factor, hipStream_t s);
 hipError_t launch_zero_psi(Run &R, hipStream_t s);
+// the guesses of src/poisson_correction.c:564-615 for iz < nz, ir < nr: Psi
+// row ir set to row[ir] (line charge, the rows formed on the host), or the
+// point-charge potential of charge *q at z = *centroid
+hipError_t launch_line_guess(Run &R, const double *row, hipStream_t s);
+hipError_t launch_point_guess(Run &R, const double *q, const double *centroid, hipStream_t s);
 // resolves, once per device, the address of the no-allocation solve's scratch array (spf_noalloc.hip)
 int noalloc_setup(int device);
 

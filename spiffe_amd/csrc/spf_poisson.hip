@@ -832,6 +832,28 @@ __global__ __launch_bounds__(256) void q_rhs_kernel(QArgs a) {
   a.Q[p] = -a.Q[p] / (a.volume[ir] * a.eps0);
 }
 
+// src/poisson_correction.c:580-587: Psi[iz][ir] = row[ir]
+__global__ __launch_bounds__(256) void line_guess_kernel(double *Psi, int nz, int nr, int ldz, const double *row) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)nz * nr) return;
+  const int ir = (int)(i / nz), iz = (int)(i % nz);
+  Psi[fix(ldz, iz, ir)] = row[ir];
+}
+
+// src/poisson_correction.c:588-598: the potential of charge *q at z = *centroid,
+// r = 0, at (iz*dz + zmin, (ir + 0.5)*dr); sqrt and the division are IEEE
+// correctly rounded, like the host's
+__global__ __launch_bounds__(256) void point_guess_kernel(double *Psi, int nz, int nr, int ldz, double dz, double dr,
+                                                          double zmin, double eps0, const double *q,
+                                                          const double *centroid) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)nz * nr) return;
+  const int ir = (int)(i / nz), iz = (int)(i % nz);
+  const double a = ((double)iz * dz + zmin) - *centroid, b = dr * ((double)ir + 0.5);
+  const double dist = __builtin_sqrt(a * a + b * b);
+  Psi[fix(ldz, iz, ir)] = *q / (4 * 3.141592653589793238462643 * eps0 * dist);
+}
+
 __global__ __launch_bounds__(256) void zero_psi_kernel(double *Psi, int nz, int nr, int ldz) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t n = (int64_t)nz * nr;
@@ -1136,6 +1158,20 @@ hipError_t launch_q_to_rhs(Run &R, hipStream_t s) {
   QArgs q = q_args(R);
   int64_t n = (int64_t)q.nz * q.nr;
   hipLaunchKernelGGL(q_rhs_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, q);
+  return hipGetLastError();
+}
+
+hipError_t launch_line_guess(Run &R, const double *row, hipStream_t s) {
+  const int64_t n = (int64_t)R.d.nz * R.d.nr;
+  hipLaunchKernelGGL(line_guess_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, R.Psi, R.d.nz, R.d.nr,
+                     R.ldz, row);
+  return hipGetLastError();
+}
+
+hipError_t launch_point_guess(Run &R, const double *q, const double *centroid, hipStream_t s) {
+  const int64_t n = (int64_t)R.d.nz * R.d.nr;
+  hipLaunchKernelGGL(point_guess_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, R.Psi, R.d.nz, R.d.nr,
+                     R.ldz, R.d.dz, R.d.dr, R.d.zmin, R.d.epsilon_o, q, centroid);
   return hipGetLastError();
 }
 

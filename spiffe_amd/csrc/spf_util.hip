@@ -141,6 +141,16 @@ struct TermsIrIz {   // for (ir) for (iz): element (iz, ir)
   }
 };
 
+struct TermsMomentIrIz {   // for (ir) for (iz): element (iz, ir) times iz*dz + zmin
+  const double *a;
+  int ldz, iz_n;
+  double dz, zmin;
+  __device__ double operator()(int64_t i) const {
+    const int ir = (int)(i / iz_n), iz = (int)(i % iz_n);
+    return a[(int64_t)ir * ldz + iz] * ((double)iz * dz + zmin);
+  }
+};
+
 template <class T>
 __global__ __launch_bounds__(64) void serial_sum_kernel(T term, int64_t n, double *out) {
   __shared__ double buf[2][64];
@@ -173,6 +183,13 @@ hipError_t launch_serial_sum_iz_ir(const double *a, int ldz, int iz_n, int ir_n,
                                    hipStream_t s) {
   TermsIzIr t{a, ldz, ir_n, absval};
   hipLaunchKernelGGL(serial_sum_kernel<TermsIzIr>, dim3(1), dim3(64), 0, s, t, (int64_t)iz_n * ir_n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_serial_moment_ir_iz(const double *a, int ldz, int iz_n, int ir_n, double dz, double zmin, double *out,
+                                      hipStream_t s) {
+  TermsMomentIrIz t{a, ldz, iz_n, dz, zmin};
+  hipLaunchKernelGGL(serial_sum_kernel<TermsMomentIrIz>, dim3(1), dim3(64), 0, s, t, (int64_t)iz_n * ir_n, out);
   return hipGetLastError();
 }
 

@@ -2,6 +2,8 @@
 brute force on tiny inputs.  The reference ships no tests or golden vectors,
 so each check is chosen so that a plausible slip in the restatement (a wrong
 sign, index, stagger offset or coefficient) fails it."""
+import math
+
 import numpy as np
 import pytest
 from scipy.special import j0, j1
@@ -720,3 +722,145 @@ def test_field_correction_restores_gauss_law():
     b, a = np.abs(before[:g.nz, :g.nr - 1][interior]), np.abs(after[:g.nz, :g.nr - 1][interior])
     assert b.max() > 0
     assert a.max() < 1e-6 * b.max()
+
+
+# ---------------------------------------------------------------- area-weighted (smeared) deposition
+SMEARS = {"radial": abi.RADIAL_SMEAR, "longitudinal": abi.LONGIT_SMEAR,
+          "both": abi.RADIAL_SMEAR | abi.LONGIT_SMEAR}
+
+
+def _interior_particles(g, n, seed):
+    """Particles away from the z ends and the top rows, r >= dr/2 (the Jr row-0 rule
+    puts every current of a particle below dr/2 on row 1)."""
+    rng = np.random.default_rng(seed)
+    p = random_particles(g, n, seed)
+    p["z"] = g.zmin + g.dz * (2.0 + rng.random(n) * (g.nz - 5))
+    p["r"] = g.dr * (0.5 + rng.random(n) * (g.nr - 4))
+    return p
+
+
+@pytest.mark.parametrize("smear", sorted(SMEARS))
+def test_smeared_deposit_conserves_current_and_centres_it_on_the_particle(smear):
+    """Area weighting (src/space_charge.c, FL_RADIAL_SMEAR / FL_LONGIT_SMEAR) splits
+    each particle's current between the two nearest nodes in each smeared
+    direction with weights that sum to one and whose centroid is the particle:
+    sum J*vol = sum Q v, and sum J*vol*node = sum Q v * position along each
+    smeared direction, with Jz nodes at z = (iz - 1/2) dz and Jr/Jphi nodes at
+    iz dz, Jz/Jphi rows at r = ir dr and Jr rows at (ir - 1/2) dr (vol: the
+    annulus areas of the staggered rows).  A swapped weight, a shifted node or a
+    wrong volume factor moves a centroid or loses current."""
+    g = pillbox_grid(modes=TMTE | abi.SPACE_CHARGE | SMEARS[smear])
+    n = 3000
+    p = _interior_particles(g, n, 21)
+    o = OracleSim(g, pillbox_flags(g), max_particles=n)
+    o.append(p)
+    o.add_particle_currents()
+    ir = np.arange(g.nr + 1, dtype=float)
+    iz = np.arange(g.nz + 1, dtype=float)[:, None]
+    base = np.pi * g.dz * g.dr ** 2
+    vc = base * np.where(ir == 0, 0.25, 2.0 * ir)       # Jz, Jphi rows
+    vo = base * (2.0 * ir - 1)                           # Jr rows (ir >= 1)
+    x = (p["z"] - g.zmin) / g.dz
+    y = p["r"] / g.dr
+    for name, v, vol, zn, rn in (("Jz", p["vz"], vc, iz - 0.5, ir), ("Jphi", p["vphi"], vc, iz, ir),
+                                 ("Jr", p["vr"], vo, iz, ir - 0.5)):
+        Jv = o.arr[name] * vol
+        Qv = p["Q"] * v
+        assert np.sum(Jv) == pytest.approx(np.sum(Qv), rel=1e-11), name
+        if SMEARS[smear] & abi.LONGIT_SMEAR:
+            assert np.sum(Jv * zn) == pytest.approx(np.sum(Qv * x), rel=1e-11), name + " z centroid"
+        if SMEARS[smear] & abi.RADIAL_SMEAR:
+            assert np.sum(Jv * rn) == pytest.approx(np.sum(Qv * y), rel=1e-11), name + " r centroid"
+
+
+@pytest.mark.parametrize("smear", sorted(SMEARS))
+def test_smeared_charge_removal_centres_the_charge_on_the_particle(smear):
+    """remove_particle_charge with smearing (src/poisson_correction.c:188-239):
+    with zero fields Q holds minus the removed charge; away from metal and the
+    Neumann rows each particle's charge comes off in full, centred on the
+    particle in each smeared direction (nodes at iz dz, ir dr), and Qreal is the
+    charge removed."""
+    nz, nr = 24, 16
+    m = np.zeros((nz + 1, nr + 1), dtype=np.int16)
+    g, o = _lattice_sim(nz, nr, m, modes=abi.TM_FIELDS | abi.SPACE_CHARGE | SMEARS[smear], cap=400)
+    rng = np.random.default_rng(4)
+    n = 400
+    zs = g.zmin + g.dz * (1.5 + rng.random(n) * (nz - 4))
+    rs = g.dr * rng.random(n) * (nr - 3)
+    q = -1e-15 * (1 + rng.random(n))
+    p = _particles_at(zs, rs, 0.0)
+    p["Q"] = q
+    o.append(p)
+    o.setup_poisson_correction(step_interval=1, accuracy=1e-6, maximum_iterations=10, error_charge_threshold=1.0)
+    assert o.add_poisson_correction(True) == -2
+    removed = -o.field("Q")
+    assert np.sum(removed) == pytest.approx(np.sum(q), rel=1e-12)
+    assert o.pc.Qreal == pytest.approx(np.sum(q), rel=1e-12)
+    x, y = (zs - g.zmin) / g.dz, rs / g.dr
+    izg = np.arange(nz + 1, dtype=float)[:, None]
+    irg = np.arange(nr + 1, dtype=float)[None, :]
+    if SMEARS[smear] & abi.LONGIT_SMEAR:
+        assert np.sum(removed * izg) == pytest.approx(np.sum(q * x), rel=1e-12)
+    if SMEARS[smear] & abi.RADIAL_SMEAR:
+        assert np.sum(removed * irg) == pytest.approx(np.sum(q * y), rel=1e-12)
+
+
+# ---------------------------------------------------------------- Poisson guesses
+def _guess_case(code):
+    """A lattice with metal points (some on the axis), random fields and a few
+    particles; one correction with the given guess and a short solve.  The SOR
+    never updates metal points or the row ir = nr-1, so Psi there keeps the
+    guess of src/poisson_correction.c:564-615."""
+    nz, nr = 30, 14
+    S = abi.FL_IS_SURFACE
+    m = np.zeros((nz + 1, nr + 1), dtype=np.int16)
+    m[10:13, 0:3] = S
+    m[20, 5] = S
+    g, o = _lattice_sim(nz, nr, m, modes=abi.TM_FIELDS | abi.SPACE_CHARGE, cap=50)
+    rng = np.random.default_rng(8)
+    fields = {k: rng.standard_normal(g.shape) * 1e3 for k in ("Ez", "Er")}
+    for k, v in fields.items():
+        o.set_field(k, v)
+    p = _particles_at(g.zmin + g.dz * (3 + rng.random(50) * 20), g.dr * rng.random(50) * 8, -3e-16)
+    o.append(p)
+    # the Gauss-law charge before any particle charge comes off: the same
+    # fields with a correction that stops at its error-charge test
+    probe = _lattice_sim(nz, nr, m, modes=abi.TM_FIELDS | abi.SPACE_CHARGE, cap=50)[1]
+    for k, v in fields.items():
+        probe.set_field(k, v)
+    probe.setup_poisson_correction(step_interval=1, error_charge_threshold=1e30)
+    probe.add_poisson_correction(False)
+    gauss = probe.field("Q")
+    o.setup_poisson_correction(step_interval=1, accuracy=1e-3, maximum_iterations=3, guess_code=code)
+    o.add_poisson_correction(True)
+    keep = np.zeros(g.shape, dtype=bool)
+    keep[:nz, nr - 1] = True
+    keep[:nz, :nr] |= (m[:nz, :nr] & abi.FL_IS_METAL) != 0
+    return g, o, gauss, keep
+
+
+def test_line_charge_guess():
+    """guess_type "line-charge": Psi = Qerror/(2 pi eps0) ln r on every row,
+    r = ir dr, and dr/2 on the axis row."""
+    g, o, _, keep = _guess_case(1)
+    psi = o.field("Psi")
+    assert o.pc.Qerror != 0
+    r = g.dr * np.where(np.arange(g.nr + 1) == 0, 0.5, np.arange(g.nr + 1))
+    want = np.broadcast_to(o.pc.Qerror / (2 * math.pi * K.epsilon_o) * np.log(r)[None, :], psi.shape)
+    assert keep[11, 0] and keep[5, g.nr - 1]
+    np.testing.assert_allclose(psi[keep], want[keep], rtol=1e-14)
+
+
+def test_point_charge_guess():
+    """guess_type "point-charge": Psi = Qerror/(4 pi eps0 d), d the distance from
+    (iz dz + zmin, (ir + 1/2) dr) to (c, 0), where c is the reference's
+    "centroid": the Gauss-law charges times their z, summed without dividing by
+    the total (src/poisson_correction.c:176)."""
+    g, o, gauss, keep = _guess_case(2)
+    psi = o.field("Psi")
+    c = math.fsum((gauss[iz, ir] * (iz * g.dz + g.zmin) for ir in range(g.nr) for iz in range(g.nz)))
+    assert c != 0 and o.pc.Qerror != 0
+    izg = np.arange(g.nz + 1)[:, None] * g.dz + g.zmin
+    irg = (np.arange(g.nr + 1)[None, :] + 0.5) * g.dr
+    want = o.pc.Qerror / (4 * math.pi * K.epsilon_o * np.sqrt((izg - c) ** 2 + irg ** 2))
+    np.testing.assert_allclose(psi[keep], want[keep], rtol=1e-12)
