@@ -179,6 +179,11 @@ int spf_get_step(spf_ctx *ctx, int32_t run, int64_t *time_step, double *time);
  * hands to the push as advance_particles' zForcesStart (src/integrate.c:186-188,
  * src/advance_particles.c:190-200); a run starts with -1 */
 int spf_set_z_forces_start(spf_ctx *ctx, int32_t run, double z_forces_start);
+/* integrate's J_filter_multiplier m (src/integrate.nl, src/integrate.c:110-116;
+ * 0 <= m < 1, default 0): with m != 0 spf_step and spf_step_batch scale J by
+ * m/(1-m) instead of clearing it, deposit and add the line currents on top,
+ * and scale it by 1-m (src/integrate.c:168-178); a run starts with 0 */
+int spf_set_j_filter(spf_ctx *ctx, int32_t run, double multiplier);
 
 /* No counterpart in the reference (which keeps particles in emission order):
  * re-sorts a cell-sort run's particles by cell now (spf_step does it every
@@ -436,8 +441,9 @@ int spf_sor_batch_half_sweeps(spf_ctx *ctx, int32_t run, int32_t *half_sweeps);
 
 /* ---- one whole step of the loop body, src/integrate.c:134-194 ----
  * ramped imposed E (if set); time <- step*dt+start; emission from every
- * cathode of the run; J reset; deposit (if SPF_SPACE_CHARGE); n_lines line
- * currents; Poisson correction (if set up); FDTD; push (with the run's
+ * cathode of the run; J reset (or the J filter's first scaling); deposit (if
+ * SPF_SPACE_CHARGE); n_lines line currents; the J filter's second scaling (if
+ * set); Poisson correction (if set up); FDTD; push (with the run's
  * z_forces_start); current densities of the cathodes; step++. */
 int spf_step(spf_ctx *ctx, int32_t run, int32_t n_lines, const int32_t *line_dir, const int64_t *line_pos,
              const int64_t *line_start, const int64_t *line_end, const double *line_J);

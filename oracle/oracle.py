@@ -408,17 +408,26 @@ class OracleSim:
         self.lib.oracle_compute_imposed_efield(ctypes.byref(self.f), 1 if initialize else 0, ctypes.byref(self.pc),
                                                ctypes.byref(self.last_field_factor))
 
-    def step(self, lines=(), screens=(), compat=True, z_forces_start=-1.0):
-        """Loop body of src/integrate.c:134-194, emission included, in the order of the device's spf_step."""
+    def step(self, lines=(), screens=(), compat=True, z_forces_start=-1.0, j_filter=0.0):
+        """Loop body of src/integrate.c:134-194, emission included, in the order of the device's spf_step.
+        j_filter: integrate's J_filter_multiplier m (:110-116): J is scaled by m/(1-m)
+        instead of cleared, and by 1-m after the particle and line currents (:168-178)."""
         f = self.f
         f.time = f.time_step * f.dti + f.start_time
         for k in range(self.n_cathodes):
             self.emit_electrons_from_cathode(k)
-        self.zero_current_arrays()
+        if j_filter:
+            jm2 = 1 - j_filter
+            jm1 = j_filter / jm2
+            self.multiply_current_arrays(jm1)
+        else:
+            self.zero_current_arrays()
         if f.modes & 4:
             self.add_particle_currents()
         for (d, pos, s, e, J) in lines:
             self.add_line_current(d, pos, s, e, J)
+        if j_filter:
+            self.multiply_current_arrays(jm2)
         if self.pc is not None:
             self.add_poisson_correction((f.modes & 4) != 0)
         self.advance_field_solutions()

@@ -164,6 +164,7 @@ SIGNATURES = {
     "spf_set_step": (ctypes.c_int, [_vp, _i32, _i64]),
     "spf_get_step": (ctypes.c_int, [_vp, _i32, _pi64, _dp]),
     "spf_set_z_forces_start": (ctypes.c_int, [_vp, _i32, _d]),
+    "spf_set_j_filter": (ctypes.c_int, [_vp, _i32, _d]),
     "spf_fields_advance": (ctypes.c_int, [_vp, _i32]),
     "spf_currents_reset": (ctypes.c_int, [_vp, _i32]),
     "spf_currents_scale": (ctypes.c_int, [_vp, _i32, _d]),

@@ -558,6 +558,20 @@ int spf_set_step(spf_ctx *c, int32_t run, int64_t time_step) {
   return SPF_OK;
 }
 
+int spf_set_j_filter(spf_ctx *c, int32_t run, double multiplier) {
+  Run *R = get_run(c, run);
+  if (!R) return fail(c, SPF_EINVAL, "bad run id");
+  if (!(multiplier >= 0 && multiplier < 1)) return fail(c, SPF_EINVAL, "J_filter_multiplier must be >=0 and < 1");
+  R->jfm = multiplier;
+  R->jm1 = 1;
+  R->jm2 = 0;
+  if (multiplier != 0) {   // src/integrate.c:110-116
+    R->jm2 = 1 - multiplier;
+    R->jm1 = multiplier / R->jm2;
+  }
+  return SPF_OK;
+}
+
 int spf_set_z_forces_start(spf_ctx *c, int32_t run, double z_forces_start) {
   Run *R = get_run(c, run);
   if (!R) return fail(c, SPF_EINVAL, "bad run id");

@@ -596,17 +596,21 @@ int spf_step(spf_ctx *c, int32_t run, int32_t n_lines, const int32_t *line_dir, 
   for (int32_t k = 0; k < (int32_t)R->cathodes.size(); k++)   // src/integrate.c:162-163
     if ((rc = cathode_emit_step(c, run, k, nullptr))) return rc;
   if (R->cell_sort > 0 && R->time_step % R->cell_sort == 0 && (rc = spf_particles_sort(c, run))) return rc;
+  // with the J time filter, J is scaled instead of cleared (src/integrate.c:168-171)
+  const bool filt = R->jfm != 0;
+  if (filt && (rc = spf_currents_scale(c, run, R->jm1))) return rc;
   if (R->d.modes & SPF_SPACE_CHARGE) {
     // reset and deposition (src/integrate.c:169-173) in one pass over the
     // keys: the key kernel clears J, the folds add into it
     PhaseTimer dt(c, SPF_T_DEPOSIT);
-    HIPCALL(c, deposit_currents(*R, c->stream, true));
-  } else if ((rc = spf_currents_reset(c, run))) {
+    HIPCALL(c, deposit_currents(*R, c->stream, !filt));
+  } else if (!filt && (rc = spf_currents_reset(c, run))) {
     return rc;
   }
   for (int32_t k = 0; k < n_lines; k++)
     if ((rc = spf_currents_add_line(c, run, line_dir[k], line_pos[k], line_start[k], line_end[k], line_J[k])))
       return rc;
+  if (filt && (rc = spf_currents_scale(c, run, R->jm2))) return rc;   // src/integrate.c:177-178
   if (R->pc_set && (rc = spf_poisson_correct(c, run, (R->d.modes & SPF_SPACE_CHARGE) != 0, nullptr))) return rc;
   if ((rc = spf_fields_advance(c, run))) return rc;
   if ((rc = spf_particles_push(c, run, R->z_forces_start))) return rc;   // src/integrate.c:186-188
@@ -642,6 +646,16 @@ int spf_step_batch(spf_ctx *c, int32_t n_runs, const int32_t *run_ids) {
     for (int32_t k = 0; k < (int32_t)R->cathodes.size(); k++)
       if ((rc = cathode_emit_step(c, run_ids[i], k, nullptr))) return rc;
     if (R->cell_sort > 0 && R->time_step % R->cell_sort == 0 && (rc = spf_particles_sort(c, run_ids[i]))) return rc;
+    if (R->jfm != 0) {
+      // the J time filter (src/integrate.c:168-178): scaled, deposited on, scaled
+      if ((rc = spf_currents_scale(c, run_ids[i], R->jm1))) return rc;
+      if (R->d.modes & SPF_SPACE_CHARGE) {
+        PhaseTimer dt(c, SPF_T_DEPOSIT);
+        HIPCALL(c, deposit_currents(*R, s, false));
+      }
+      if ((rc = spf_currents_scale(c, run_ids[i], R->jm2))) return rc;
+      continue;
+    }
     if (R->d.modes & SPF_SPACE_CHARGE) {
       if (R->cell_sort > 0 && R->np_ub > 0 && (R->d.modes & (SPF_TM_FIELDS | SPF_TE_FIELDS)) &&
           deposit_slots_per_particle(R->d.modes) == 1) {

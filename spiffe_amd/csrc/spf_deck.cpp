@@ -683,7 +683,7 @@ struct Deck {
     if (finish_time < start_time) bomb("finish_time < start_time");
     if (smoothing < 0 || smoothing > 1) bomb("smoothing_parameter must be >= 0 and < 1");
     if (jfm < 0 || jfm >= 1) bomb("J_filter_multiplier must be >=0 and < 1");
-    if (smoothing || jfm || sg) bomb("current smoothing and filtering are not supported", SPF_EUNSUPPORTED);
+    if (smoothing || sg) bomb("current smoothing is not supported", SPF_EUNSUPPORTED);
     if (!status_file.empty()) say("note: status_output is not written\n");
     if (integrations++) bomb("a second integrate command is not supported", SPF_EUNSUPPORTED);
     if (space_charge) d.modes |= SPF_SPACE_CHARGE;
@@ -699,6 +699,7 @@ struct Deck {
     if (opt.max_steps > 0 && n_steps > opt.max_steps) n_steps = opt.max_steps;
     create_run();
     check(spf_set_z_forces_start(ctx, run, z_forces_start));
+    check(spf_set_j_filter(ctx, run, jfm));
     // outputs
     SddsWriter lost_out;
     if (!lost_file.empty()) {

@@ -58,6 +58,7 @@ struct KeyArgs {
   int32_t *wg_count;            // reduce path: partials per workgroup
   int smear_r, smear_z;         // FL_RADIAL_SMEAR, FL_LONGIT_SMEAR
   int tpp;                      // key slots per particle and component: 4 when smeared, else 1
+  double *jzero[3];             // J targets of slot k that may hold -0 (not cleared first), or null
 };
 
 // NGP value of the Jz/Jphi loops for cell row ir (0 <= ir <= nr-2), :92-100, :254-262
@@ -397,6 +398,38 @@ __device__ inline void smear_charge(const KeyArgs &a, const PartVals &pv, Smear4
   }
 }
 
+// The reference's nearest-point loops also add +0 (a literal zero times a
+// weight of 1) to J at points this deposition leaves out: for Jz and Jphi the
+// row below when ir >= nr-1 (:92-101, :254-263), for Jr row 0 when ir == 0
+// and row nr-2 when ir == nr-1 (:183-193).  Adding +0 changes only a -0 (into
+// +0), and a point holds -0 only while nothing but -0 has been added to a -0
+// start, so turning such a -0 into +0 before the ordered fold gives the
+// reference's bits wherever the zero falls in its sequence.  J is +0 after a
+// clear, so this matters only when J is not cleared first (the J time
+// filter, or a deposition onto uploaded currents).
+__device__ inline void plus_zero(double *T, int ldz, long long iz, long long ir) {
+  double *q = T + fix(ldz, (int)iz, (int)ir);
+  if (__double_as_longlong(*q) == (long long)0x8000000000000000ull) *q = 0.0;
+}
+
+__device__ inline void dropped_j_zeros(const KeyArgs &a, const PartCells &pc) {
+  if (!pc.live) return;
+  const long long nz = a.nz, nr = a.nr;
+  for (int k = 0; k < a.ncomp; k++) {
+    double *T = a.jzero[k < 3 ? k : 0];
+    if (!T) continue;
+    const int c = a.comp[k];
+    if (c == COMP_JZ) {
+      if (pc.izA >= 0 && pc.izA <= nz && (pc.irh == nr - 1 || pc.irh == nr)) plus_zero(T, a.ldz, pc.izA, pc.irh - 1);
+    } else if (c == COMP_JR) {
+      if (pc.izh >= 0 && pc.izh <= nz && (pc.irA == 0 || pc.irA == nr - 1))
+        plus_zero(T, a.ldz, pc.izh, pc.irA == 0 ? 0 : nr - 2);
+    } else if (c == COMP_JPHI) {
+      if (pc.izh >= 0 && pc.izh <= nz && (pc.irh == nr - 1 || pc.irh == nr)) plus_zero(T, a.ldz, pc.izh, pc.irh - 1);
+    }
+  }
+}
+
 // the targets' reset (src/integrate.c:169 zeroes J before the deposition),
 // done by the key kernels to save a launch; the folds that add into them come later
 __device__ inline void clear_targets(const KeyArgs &a, int64_t ip, int64_t stride) {
@@ -442,6 +475,7 @@ __global__ __launch_bounds__(256) void key_kernel(KeyArgs a) {
   int64_t np = in ? *a.np : 0;
   const PartVals pv = load_part(a, ip, np);
   const PartCells pc = part_cells(a, pv);
+  if (in && a.jzero[0]) dropped_j_zeros(a, pc);
   for (int k = 0; in && k < a.ncomp; k++) {
     double v;
     const uint32_t key = part_contrib(a, pc, pv, k, v);
@@ -1132,6 +1166,9 @@ static KeyArgs make_key_args(Run &R, const int *comps, int ncomp, double *const 
   k.smear_r = (R.d.modes & SPF_RADIAL_SMEAR) != 0;
   k.smear_z = (R.d.modes & SPF_LONGIT_SMEAR) != 0;
   k.tpp = deposit_slots_per_particle(R.d.modes);
+  for (int i = 0; i < 3; i++) k.jzero[i] = nullptr;
+  if (!clear_first && ncomp > 0 && comps[0] != COMP_Q)
+    for (int i = 0; i < ncomp && i < 3; i++) k.jzero[i] = targets[i];
   // smeared charge removal keeps its zero-weight terms in the ordered fold
   if (k.tpp != 1) {
     k.zero_terms = 0;

@@ -190,6 +190,10 @@ struct Run {
   std::vector<int> screen_dir;
   double ramp_time = 0, flat_top_time = 0;
   double z_forces_start = -1;   // integrate's z_forces_start (src/integrate.nl:19), used by spf_step
+  // integrate's J_filter_multiplier m (src/integrate.c:110-116): when nonzero
+  // the steps scale J by jm1 = m/(1-m) instead of clearing it and by jm2 = 1-m
+  // after the particle and line currents
+  double jfm = 0, jm1 = 1, jm2 = 0;
   // cathode emission: host-side state, one uniform stream per run
   EmitRng emit_rng;
   std::vector<Cathode> cathodes;

@@ -217,6 +217,10 @@ class DeviceRun:
         """integrate's z_forces_start, used by step() (src/integrate.c:186-188)."""
         self.ctx.check(self.lib.spf_set_z_forces_start(self.ctx.h, self.id, float(z)))
 
+    def set_j_filter(self, multiplier):
+        """integrate's J_filter_multiplier, used by step() (src/integrate.c:110-116, :168-178)."""
+        self.ctx.check(self.lib.spf_set_j_filter(self.ctx.h, self.id, float(multiplier)))
+
     def get_step(self):
         s, t = ctypes.c_int64(), ctypes.c_double()
         self.ctx.check(self.lib.spf_get_step(self.ctx.h, self.id, ctypes.byref(s), ctypes.byref(t)))

@@ -1,7 +1,8 @@
 """Reference options and semantics beyond the plain step, device vs the CPU
-restatement bit for bit: integrate's z_forces_start, the signed zeros of the
-charge removal's zero-weight terms, the relative error-charge threshold, and
-runs in two contexts driven from two host threads at once.
+restatement bit for bit: integrate's z_forces_start and J_filter_multiplier,
+the signed zeros of the charge removal's zero-weight terms and of deposition
+onto uncleared currents, the relative error-charge threshold, and runs in two
+contexts driven from two host threads at once.
 """
 import threading
 
@@ -327,4 +328,65 @@ def test_error_charge_threshold_at_the_rounding_edge(ctx, mode):
     assert it_o == it_d
     for name in ("Ez", "Er", "Psi", "Q"):
         same(o.field(name), d.download(name), name)
+    d.close()
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_j_filter_steps(ctx, batched):
+    """integrate's J_filter_multiplier (src/integrate.c:110-116, :168-178): J is
+    scaled by m/(1-m) instead of cleared, deposited on, and scaled by 1-m;
+    whole steps (with line currents when stepped alone, through spf_step_batch
+    otherwise) match the oracle bit for bit."""
+    g = gun_grid(modes=TMTE | abi.SPACE_CHARGE)
+    f = gun_flags(g)
+    n = 3000
+    m = 0.3
+    o = OracleSim(g, f, max_particles=n)
+    d = DeviceRun(ctx, g, f, max_particles=n, compat=False)
+    for k, v in random_fields(g, 61).items():
+        o.set_field(k, v)
+        d.upload(k, v)
+    p = random_particles(g, n, 62, flags=f)
+    o.append(p)
+    d.append(p)
+    d.set_j_filter(m)
+    lines = () if batched else ((0, 3, 10, 40, 2.5e3), (1, 20, 5, 9, -1.5e3))
+    for _ in range(8):
+        o.step(lines=lines, screens=(), compat=False, j_filter=m)
+        if batched:
+            ctx.step_batch([d])
+        else:
+            d.step(lines=lines)
+    for name in TM_NAMES + TE_NAMES + ["Jz", "Jr", "Jphi"]:
+        same(o.field(name), d.download(name), name)
+    same_particles(o, d)
+    d.close()
+
+
+def test_deposit_onto_negative_zero_currents(ctx):
+    """A deposition onto currents that hold -0 (not cleared first, as with the J
+    filter): the reference's nearest-point loops add +0 at points below the top
+    rows and on Jr's row 0, turning those -0 into +0; the device gives the same
+    signed zeros."""
+    g = gun_grid(modes=TMTE | abi.SPACE_CHARGE)
+    f = gun_flags(g)
+    n = 4000
+    o = OracleSim(g, f, max_particles=n)
+    d = DeviceRun(ctx, g, f, max_particles=n)
+    for name in ("Jz", "Jr", "Jphi"):
+        o.set_field(name, np.full(g.shape, -0.0))
+        d.upload(name, np.full(g.shape, -0.0))
+    p = random_particles(g, n, 63)
+    rng = np.random.default_rng(64)
+    k = n // 2
+    p["r"][:k] = g.dr * (g.nr - 1.5 + rng.random(k) * 1.4)   # rows nr-1 and nr
+    p["r"][k:k + 50] = -0.4 * g.dr * rng.random(50)           # Jr's row 0
+    o.append(p)
+    d.append(p)
+    o.add_particle_currents()
+    d.add_particle_currents()
+    for name in ("Jz", "Jr", "Jphi"):
+        a = o.field(name)
+        assert np.any(bits(a) == bits(np.float64(-0.0))) and np.any(bits(a) == bits(np.float64(0.0)))
+        same(a, d.download(name), name)
     d.close()

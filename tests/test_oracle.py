@@ -864,3 +864,20 @@ def test_point_charge_guess():
     irg = (np.arange(g.nr + 1)[None, :] + 0.5) * g.dr
     want = o.pc.Qerror / (4 * math.pi * K.epsilon_o * np.sqrt((izg - c) ** 2 + irg ** 2))
     np.testing.assert_allclose(psi[keep], want[keep], rtol=1e-12)
+
+
+def test_j_filter_is_an_exponential_average():
+    """integrate's J_filter_multiplier m: J_n = m J_(n-1) + (1-m) L for a line
+    current L added every step, so J_n = L (1 - m^n) from J_0 = 0; scaling in
+    the wrong place (before the lines, or not at all) gives L (1-m^n)/(1-m) or
+    n L instead."""
+    g = pillbox_grid(modes=abi.TM_FIELDS | abi.RADIAL_INTERP | abi.LONGIT_INTERP)
+    o = OracleSim(g, pillbox_flags(g), max_particles=1)
+    m, L = 0.35, 4.0e3
+    lines = ((0, 5, 10, 20, L),)
+    for step in range(1, 9):
+        o.step(lines=lines, j_filter=m)
+        J = o.arr["Jz"]
+        want = L * (1 - m ** step)
+        assert J[10:21, 5] == pytest.approx(np.full(11, want), rel=1e-13)
+        assert np.count_nonzero(J) == 11
