@@ -323,7 +323,7 @@ __device__ inline void sor_setup(const SorCommon &a, const SorTile<G> &t, int ti
 // one region with no branch between them and their dependent chains
 // interleave.  The same operations per point as sor_half_sweep, so the same
 // Psi; the residual sum adds the rows of column 0, then those of column 1.
-template <class G, int C>
+template <class G, int C, bool OWNALL>
 __device__ inline double sor_half_sweep_rows(const SorCommon &a, SorLane<G> &ln, uint64_t zv0, uint64_t zv1,
                                              const double2 up_row, const double2 dn_row, double omega) {
   const double A = a.a, E = a.e, y = a.y_e;
@@ -360,7 +360,7 @@ __device__ inline double sor_half_sweep_rows(const SorCommon &a, SorLane<G> &ln,
       const double x = omega * res[g];
       ln.P[i][col] = ln.P[i][col] - div_fast(x, E, y);
       ln.eacc = max(ln.eacc, div_fast_exp(x));
-      acc += fabs(res[g]);
+      if (OWNALL || ((ln.ownrows >> i) & 1u)) acc += fabs(res[g]);
     }
   }
   return acc;
@@ -455,7 +455,7 @@ __device__ inline double sor_step(const SorCommon &a, const SorTile<G> &t, SorLa
 
 // step j of a wavefront that takes sor_half_sweep_rows; the lanes outside
 // the owned columns return 0
-template <class G, int C>
+template <class G, int C, bool OWNALL>
 __device__ inline double sor_step_rows(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane,
                                        double omega, int j, double2 (*xrow)[G::SW][2][64]) {
   const int par = j & 1;
@@ -464,8 +464,19 @@ __device__ inline double sor_step_rows(const SorCommon &a, const SorTile<G> &t, 
   __syncthreads();
   const double2 up_row = w + 1 < G::SW ? xrow[par][w + 1][0][lane] : make_double2(0.0, 0.0);
   const double2 dn_row = w > 0 ? xrow[par][w - 1][1][lane] : make_double2(0.0, 0.0);
-  const double acc = sor_half_sweep_rows<G, C>(a, ln, __ballot(ln.zdepth[0] > j), __ballot(ln.zdepth[1] > j), up_row,
-                                               dn_row, omega);
+  const uint64_t zv0 = __ballot(ln.zdepth[0] > j), zv1 = __ballot(ln.zdepth[1] > j);
+  double acc;
+  if (OWNALL) {
+    acc = sor_half_sweep_rows<G, C, true>(a, ln, zv0, zv1, up_row, dn_row, omega);
+  } else {
+    // rows valid at this step (uniform): all of them, or the per-row loop
+    const int vr0 = t.sr_lo ? j + 1 : 0, vr1 = t.LR - (t.sr_hi ? j + 1 : 0);
+    const int lo = min(max(vr0 - w * G::RS, 0), G::RS), hi = min(max(vr1 - w * G::RS, 0), G::RS);
+    if (lo == 0 && hi == G::RS)
+      acc = sor_half_sweep_rows<G, C, false>(a, ln, zv0, zv1, up_row, dn_row, omega);
+    else
+      acc = sor_half_sweep<G, C, false, 0, false>(a, ln, lo, hi, zv0, zv1, up_row, dn_row, omega, false, false, false);
+  }
   return __builtin_amdgcn_inverse_ballot_w64(ln.colown[0]) ? acc : 0.0;
 }
 
@@ -499,13 +510,14 @@ __device__ inline void sor_pairs(const SorCommon &a, const SorTile<G> &t, SorLan
 }
 
 // the pair loop of the wavefronts that take sor_half_sweep_rows
-template <class G>
+template <class G, bool OWNALL>
 __device__ inline void sor_pairs_rows(const SorCommon &a, const SorTile<G> &t, SorLane<G> &ln, int w, int lane, int k,
                                       const double *om, double2 (*xrow)[G::SW][2][64], double (*lres)[G::SWL]) {
 #pragma unroll 1
   for (int j = 0; j < k; j += 2) {
-    sor_wave_sum<G>(sor_step_rows<G, 0>(a, t, ln, w, lane, om[j], j, xrow), j, w, lane, lres);
-    if (j + 1 < k) sor_wave_sum<G>(sor_step_rows<G, 1>(a, t, ln, w, lane, om[j + 1], j + 1, xrow), j + 1, w, lane, lres);
+    sor_wave_sum<G>(sor_step_rows<G, 0, OWNALL>(a, t, ln, w, lane, om[j], j, xrow), j, w, lane, lres);
+    if (j + 1 < k)
+      sor_wave_sum<G>(sor_step_rows<G, 1, OWNALL>(a, t, ln, w, lane, om[j + 1], j + 1, xrow), j + 1, w, lane, lres);
   }
 }
 
@@ -548,7 +560,9 @@ __device__ inline void sor_sweeps(const SorCommon &a, const SorTile<G> &t, SorLa
     if (axis_wave)
       sor_pairs<G, false, 1>(a, t, ln, w, lane, k, om, xrow, lres, axis_wave);
     else if (ROWS && ln.full && rows_valid && ln.ownrows == (1u << G::RS) - 1u)
-      sor_pairs_rows<G>(a, t, ln, w, lane, k, om, xrow, lres);
+      sor_pairs_rows<G, true>(a, t, ln, w, lane, k, om, xrow, lres);
+    else if (ROWS && ln.full && ln.ownrows != 0)
+      sor_pairs_rows<G, false>(a, t, ln, w, lane, k, om, xrow, lres);
     else
       sor_pairs<G, false, 0>(a, t, ln, w, lane, k, om, xrow, lres, axis_wave);
   }
