@@ -768,6 +768,15 @@ __device__ inline void sor_batch_block(const SorArgs &a, int bid, const double *
   sor_write_owned<G>(a.m, t, ln, w, lane, Pout);
 }
 
+// The grouped-row loop (sor_half_sweep_rows) where it was measured faster:
+// the large tile shape (3.3% per half-sweep on the 4201 x 1681-line grid, 2.3%
+// on the 64-run scan's ragged solves); on the small shape at 1051 x 421 it
+// was no faster.
+template <class G>
+constexpr bool rows_loop() {
+  return G::SK == SorLarge::SK && G::RS == SorLarge::RS && G::SW == SorLarge::SW;
+}
+
 // Four wavefronts per SIMD (128 VGPRs): two 8-wavefront workgroups share a
 // CU, so one hides the other's barrier and DPP latencies.  The compiler keeps
 // a few batch constants in scratch to fit; measured 1.37x faster per batch
@@ -775,11 +784,7 @@ __device__ inline void sor_batch_block(const SorArgs &a, int bid, const double *
 template <class G>
 __global__ __launch_bounds__(G::NT) __attribute__((amdgpu_waves_per_eu(SPF_SOR_WAVES, SPF_SOR_WAVES))) void sor_batch_kernel(
     SorArgs a) {
-  // the grouped-row loop (sor_half_sweep_rows) only where it was measured
-  // faster: one run on the large tile shape (1.5% per half-sweep on the
-  // 4201 x 1681-line grid; 1.7% slower on the small shape at 1051 x 421)
-  constexpr bool rows = G::SK == SorLarge::SK && G::RS == SorLarge::RS && G::SW == SorLarge::SW;
-  sor_batch_block<G, rows>(a, (int)blockIdx.x, a.Pin, a.Pout, a.n0, a.k, a.partials, a.check_partials, a.check_n0,
+  sor_batch_block<G, rows_loop<G>()>(a, (int)blockIdx.x, a.Pin, a.Pout, a.n0, a.k, a.partials, a.check_partials, a.check_n0,
                            a.check_k);
 }
 
@@ -819,7 +824,7 @@ __global__ __launch_bounds__(G::NT) __attribute__((amdgpu_waves_per_eu(SPF_SOR_W
   const int bid = (int)blockIdx.x - first[r];
   if (bid > x.a.m.ntiles || (bid == x.a.m.ntiles && L.check_k == 0)) return;   // span padding
   const int ntiles = x.a.m.ntiles;
-  sor_batch_block<G>(x.a, bid, x.ring[L.b % 6], const_cast<double *>(x.ring[(L.b + 1) % 6]), L.n0, L.k,
+  sor_batch_block<G, rows_loop<G>()>(x.a, bid, x.ring[L.b % 6], const_cast<double *>(x.ring[(L.b + 1) % 6]), L.n0, L.k,
                   x.a.partials + L.partial_row * ntiles, x.a.partials + L.check_row * ntiles, L.check_n0, L.check_k);
 }
 
