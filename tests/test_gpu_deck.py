@@ -64,14 +64,18 @@ def compare_lost(path, lost_rows):
     return sum(len(l) for l in want)
 
 
-@pytest.mark.parametrize("nz,nr,N,dt,steps,interval", [
-    (151, 61, 20000, 4e-13, 700, 10),        # 0.2 mm mesh, 20 k macroparticles, 700 steps
-    (601, 241, 200000, 1e-13, 1600, 100),    # the serving stream's DC gun: 0.05 mm, 200 k, its own dt
+@pytest.mark.parametrize("nz,nr,N,dt,steps,interval,options", [
+    (151, 61, 20000, 4e-13, 700, 10, False),        # 0.2 mm mesh, 20 k macroparticles, 700 steps
+    (151, 61, 20000, 4e-13, 700, 10, True),         # the same with the options below
+    (601, 241, 200000, 1e-13, 1600, 100, False),    # the serving stream's DC gun: 0.05 mm, 200 k, its own dt
 ])
-def test_dc_gun_deck(ctx_free, tmp_path, nz, nr, N, dt, steps, interval):
+def test_dc_gun_deck(ctx_free, tmp_path, nz, nr, N, dt, steps, interval, options):
     """The DC-gun deck with field saving; on the full 0.05 mm mesh with all 200 k
     macroparticles over the first 1600 steps (the bunch reaches the first screen),
-    with a Poisson correction every 100 steps so the restatement keeps up."""
+    with a Poisson correction every 100 steps so the restatement keeps up.  With
+    `options` the deck also sets radial and longitudinal smearing
+    (define_geometry), the point-charge guess (poisson_correction) and
+    J_filter_multiplier (integrate)."""
     for f in ("dc_gun.geo", "flat_top_10ps.sdds"):
         shutil.copy(os.path.join(DECKS, f), tmp_path / f)
     text = open(os.path.join(DECKS, "dc_gun.spiffe")).read()
@@ -83,6 +87,12 @@ def test_dc_gun_deck(ctx_free, tmp_path, nz, nr, N, dt, steps, interval):
                         "dt_integration = %r, finish_time = %r" % (dt, (steps + 0.5) * dt))
     text = text.replace("&integrate", '&define_field_saving filename = "%%s.fsave", time_interval = %r, '
                                       'save_before_exiting = 1 &end\n\n&integrate' % (100 * dt))
+    if options:
+        text = text.replace('boundary = "dc_gun.geo"',
+                            'boundary = "dc_gun.geo", radial_smearing = 1, longitudinal_smearing = 1')
+        text = text.replace("maximum_iterations = 2000", 'maximum_iterations = 2000, guess_type = "point-charge"')
+        text = text.replace("space_charge = 1,", "space_charge = 1, J_filter_multiplier = 0.2,")
+        assert text.count("smearing") == 2 and "point-charge" in text and "J_filter" in text
     (tmp_path / "dc.spiffe").write_text(text)
     cap_particles = int(1.5 * N)
     stats = deck.run_deck(str(tmp_path / "dc.spiffe"), max_particles=cap_particles, compat=False)
@@ -90,11 +100,13 @@ def test_dc_gun_deck(ctx_free, tmp_path, nz, nr, N, dt, steps, interval):
 
     # the same run from the oracle
     g = Grid(nz=nz, nr=nr, zmin=0.0, zmax=0.030, rmax=0.012, dt=dt,
-             modes=abi.TM_FIELDS | abi.RADIAL_INTERP | abi.LONGIT_INTERP | abi.SPACE_CHARGE)
+             modes=abi.TM_FIELDS | abi.RADIAL_INTERP | abi.LONGIT_INTERP | abi.SPACE_CHARGE |
+             (abi.RADIAL_SMEAR | abi.LONGIT_SMEAR if options else 0))
     f, psi = oracle_flags(nz, nr, 0.0, 0.030, 0.012, "dc_gun.geo")
     o = O.OracleSim(g, f, max_particles=cap_particles)
     o.set_field("PsiImposed", psi)
-    o.setup_poisson_correction(step_interval=interval, accuracy=1e-6, maximum_iterations=2000)
+    o.setup_poisson_correction(step_interval=interval, accuracy=1e-6, maximum_iterations=2000,
+                               guess_code=2 if options else 0)
     o.define_cathode(profile=profile("flat_top_10ps.sdds"), z_position=0.0, outer_radius=3e-3,
                      current_density=1.768388e5, electrons_per_macroparticle=epm, initial_pz=1e-3,
                      zoned_emission=0, random_number_seed=987654321)
@@ -108,7 +120,7 @@ def test_dc_gun_deck(ctx_free, tmp_path, nz, nr, N, dt, steps, interval):
     for step in range(steps):
         if step % 100 == 0:
             saves.append((step, {n: o.field(n) for n in ("Ez", "Er", "Jz", "Jr", "Bphi", "Bphi2")}))
-        rows, lost = o.step(screens=screens, compat=False)
+        rows, lost = o.step(screens=screens, compat=False, j_filter=0.2 if options else 0.0)
         for s in range(2):
             rows_by_screen[s].append((step, rows[rows["screen"] == s]))
         lost_pages.append(lost)
