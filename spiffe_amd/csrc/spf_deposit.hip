@@ -677,27 +677,42 @@ __global__ __launch_bounds__(64 * RW) void key_reduce_kernel(KeyArgs a) { key_re
 // exclusive scan of the reduce workgroups' partial counts (one workgroup);
 // the total goes to *total
 __device__ inline void partial_scan_body(const int32_t *cnt, int64_t n, int64_t *off, int64_t *total) {
+  // 1024 threads; chunks of 8192 counts, thread t taking counts 8t .. 8t+7 of
+  // a chunk (their eight loads issued together), one block scan per chunk
+  constexpr int V = 8;
   __shared__ int64_t ws[16];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t per = (n + 1023) / 1024;
-  const int64_t t0 = std::min<int64_t>((int64_t)threadIdx.x * per, n), t1 = std::min<int64_t>(t0 + per, n);
-  int64_t s = 0;
-  for (int64_t t = t0; t < t1; t++) s += cnt[t];
-  int64_t incl = s;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int64_t u = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += u;
+  int64_t carry = 0;
+  for (int64_t c0 = 0; c0 < n; c0 += 1024 * V) {
+    const int64_t t0 = c0 + (int64_t)threadIdx.x * V;
+    int32_t v[V];
+#pragma unroll
+    for (int k = 0; k < V; k++) v[k] = t0 + k < n ? cnt[t0 + k] : 0;
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < V; k++) s += v[k];
+    int64_t incl = s;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int64_t u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
+    }
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    int64_t before = carry, chunk = 0;
+    for (int k = 0; k < 16; k++) {
+      if (k < w) before += ws[k];
+      chunk += ws[k];
+    }
+    __syncthreads();   // ws is rewritten by the next chunk
+    int64_t run = before + incl - s;
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+      if (t0 + k < n) off[t0 + k] = run;
+      run += v[k];
+    }
+    carry += chunk;
   }
-  if (lane == 63) ws[w] = incl;
-  __syncthreads();
-  int64_t before = 0;
-  for (int k = 0; k < w; k++) before += ws[k];
-  int64_t run = before + incl - s;
-  for (int64_t t = t0; t < t1; t++) {
-    off[t] = run;
-    run += cnt[t];
-  }
-  if (threadIdx.x == 1023) *total = run;
+  if (threadIdx.x == 0) *total = carry;
 }
 
 __global__ __launch_bounds__(1024) void partial_scan_kernel(const int32_t *cnt, int64_t n, int64_t *off,
