@@ -41,7 +41,7 @@ B_SOR = 22.0             # SOR point of one half-sweep: Psi rhs flag read, half 
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE on each workload, scaled by the
 # counters' reading for a plain 512 MiB device copy (profiles/*_hbm_traffic.md); None where not measured
 TRAFFIC = {
-    "hires": {"sor_batch_kernel": 193.20e6, "fdtd_kernel": 1087.06e6, "push phase": 3016.54e6},
+    "hires": {"sor_batch_kernel": 195.20e6, "fdtd_kernel": 1086.81e6, "push phase": 3016.52e6},
     "gun": {"sor_batch_kernel": 32.56e6, "fdtd_kernel": 66.03e6, "push phase": 188.26e6},
 }
 TIMER_EVERY = 2
